@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X FBP engine on the batched FP32 workload:
+256 slices of 2048 px x 1800 angles over 180 deg, seeded 20-ellipse phantoms
+(semi-axes x4), Poisson noise at 1e5 photons/ray, ABS contrast, HANN filter
+padded to 4096, centre offset +3.37 px, output 2048^2 x 256 float32, run as
+one hipGraph-captured launch sequence (min/max -> filter -> backprojection).
+
+One step = one reconstruction of the whole stack with the inputs already in
+HBM. Prints one JSON line (see DESIGN.md for the roofline accounting).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--slices S] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s
+HBM_PEAK_GBS = 8000.0
+
+
+def in_mask_fraction(w: int, c: float) -> float:
+    import numpy as np
+    hp = w // 2
+    i = np.arange(w, dtype=np.float32)[None, :]
+    j = np.arange(w, dtype=np.float32)[:, None]
+    ac = abs(c)
+    d2 = (j - hp - c) ** 2 + (i - hp) ** 2
+    up, dn = hp + c, hp - c
+    masked = (i <= ac) | (i >= w - ac - 1) | ((j <= up) & (d2 >= up * up)) | ((j >= up) & (d2 >= dn * dn))
+    return float(1.0 - masked.mean())
+
+
+def make_inputs(torch, ca, S, N, w, c, seed0=1000):
+    d_sino = torch.empty((S, N, w), dtype=torch.float32, device="cuda")
+    for s in range(S):
+        ell = ca.phantom_ellipses(20, w, 4.0, seed0 + s)
+        ca.phantom_sino_dev(d_sino[s], ell, c=c, N0=1e5, seed=seed0 + s)
+    torch.cuda.synchronize()
+    return d_sino
+
+
+def time_bp_kernel(torch, rec, d_sino, d_out, centers, stream, reps=3):
+    """Average duration of the backprojection launch alone (HIP events on the
+    stream the kernel runs on)."""
+    rec.prepare_dev(d_sino, stream)
+    rec.backproject_dev(d_out, centers, stream)  # warm
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        rec.backproject_dev(d_out, centers, stream)
+    ev1.record(stream)
+    ev1.synchronize()
+    return ev0.elapsed_time(ev1) / reps
+
+
+def time_prepare_kernel(torch, rec, d_sino, stream, reps=3):
+    rec.prepare_dev(d_sino, stream)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        rec.prepare_dev(d_sino, stream)
+    ev1.record(stream)
+    ev1.synchronize()
+    return ev0.elapsed_time(ev1) / reps
+
+
+def cpu_baseline(d_sino, w, N, c, threads):
+    """The oracle's -O3 -ffast-math build (the reference's flags), one slice
+    per thread as CTAS's RecInThread does, on 2 x threads slices."""
+    import numpy as np
+    import oracle
+    S = min(d_sino.shape[0], 2 * threads)
+    host = d_sino[:S].cpu().numpy()
+    t0 = time.perf_counter()
+    oracle.fbp_batch(host, [c] * S, oracle.ABS, 5, 0.0, 180.0, threads=threads, fast=True)
+    dt = time.perf_counter() - t0
+    return {"value": S * w * w * N / dt / 1e9, "unit": "Gvoxel-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{S} slices of {w}x{N} (ABS, HANN, c={c}), {threads} threads, "
+                      f"{dt:.1f} s, oracle/liboracle_fast.so (-O3 -ffast-math)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--slices", type=int, default=256)
+    ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--angles", type=int, default=1800)
+    ap.add_argument("--center", type=float, default=3.37)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); slices are "
+              "independent and would shard without collectives, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import ctas_amd as ca
+
+    if not torch.cuda.is_available():
+        print("no GPU", file=sys.stderr)
+        return 1
+    S, w, N, c = args.slices, args.width, args.angles, args.center
+    print(f"[bench] generating {S} x {N} x {w} phantoms", file=sys.stderr, flush=True)
+    d_sino = make_inputs(torch, ca, S, N, w, c)
+    d_out = torch.empty((S, w, w), dtype=torch.float32, device="cuda")
+    centers = [c] * S
+    stream = torch.cuda.Stream()
+    updates = S * w * w * N
+
+    def run_variant(store, interp, steps, warmup, autotune=True):
+        rec = ca.CTrec(w, N, ca.ABS, "HANN", interp=interp, store=store)
+        if autotune:
+            rec.autotune(min(S, 8))
+        rec.graph_capture(d_sino, d_out, centers, stream)
+        for _ in range(warmup):
+            rec.graph_launch(stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rec.graph_launch(stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = rec.status(S)
+        assert (st == 0).all(), "deAbs reported bad data"
+        bp_ms = time_bp_kernel(torch, rec, d_sino, d_out, centers, stream)
+        prep_ms = time_prepare_kernel(torch, rec, d_sino, stream)
+        return rec, dt / steps * 1e3, bp_ms, prep_ms
+
+    print("[bench] fp32 CTAS", file=sys.stderr, flush=True)
+    rec, ms, bp_ms, prep_ms = run_variant(ca.STORE_FP32, ca.INTERP_CTAS, args.steps, args.warmup)
+    kind, ac = rec.tiles()
+    value = updates / (ms / 1e3) / 1e9
+    lds_bytes = updates * 4.0  # one 4-byte sample per voxel-update (CTAS mode, FP32)
+    achieved = lds_bytes / (bp_ms / 1e3) / 1e9
+    frac_in = in_mask_fraction(w, c)
+    result = {
+        "metric": "Gvoxel-updates/s FBP 2048^2x1800 FP32 vs FP16; axis-find ms CNN vs sweep",
+        "value": round(value, 2),
+        "unit": "Gvoxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipse phantoms, Poisson 1e5)",
+        "config": {"workload": f"batched FBP {S} slices x {w} px x {N} angles, HANN, ABS, c=+{c}, "
+                               "CTAS floor sampling, one hipGraph",
+                   "slices": S, "width": w, "angles": N, "filter": "HANN", "tile_kind": kind,
+                   "angles_per_pass": ac, "in_mask_fraction": round(frac_in, 4)},
+        "roofline": {"bound": "lds", "achieved": round(achieved, 1), "peak": LDS_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / LDS_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "bp_grid_kernel", "kernel_ms": round(bp_ms, 3),
+                     "bytes_per_update": 4, "note": "LDS sample bytes per full-grid voxel-update"},
+        "stages_ms": {"prepare": round(prep_ms, 3), "backproject": round(bp_ms, 3)},
+    }
+    if not args.no_variants:
+        var = {}
+        for name, store in (("fp16", ca.STORE_FP16), ("bf16", ca.STORE_BF16)):
+            print(f"[bench] {name} CTAS", file=sys.stderr, flush=True)
+            _, vms, vbp, _ = run_variant(store, ca.INTERP_CTAS, max(2, args.steps // 2), 1)
+            var[name] = {"value": round(updates / (vms / 1e3) / 1e9, 2), "ms_per_step": round(vms, 3),
+                         "bp_kernel_ms": round(vbp, 3), "speedup_vs_fp32": round(ms / vms, 3)}
+        print("[bench] fp32 LINEAR", file=sys.stderr, flush=True)
+        _, lms, lbp, _ = run_variant(ca.STORE_FP32, ca.INTERP_LINEAR, max(2, args.steps // 2), 1)
+        var["fp32_linear"] = {"value": round(updates / (lms / 1e3) / 1e9, 2), "ms_per_step": round(lms, 3),
+                              "bp_kernel_ms": round(lbp, 3)}
+        result["variants"] = var
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        result["cpu_baseline"] = cpu_baseline(d_sino, w, N, c, threads)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

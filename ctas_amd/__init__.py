@@ -1,0 +1,358 @@
+"""ctas_amd - MI355X filtered-backprojection engine for CTAS.
+
+Python view of the C ABI in ``include/ctas_hip.h`` (``libctas_hip.so``), used by
+the tests and the benchmark. The engine itself is HIP/C++; this module only
+loads it with ctypes and moves numpy arrays or torch device tensors across the
+boundary. It has no compute path of its own: if the library cannot be loaded
+every call raises.
+
+Names follow the reference API (``CTrec``, ``Filter``, ``Contrast``, ``raxis``
+in CTAS ``src/common/ct.h``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libctas_hip.so"
+
+CTAS_OK = 0
+CTAS_E_ARG, CTAS_E_SHAPE, CTAS_E_CENTER, CTAS_E_DATA = -1, -2, -3, -4
+CTAS_E_DEVICE, CTAS_E_NOMEM, CTAS_E_STATE, CTAS_E_AXIS = -5, -6, -7, -8
+
+ABS, REF, PHS, FLT = 0, 1, 2, 3
+FILTERS = ["NONE", "RAMP", "BARLETT", "WELCH", "PARZEN", "HANN", "HAMMING", "BLACKMAN",
+           "LANCKZOS", "KAISER", "GAUSS"]
+INTERP_CTAS, INTERP_LINEAR = 0x0, 0x1
+STORE_FP32, STORE_FP16, STORE_BF16 = 0x00, 0x10, 0x20
+TILE_AUTO, TILE_64x64, TILE_128x64, TILE_128x128 = 0, 1, 2, 3
+
+
+class CtasError(RuntimeError):
+    """Raised for a negative return code; ``code`` holds it."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def _f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def lib() -> C.CDLL:
+    """Load libctas_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise CtasError(CTAS_E_DEVICE, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i, f, d = C.c_void_p, C.c_int, C.c_float, C.c_double
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int)
+    sig = {
+        "ctas_last_error": (C.c_char_p, []),
+        "ctas_version": (C.c_char_p, []),
+        "ctas_device_count": (i, [ip]),
+        "ctas_zidth": (i, [i]),
+        "ctas_arc_rad": (f, [f, i]),
+        "ctas_step_rad": (f, [f, i]),
+        "ctas_filter_window": (i, [i, f, i, i, fp]),
+        "ctas_filter_parse": (i, [C.c_char_p, ip, fp]),
+        "ctas_contrast_parse": (i, [C.c_char_p, ip]),
+        "ctas_angles": (i, [i, f, fp]),
+        "ctas_fbp_create": (i, [C.POINTER(vp), i, i, i, i, f, f, i]),
+        "ctas_fbp_clone": (i, [vp, C.POINTER(vp)]),
+        "ctas_fbp_destroy": (i, [vp]),
+        "ctas_fbp_info": (i, [vp, ip, ip, ip, ip]),
+        "ctas_fbp_set_physics": (i, [vp, f, f]),
+        "ctas_fbp_get_reccof": (i, [vp, fp]),
+        "ctas_fbp_set_angles": (i, [vp, fp]),
+        "ctas_fbp_get_angles": (i, [vp, fp]),
+        "ctas_fbp_get_window": (i, [vp, fp]),
+        "ctas_fbp_prepare": (i, [vp, fp, i]),
+        "ctas_fbp_sino": (i, [vp, fp]),
+        "ctas_fbp_repeat": (i, [vp, fp, fp, i]),
+        "ctas_fbp_reconstruct": (i, [vp, fp, fp, fp, i]),
+        "ctas_fbp_reconstruct_dev": (i, [vp, vp, vp, fp, i, vp]),
+        "ctas_fbp_repeat_dev": (i, [vp, i, vp, fp, i, vp]),
+        "ctas_fbp_prepare_dev": (i, [vp, vp, i, vp]),
+        "ctas_fbp_backproject_dev": (i, [vp, vp, fp, i, vp]),
+        "ctas_fbp_graph_capture": (i, [vp, vp, vp, fp, i, vp]),
+        "ctas_fbp_graph_launch": (i, [vp, vp]),
+        "ctas_fbp_status": (i, [vp, i, ip]),
+        "ctas_fbp_set_tiles": (i, [vp, i, i]),
+        "ctas_fbp_get_tiles": (i, [vp, ip, ip]),
+        "ctas_fbp_autotune": (i, [vp, i]),
+        "ctas_fbp_batch_uneven": (i, [vp, i, i, i, f, f, i, vp]),
+        "ctas_ring_filter": (i, [fp, i, i, i]),
+        "ctas_ring_filter_dev": (i, [vp, i, i, i, i, vp]),
+        "ctas_axis_scores": (i, [vp, fp, i, fp]),
+        "ctas_raxis": (i, [fp, i, i, f, f, i, fp]),
+        "ctas_raxis_pair": (i, [fp, fp, i, i, f, fp]),
+        "ctas_phantom_ellipses": (i, [i, i, d, C.c_uint64, C.POINTER(d)]),
+        "ctas_phantom_sino_dev": (i, [i, i, d, i, C.POINTER(d), d, C.c_uint64, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc < 0:
+        raise CtasError(rc, lib().ctas_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().ctas_device_count(C.byref(n)))
+    return n.value
+
+
+def zidth(w: int) -> int:
+    return lib().ctas_zidth(w)
+
+
+def filter_window(filt: int, param: float, w: int, contrast: int) -> np.ndarray:
+    W = np.zeros(zidth(w), np.float32)
+    _check(lib().ctas_filter_window(filt, param, w, contrast, _f32p(W)))
+    return W
+
+
+def parse_filter(spec: str) -> tuple[int, float]:
+    f, p = C.c_int(0), C.c_float(0)
+    _check(lib().ctas_filter_parse(spec.encode(), C.byref(f), C.byref(p)))
+    return f.value, p.value
+
+
+def parse_contrast(name: str) -> int:
+    c = C.c_int(0)
+    _check(lib().ctas_contrast_parse(name.encode(), C.byref(c)))
+    return c.value
+
+
+def angles(n: int, anarc: float = 180.0) -> np.ndarray:
+    sc = np.zeros((n, 2), np.float32)
+    _check(lib().ctas_angles(n, anarc, _f32p(sc)))
+    return sc
+
+
+class _Item(C.Structure):
+    _fields_ = [("w", C.c_int), ("n_angles", C.c_int), ("center", C.c_float),
+                ("d_sino", C.c_void_p), ("d_out", C.c_void_p)]
+
+
+def _ptr(t) -> int:
+    """Device address of a torch tensor (contiguous float32)."""
+    assert t.is_contiguous()
+    return t.data_ptr()
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+class CTrec:
+    """Reconstruction handle; mirror of CTrec (ct.h:235-298) over the C ABI."""
+
+    def __init__(self, w: int, n_angles: int, contrast: int = ABS, filt: int | str = 1,
+                 param: float = 0.0, anarc: float = 180.0, interp: int = INTERP_CTAS,
+                 store: int = STORE_FP32, _handle=None):
+        if isinstance(filt, str):
+            filt, param = parse_filter(filt)
+        self.w, self.n = w, n_angles
+        self._h = C.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            _check(lib().ctas_fbp_create(C.byref(self._h), w, n_angles, contrast, filt, param,
+                                         anarc, interp | store))
+
+    def clone(self) -> "CTrec":
+        h = C.c_void_p()
+        _check(lib().ctas_fbp_clone(self._h, C.byref(h)))
+        return CTrec(self.w, self.n, _handle=h)
+
+    def close(self) -> None:
+        if self._h:
+            lib().ctas_fbp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def zidth(self) -> int:
+        z = C.c_int()
+        _check(lib().ctas_fbp_info(self._h, None, None, C.byref(z), None))
+        return z.value
+
+    def window(self) -> np.ndarray:
+        W = np.zeros(self.zidth, np.float32)
+        _check(lib().ctas_fbp_get_window(self._h, _f32p(W)))
+        return W
+
+    def angles(self) -> np.ndarray:
+        sc = np.zeros((self.n, 2), np.float32)
+        _check(lib().ctas_fbp_get_angles(self._h, _f32p(sc)))
+        return sc
+
+    def set_angles(self, sincos: np.ndarray) -> None:
+        sc = np.ascontiguousarray(sincos, np.float32)
+        _check(lib().ctas_fbp_set_angles(self._h, _f32p(sc)))
+
+    def set_physics(self, pixel_size: float = 0.0, wavelength: float = 0.0) -> None:
+        _check(lib().ctas_fbp_set_physics(self._h, pixel_size, wavelength))
+
+    def rec_cof(self) -> float:
+        r = C.c_float()
+        _check(lib().ctas_fbp_get_reccof(self._h, C.byref(r)))
+        return r.value
+
+    def set_tiles(self, kind: int, angles_per_pass: int = 0) -> None:
+        _check(lib().ctas_fbp_set_tiles(self._h, kind, angles_per_pass))
+
+    def tiles(self) -> tuple[int, int]:
+        k, a = C.c_int(), C.c_int()
+        _check(lib().ctas_fbp_get_tiles(self._h, C.byref(k), C.byref(a)))
+        return k.value, a.value
+
+    def autotune(self, slices: int = 4) -> None:
+        _check(lib().ctas_fbp_autotune(self._h, slices))
+
+    # host-array API (numpy float32, filtered in place like the reference)
+    def prepare(self, sino: np.ndarray) -> np.ndarray:
+        """Filtered copy of the sinogram(s); the input array is not modified."""
+        s = np.array(sino, np.float32, copy=True, order="C")
+        S = 1 if s.ndim == 2 else s.shape[0]
+        _check(lib().ctas_fbp_prepare(self._h, _f32p(s), S))
+        return s
+
+    def sino(self, sino: np.ndarray) -> np.ndarray:
+        """Keep the filtered sinogram on the device; returns a filtered copy."""
+        s = np.array(sino, np.float32, copy=True, order="C")
+        _check(lib().ctas_fbp_sino(self._h, _f32p(s)))
+        return s
+
+    def repeat(self, centers) -> np.ndarray:
+        c = np.atleast_1d(np.asarray(centers, np.float32))
+        out = np.zeros((c.size, self.w, self.w), np.float32)
+        _check(lib().ctas_fbp_repeat(self._h, _f32p(out), _f32p(c), c.size))
+        return out
+
+    def reconstruct(self, sino: np.ndarray, centers) -> tuple[np.ndarray, np.ndarray]:
+        """Returns (filtered sinogram, slices); the input array is not modified."""
+        s = np.array(sino, np.float32, copy=True, order="C")
+        S = 1 if s.ndim == 2 else s.shape[0]
+        c = np.broadcast_to(np.asarray(centers, np.float32), (S,)).copy()
+        out = np.zeros((S, self.w, self.w), np.float32)
+        _check(lib().ctas_fbp_reconstruct(self._h, _f32p(s), _f32p(out), _f32p(c), S))
+        return s, out
+
+    # device API (torch tensors on the GPU)
+    def reconstruct_dev(self, d_sino, d_out, centers, stream=None) -> None:
+        S = d_sino.shape[0] if d_sino.dim() == 3 else 1
+        c = np.broadcast_to(np.asarray(centers, np.float32), (S,)).copy()
+        _check(lib().ctas_fbp_reconstruct_dev(self._h, _ptr(d_sino), _ptr(d_out), _f32p(c), S,
+                                              _stream_ptr(stream)))
+
+    def prepare_dev(self, d_sino, stream=None) -> None:
+        S = d_sino.shape[0] if d_sino.dim() == 3 else 1
+        _check(lib().ctas_fbp_prepare_dev(self._h, _ptr(d_sino), S, _stream_ptr(stream)))
+
+    def backproject_dev(self, d_out, centers, stream=None) -> None:
+        S = d_out.shape[0] if d_out.dim() == 3 else 1
+        c = np.broadcast_to(np.asarray(centers, np.float32), (S,)).copy()
+        _check(lib().ctas_fbp_backproject_dev(self._h, _ptr(d_out), _f32p(c), S, _stream_ptr(stream)))
+
+    def repeat_dev(self, d_out, centers, slice_index: int = 0, stream=None) -> None:
+        c = np.atleast_1d(np.asarray(centers, np.float32))
+        _check(lib().ctas_fbp_repeat_dev(self._h, slice_index, _ptr(d_out), _f32p(c), c.size,
+                                         _stream_ptr(stream)))
+
+    def graph_capture(self, d_sino, d_out, centers, stream=None) -> None:
+        S = d_sino.shape[0] if d_sino.dim() == 3 else 1
+        c = np.broadcast_to(np.asarray(centers, np.float32), (S,)).copy()
+        _check(lib().ctas_fbp_graph_capture(self._h, _ptr(d_sino), _ptr(d_out), _f32p(c), S,
+                                            _stream_ptr(stream)))
+
+    def graph_launch(self, stream=None) -> None:
+        _check(lib().ctas_fbp_graph_launch(self._h, _stream_ptr(stream)))
+
+    def status(self, S: int) -> np.ndarray:
+        st = np.zeros(S, np.int32)
+        _check(lib().ctas_fbp_status(self._h, S, st.ctypes.data_as(C.POINTER(C.c_int))))
+        return st
+
+    def axis_scores(self, axis_positions) -> np.ndarray:
+        a = np.atleast_1d(np.asarray(axis_positions, np.float32))
+        sc = np.zeros(a.size, np.float32)
+        _check(lib().ctas_axis_scores(self._h, _f32p(a), a.size, _f32p(sc)))
+        return sc
+
+
+def batch_uneven(items, contrast=ABS, filt=1, param=0.0, anarc=180.0, flags=0, stream=None):
+    """items: list of (d_sino, d_out, center) torch tensors [N][w] / [w][w]."""
+    arr = (_Item * len(items))()
+    for k, (s, o, c) in enumerate(items):
+        arr[k] = _Item(s.shape[1], s.shape[0], float(c), _ptr(s), _ptr(o))
+    _check(lib().ctas_fbp_batch_uneven(arr, len(items), contrast, filt, param, anarc, flags,
+                                       _stream_ptr(stream)))
+
+
+def ring_filter(sino: np.ndarray, box: int) -> np.ndarray:
+    s = np.array(sino, np.float32, copy=True, order="C")
+    _check(lib().ctas_ring_filter(_f32p(s), s.shape[0], s.shape[1], box))
+    return s
+
+
+def ring_filter_dev(d_sino, box: int, stream=None) -> None:
+    S, N, w = (d_sino.shape if d_sino.dim() == 3 else (1,) + tuple(d_sino.shape))
+    _check(lib().ctas_ring_filter_dev(_ptr(d_sino), S, N, w, box, _stream_ptr(stream)))
+
+
+def raxis(sino: np.ndarray, anarc: float, max_dev: float = 0.25, algo: int = 0) -> float:
+    s = np.ascontiguousarray(sino, np.float32)
+    r = C.c_float()
+    _check(lib().ctas_raxis(_f32p(s), s.shape[0], s.shape[1], anarc, max_dev, algo, C.byref(r)))
+    return r.value
+
+
+def raxis_pair(p0: np.ndarray, p180: np.ndarray, max_dev: float = 0.25) -> float:
+    a = np.ascontiguousarray(p0, np.float32)
+    b = np.ascontiguousarray(p180, np.float32)
+    r = C.c_float()
+    _check(lib().ctas_raxis_pair(_f32p(a), _f32p(b), a.shape[0], a.shape[1], max_dev, C.byref(r)))
+    return r.value
+
+
+def phantom_ellipses(n_ell: int, w: int, axis_scale: float = 1.0, seed: int = 0) -> np.ndarray:
+    ell = np.zeros((max(n_ell, 1), 6), np.float64)
+    _check(lib().ctas_phantom_ellipses(n_ell, w, axis_scale, seed,
+                                       ell.ctypes.data_as(C.POINTER(C.c_double))))
+    return ell[:n_ell]
+
+
+def phantom_sino_dev(d_out, ell: np.ndarray, c: float = 0.0, N0: float = 1e5, seed: int = 0,
+                     d_gain=None, stream=None) -> None:
+    N, w = d_out.shape[-2], d_out.shape[-1]
+    e = np.ascontiguousarray(ell, np.float64)
+    _check(lib().ctas_phantom_sino_dev(N, w, c, e.shape[0], e.ctypes.data_as(C.POINTER(C.c_double)),
+                                       N0, seed, None if d_gain is None else _ptr(d_gain),
+                                       _ptr(d_out), _stream_ptr(stream)))

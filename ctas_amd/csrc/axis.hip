@@ -1,0 +1,474 @@
+// Rotation-axis search (raxis, lct.cpp:1133-1326) with the trial
+// reconstructions and their sharpness scores (AverageGradient::apply,
+// lct.cpp:872-915) evaluated on the GPU in batches of centres.
+//
+// Score of one trial slice: the one-sided Gaussian smoothing of
+// lct.cpp:837-843 / 878-884 is the circular convolution with
+// g[u]*g[v], g[n] = exp(-n^2 / (2 sigma^2)), evaluated separably with the
+// taps above 1e-9 (a row pass through LDS, then a column pass); then the
+// rows [r, w-r) with r = ceil|c| are Sobel-filtered into the matching rows
+// of zeroed full-size derivative arrays, and the magnitude is read from the
+// top w-2r rows of those arrays (the shape mismatch of lct.cpp:905), the
+// border of that block is zeroed and the mean of its positive values is the
+// score. The block sums are reduced in a fixed order (deterministic).
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ctas_internal.hpp"
+
+namespace ctas {
+namespace {
+
+constexpr int kMaxTaps = 64;
+
+struct Taps {
+  float g[kMaxTaps];
+  int nt;
+};
+
+Taps make_taps(float sigma) {
+  Taps t{};
+  const float sig22 = 2 * sigma * sigma;
+  t.nt = 0;
+  for (int n = 0; n < kMaxTaps; ++n) {
+    const float v = std::exp(-static_cast<float>(n * n) / sig22);
+    t.g[n] = v;
+    if (v > 1e-9f) t.nt = n + 1;
+  }
+  return t;
+}
+
+// Row pass: dst[u][v] = sum_n g[n] src[u][(v - n) mod w]. Grid (w, n).
+__global__ __launch_bounds__(256) void smooth_rows_kernel(const float *__restrict__ src, float *__restrict__ dst,
+                                                          int w, Taps taps) {
+  extern __shared__ float row[];
+  const size_t base = (static_cast<size_t>(blockIdx.y) * w + blockIdx.x) * w;
+  for (int v = threadIdx.x; v < w; v += blockDim.x) row[v] = src[base + v];
+  __syncthreads();
+  for (int v = threadIdx.x; v < w; v += blockDim.x) {
+    float a = 0.0f;
+    for (int n = 0; n < taps.nt; ++n) {
+      int idx = (v - n) % w;
+      if (idx < 0) idx += w;
+      a = a + taps.g[n] * row[idx];
+    }
+    dst[base + v] = a;
+  }
+}
+
+// Column pass: dst[u][v] = sum_m g[m] src[(u - m) mod w][v]. Grid (w, n).
+__global__ __launch_bounds__(256) void smooth_cols_kernel(const float *__restrict__ src, float *__restrict__ dst,
+                                                          int w, Taps taps) {
+  const int u = blockIdx.x;
+  const size_t sl = static_cast<size_t>(blockIdx.y) * w * w;
+  for (int v = threadIdx.x; v < w; v += blockDim.x) {
+    float a = 0.0f;
+    for (int m = 0; m < taps.nt; ++m) {
+      int r = (u - m) % w;
+      if (r < 0) r += w;
+      a = a + taps.g[m] * src[sl + static_cast<size_t>(r) * w + v];
+    }
+    dst[sl + static_cast<size_t>(u) * w + v] = a;
+  }
+}
+
+// Sobel derivative of the cropped slice cs (h x w) at (yy, x), accumulated
+// in the statement order of lct.cpp:887-902; (dy, dx, coefficient) terms.
+__device__ __forceinline__ float sobel_term(const float *cs, int h, int w, int yy, int x, int a, int b, float k,
+                                            float acc) {
+  const int sy = yy - a, sx = x - b;
+  if (sy >= 0 && sy < h && sx >= 0 && sx < w) acc = acc + k * cs[static_cast<size_t>(sy) * w + sx];
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void sobel_score_kernel(const float *__restrict__ slices,
+                                                          const float *__restrict__ centres, int w,
+                                                          double *__restrict__ partial) {
+  const int k = blockIdx.y;
+  const float c = centres[k];
+  const int r = static_cast<int>(ceilf(fabsf(c)));
+  const int h = w - 2 * r;
+  const float *cs = slices + static_cast<size_t>(k) * w * w + static_cast<size_t>(r) * w;
+  double sm = 0.0, cnt = 0.0;
+  if (h >= 3) {
+    const long long total = static_cast<long long>(h - 2) * (w - 2);
+    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+         e += static_cast<long long>(gridDim.x) * blockDim.x) {
+      const int y = 1 + static_cast<int>(e / (w - 2));
+      const int x = 1 + static_cast<int>(e % (w - 2));
+      // full-array row y holds cropped-derivative row y - r (zero above r)
+      const int yy = y - r;
+      if (yy < 0) continue;
+      float dH = 0.0f, dV = 0.0f;
+      dH = sobel_term(cs, h, w, yy, x, -1, -1, -1.0f, dH);
+      dH = sobel_term(cs, h, w, yy, x, -1, 0, -2.0f, dH);
+      dH = sobel_term(cs, h, w, yy, x, -1, 1, -1.0f, dH);
+      dH = sobel_term(cs, h, w, yy, x, 1, -1, 1.0f, dH);
+      dH = sobel_term(cs, h, w, yy, x, 1, 0, 2.0f, dH);
+      dH = sobel_term(cs, h, w, yy, x, 1, 1, 1.0f, dH);
+      dV = sobel_term(cs, h, w, yy, x, -1, -1, -1.0f, dV);
+      dV = sobel_term(cs, h, w, yy, x, 0, -1, -2.0f, dV);
+      dV = sobel_term(cs, h, w, yy, x, 1, -1, -1.0f, dV);
+      dV = sobel_term(cs, h, w, yy, x, -1, 1, 1.0f, dV);
+      dV = sobel_term(cs, h, w, yy, x, 0, 1, 2.0f, dV);
+      dV = sobel_term(cs, h, w, yy, x, 1, 1, 1.0f, dV);
+      const float m = sqrtf(dH * dH + dV * dV);
+      sm += m;
+      cnt += (m > 0.0f) ? 1.0 : 0.0;
+    }
+  }
+  __shared__ double ps[256], pc[256];
+  ps[threadIdx.x] = sm;
+  pc[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      ps[threadIdx.x] += ps[threadIdx.x + o];
+      pc[threadIdx.x] += pc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[(static_cast<size_t>(k) * kScoreBlocks + blockIdx.x) * 2] = ps[0];
+    partial[(static_cast<size_t>(k) * kScoreBlocks + blockIdx.x) * 2 + 1] = pc[0];
+  }
+}
+
+__global__ void score_finish_kernel(const double *__restrict__ partial, float *__restrict__ scores, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s = 0.0, c = 0.0;
+  for (int b = 0; b < kScoreBlocks; ++b) {
+    s += partial[(static_cast<size_t>(k) * kScoreBlocks + b) * 2];
+    c += partial[(static_cast<size_t>(k) * kScoreBlocks + b) * 2 + 1];
+  }
+  const float sm = static_cast<float>(s);
+  scores[k] = c > 0.0 ? sm / static_cast<float>(c) : 0.0f;
+}
+
+}  // namespace
+
+hipError_t launch_ag_scores(float *d_slices, int n, int w, const float *d_centres, float sigma, float *d_tmp,
+                            double *d_partial, float *d_scores, hipStream_t s) {
+  if (sigma > 1.0f) {
+    const Taps taps = make_taps(sigma);
+    hipLaunchKernelGGL(smooth_rows_kernel, dim3(w, n), dim3(256), sizeof(float) * w, s, d_slices, d_tmp, w, taps);
+    hipLaunchKernelGGL(smooth_cols_kernel, dim3(w, n), dim3(256), 0, s, d_tmp, d_slices, w, taps);
+  }
+  hipLaunchKernelGGL(sobel_score_kernel, dim3(kScoreBlocks, n), dim3(256), 0, s, d_slices, d_centres, w, d_partial);
+  hipLaunchKernelGGL(score_finish_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d_partial, d_scores, n);
+  return hipGetLastError();
+}
+
+namespace {
+
+template <typename T>
+int grow_buf(T *&ptr, size_t &cap, size_t bytes) {
+  if (bytes <= cap && ptr) return CTAS_OK;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  ptr = static_cast<T *>(p);
+  cap = bytes;
+  return CTAS_OK;
+}
+
+constexpr int kScoreChunk = 64;  // centres per batch (bounds the scratch slices)
+
+// improveMe (lct.cpp:1108-1129): bisection towards the lower neighbour.
+float improve_me(float eps, const std::vector<float> &vax, const std::vector<float> &err, int idx, int cdir,
+                 const std::function<float(float)> &f, int *rc) {
+  float aa = vax[idx - cdir];
+  float bb = vax[idx + cdir];
+  float rf = vax[idx];
+  float vl = err[idx];
+  while (std::fabs(aa - rf) > eps || std::fabs(rf - bb) > eps) {
+    const bool afar = std::fabs(aa - rf) > std::fabs(rf - bb);
+    float &lo = afar ? aa : bb;
+    float &hi = afar ? bb : aa;
+    const float x = (rf + lo) / 2;
+    const float vx = f(x);
+    if (*rc) return rf;
+    if (vl <= vx) {
+      lo = x;
+    } else {
+      vl = vx;
+      hi = rf;
+      rf = x;
+    }
+  }
+  return rf;
+}
+
+struct HandleGuard {
+  ctas_fbp *h = nullptr;
+  ~HandleGuard() {
+    if (h) ctas_fbp_destroy(h);
+  }
+};
+
+}  // namespace
+}  // namespace ctas
+
+using namespace ctas;
+
+extern "C" int ctas_axis_scores(ctas_fbp *h, const float *axisR, int n, float *scores) {
+  if (!h || !axisR || !scores || n <= 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (h->kept_slices < 1) return fail(CTAS_E_STATE, "no filtered sinogram on the device: call ctas_fbp_sino first");
+  const int w = h->w;
+  const float sigma = static_cast<float>(step_rad(h->anarc, h->N) * static_cast<float>(w) / M_PI);
+  const size_t ww = static_cast<size_t>(w) * w;
+  for (int k0 = 0; k0 < n; k0 += kScoreChunk) {
+    const int m = std::min(kScoreChunk, n - k0);
+    std::vector<float> centres(m);
+    for (int i = 0; i < m; ++i)
+      centres[i] = static_cast<float>(static_cast<double>(axisR[k0 + i]) - 0.5 * (w - 1));
+    if (int rc = grow_buf(h->d_out, h->out_cap, sizeof(float) * ww * m)) return rc;
+    if (int rc = grow_buf(h->d_tmp, h->score_cap, sizeof(float) * ww * m)) return rc;
+    size_t pcap = h->d_partial ? sizeof(double) * 2 * kScoreBlocks * kScoreChunk : 0;
+    if (int rc = grow_buf(h->d_partial, pcap, sizeof(double) * 2 * kScoreBlocks * kScoreChunk)) return rc;
+    if (!h->d_centres) {
+      if (hipMalloc(&h->d_centres, sizeof(float) * kScoreChunk) != hipSuccess ||
+          hipMalloc(&h->d_scores, sizeof(float) * kScoreChunk) != hipSuccess)
+        return fail(CTAS_E_NOMEM, "allocation failed");
+    }
+    CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
+    CTAS_HIP_TRY(hipMemcpy(h->d_centres, centres.data(), sizeof(float) * m, hipMemcpyHostToDevice));
+    if (int rc = ctas_fbp_repeat_dev(h, 0, h->d_out, centres.data(), m, h->stream)) return rc;
+    CTAS_HIP_TRY(launch_ag_scores(h->d_out, m, w, h->d_centres, sigma, h->d_tmp, h->d_partial, h->d_scores,
+                                  h->stream));
+    CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
+    CTAS_HIP_TRY(hipMemcpy(scores + k0, h->d_scores, sizeof(float) * m, hipMemcpyDeviceToHost));
+  }
+  return CTAS_OK;
+}
+
+// raxis(proj0, proj180, maxDev) (lct.cpp:1217-1294): registration of the
+// 0 and 180 degree projections; a few rows, evaluated on the host.
+extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int w, float maxDev, float *result) {
+  if (!p0 || !p180 || !result) return fail(CTAS_E_ARG, "null argument");
+  if (maxDev < 0 || maxDev > 0.5f)
+    return fail(CTAS_E_ARG, "Invalid maximum relative shift of the rotation axis " + std::to_string(maxDev) +
+                                ". Must be within [0, 0.5] range.");
+  if (rows <= 0 || w <= 0) return fail(CTAS_E_ARG, "Empty input projections.");
+  const float cent = static_cast<float>(0.5 * (w - 1));
+  std::vector<float> proj(w, 0.0f);
+  for (int r = 0; r < rows; ++r)
+    for (int x = 0; x < w; ++x)
+      proj[x] = static_cast<float>(proj[x] + (std::fabs(static_cast<double>(p0[static_cast<size_t>(r) * w + x]) - 1.0) +
+                                              std::fabs(static_cast<double>(p180[static_cast<size_t>(r) * w + x]) - 1.0)));
+  double ssum = 0.0, wsum = 0.0;
+  for (int x = 0; x < w; ++x) {
+    ssum += proj[x];
+    wsum += static_cast<double>(static_cast<float>(x) * proj[x]);
+  }
+  if (ssum == 0.0) return fail(CTAS_E_AXIS, "Impossible input projection averaging to zero.");
+  const float fcr = static_cast<float>(wsum) / static_cast<float>(ssum);
+  const int stpsz = 16;
+  const int steps = static_cast<int>(1 + 2 * (maxDev == 0.0f ? 1.0f : maxDev * static_cast<float>(w) / stpsz));
+  std::vector<float> p180r(static_cast<size_t>(rows) * w);
+  for (int r = 0; r < rows; ++r)
+    for (int x = 0; x < w; ++x) p180r[static_cast<size_t>(r) * w + x] = p180[static_cast<size_t>(r) * w + (w - 1 - x)];
+  double a0 = 0.0, a1 = 0.0;
+  for (int r = 0; r < rows; ++r) {
+    a0 += p180r[static_cast<size_t>(r) * w];
+    a1 += p180r[static_cast<size_t>(r) * w + w - 1];
+  }
+  const float aver = (static_cast<float>(a0) + static_cast<float>(a1)) / static_cast<float>(2 * rows);
+  auto calc_err = [&](float x) -> float {
+    const long sh = static_cast<long>(x);
+    double acc = 0.0;
+    for (int r = 0; r < rows; ++r)
+      for (int col = 0; col < w; ++col) {
+        float d = aver;
+        const long src = col - sh;
+        if (col >= std::max(0L, sh) && col <= w - 1 + std::min(0L, sh) && src >= 0 && src < w)
+          d = p180r[static_cast<size_t>(r) * w + src];
+        d = d - p0[static_cast<size_t>(r) * w + col];
+        acc += static_cast<double>(d * d);
+      }
+    return std::sqrt(static_cast<float>(acc));
+  };
+  std::vector<float> vax(steps), err(steps);
+  for (int ii = 0; ii < steps; ++ii) {
+    vax[ii] = 2 * (fcr - cent) + static_cast<float>(stpsz * (ii - (steps - 1) / 2));
+    err[ii] = calc_err(vax[ii]);
+  }
+  int idx = -1;
+  float minval = FLT_MAX;
+  for (int i = 1; i < steps - 1; ++i)
+    if (err[i] < err[i - 1] && err[i] < err[i + 1] && err[i] < minval) minval = err[idx = i];
+  int rc = CTAS_OK;
+  if (idx >= 0) {
+    *result = cent + 0.5f * improve_me(1.0f, vax, err, idx, 1, calc_err, &rc);
+    return rc;
+  }
+  idx = err[0] < err[steps - 1] ? 0 : steps - 1;
+  const int cdir = idx == 0 ? -1 : 1;
+  minval = err[idx];
+  const int maxSteps = static_cast<int>((static_cast<float>(w / 2) - static_cast<float>(cdir) * vax[idx]) / stpsz);
+  if (maxSteps <= 0) return fail(CTAS_E_AXIS, "Failed to calculate rotation centre.");
+  std::vector<float> v2(maxSteps + 2), e2(maxSteps + 2);
+  v2[0] = vax[idx - cdir];
+  v2[1] = vax[idx];
+  e2[0] = err[idx - cdir];
+  e2[1] = err[idx];
+  int idx2 = 1;
+  for (int i = 0; i < maxSteps; ++i) {
+    v2[i + 2] = vax[idx] + static_cast<float>(cdir * (i + 1) * stpsz);
+    e2[i + 2] = calc_err(v2[i + 2]);
+    if (e2[i + 2] < minval) {
+      minval = e2[i + 2];
+      idx2 = i + 2;
+    } else {
+      break;
+    }
+  }
+  if (idx2 == maxSteps + 1) return fail(CTAS_E_AXIS, "Failed to calculate rotation centre.");
+  *result = cent + 0.5f * improve_me(1.0f, v2, e2, idx2, cdir, calc_err, &rc);
+  return rc;
+}
+
+// raxis(sino, anarc, maxDev, algo) (lct.cpp:1133-1212). The trial
+// reconstructions of the sweep run as one batch; the directional walk is
+// evaluated in batches and cut at its first non-improving trial, which is
+// the sequential order of AGinThread::exexafter2 (lct.cpp:1028-1040); the
+// bisection is sequential.
+extern "C" int ctas_raxis(const float *sino, int N, int w, float anarc, float maxDev, int algo, float *result) {
+  if (!sino || !result) return fail(CTAS_E_ARG, "null argument");
+  if (N < 3 || w < 3)
+    return fail(CTAS_E_AXIS, "Shape of input sinogram (" + std::to_string(N) + "," + std::to_string(w) +
+                                 ") is less than minimal (3,3).");
+  const float arc = arc_rad(anarc, N);
+  if (arc == 0.0f) return fail(CTAS_E_AXIS, "Zero sinogram arc.");
+  if (std::fabs(arc) < static_cast<float>(M_PI))
+    return fail(CTAS_E_AXIS, "Sinogram arc " + std::to_string(std::fabs(arc) * 180 / M_PI) + " is less than 180 degree.");
+  const float maxDevLM = 0.25f;
+  if (maxDev < 0 || maxDev > maxDevLM)
+    return fail(CTAS_E_AXIS, "Invalid maximum relative shift of the rotation axis " + std::to_string(maxDev) + ".");
+  const float cent = static_cast<float>(0.5 * (w - 1));
+  float fcr = 0.0f;
+  if (algo == 0 || algo == 2) {
+    const int idx180 = static_cast<int>((N - 1) * M_PI / std::fabs(arc_rad(anarc, N)));
+    const int nofln = N - idx180;
+    if (nofln <= 0) return fail(CTAS_E_AXIS, "No opposite projections in the sinogram.");
+    std::vector<float> r0(static_cast<size_t>(nofln) * w), r180(static_cast<size_t>(nofln) * w);
+    for (int j = 0; j < nofln; ++j)
+      for (int x = 0; x < w; ++x) {
+        r0[static_cast<size_t>(j) * w + x] = sino[static_cast<size_t>(j) * w + x] + 1;
+        r180[static_cast<size_t>(j) * w + x] = sino[static_cast<size_t>(j + idx180) * w + x] + 1;
+      }
+    float q = 0.0f;
+    if (int rc = ctas_raxis_pair(r0.data(), r180.data(), nofln, w, std::min(maxDev, 0.5f), &q)) return rc;
+    if (algo == 0) {
+      *result = q;
+      return CTAS_OK;
+    }
+    fcr = q;
+  } else if (algo == 1) {
+    fcr = cent;
+  } else if (algo == 3) {
+    std::vector<float> pl(w, 0.0f);
+    for (int j = 0; j < N; ++j)
+      for (int x = 0; x < w; ++x) pl[x] = pl[x] + sino[static_cast<size_t>(j) * w + x];
+    const float mx = *std::max_element(pl.begin(), pl.end());
+    const float mn = *std::min_element(pl.begin(), pl.end());
+    // lct.cpp:1168-1180 only ends its scans at a sample above the
+    // threshold; a constant projection has none.
+    if (!(mx > mn)) return fail(CTAS_E_AXIS, "Constant projection: no edge above the threshold.");
+    const float thr = static_cast<float>(0.1 * mx + 0.9 * mn);
+    int a = 0, b = w - 1;
+    while (!(pl[a] > thr)) ++a;
+    while (!(pl[b] > thr)) --b;
+    fcr = (static_cast<float>(a) + static_cast<float>(b)) / 2.0f;
+  } else {
+    return fail(CTAS_E_AXIS, "Unknown algorithm for rotation axis " + std::to_string(algo) + ".");
+  }
+  if (std::fabs(fcr - cent) / w > maxDev)
+    return fail(CTAS_E_AXIS, "Rotation axis relative shift " + std::to_string(std::fabs(fcr - cent) / w) +
+                                 " is larger than maximum shift " + std::to_string(maxDev) + ".");
+  const float rbxsz = static_cast<float>(1.0 - 2.0 * (maxDev + std::fabs(fcr - cent) / w));
+  if (rbxsz < 0.5f)
+    return fail(CTAS_E_AXIS, "Relative box size " + std::to_string(rbxsz) + " is less than miminum window of 0.5.");
+
+  // AverageGradient (lct.cpp:818-844): ABS contrast, RAMP, the sinogram
+  // filtered once.
+  HandleGuard g;
+  if (int rc = ctas_fbp_create(&g.h, w, N, CTAS_CONTRAST_ABS, CTAS_FILTER_RAMP, 0.0f, anarc, 0)) return rc;
+  std::vector<float> copy(sino, sino + static_cast<size_t>(N) * w);
+  if (int rc = ctas_fbp_sino(g.h, copy.data())) return rc;
+  int rc = CTAS_OK;
+  auto score1 = [&](float x) -> float {
+    float axr = cent + x, s = 0.0f;
+    if (!rc) rc = ctas_axis_scores(g.h, &axr, 1, &s);
+    return s;
+  };
+
+  const int stpsz = 8;
+  const int steps = static_cast<int>(1 + 2 * (maxDev == 0.0f ? 1.0f : maxDev * static_cast<float>(w) / stpsz));
+  const int hfsteps = (steps - 1) / 2;
+  const float coffee = fcr - cent;
+  std::vector<float> vax(steps), err(steps), axr(steps);
+  for (int k = 0; k < steps; ++k) {
+    vax[k] = coffee + static_cast<float>(stpsz * (k - hfsteps));
+    axr[k] = cent + vax[k];
+  }
+  if ((rc = ctas_axis_scores(g.h, axr.data(), steps, err.data()))) return rc;
+  int idxM = 0;
+  float minval = FLT_MAX;
+  for (int i = 1; i < steps - 1; ++i)
+    if (err[i] < err[i - 1] && err[i] < err[i + 1] && err[i] < minval) {
+      idxM = i;
+      minval = err[i];
+    }
+  int cdir = 1;
+  if (!idxM) {
+    const int idx = err[0] < err[steps - 1] ? 0 : steps - 1;
+    cdir = idx == 0 ? -1 : 1;
+    const int coff2 = static_cast<int>(vax[idx - cdir * 2]);
+    const int maxSteps = static_cast<int>(static_cast<float>(w) * maxDevLM - static_cast<float>(cdir * coff2 / stpsz));
+    if (maxSteps <= 0) return fail(CTAS_E_AXIS, "Impossible number of steps.");
+    std::vector<float> v2, e2;
+    bool stop = false;
+    constexpr int kWalkBatch = 8;
+    for (int k0 = 0; k0 < maxSteps && !stop; k0 += kWalkBatch) {
+      const int m = std::min(kWalkBatch, maxSteps - k0);
+      std::vector<float> a(m), s(m);
+      for (int i = 0; i < m; ++i) a[i] = cent + (static_cast<float>(coff2) + static_cast<float>(stpsz * (k0 + i) * cdir));
+      if ((rc = ctas_axis_scores(g.h, a.data(), m, s.data()))) return rc;
+      for (int i = 0; i < m && !stop; ++i) {
+        v2.push_back(static_cast<float>(coff2) + static_cast<float>(stpsz * (k0 + i) * cdir));
+        e2.push_back(s[i]);
+        if (s[i] < minval) {
+          minval = s[i];
+          idxM = k0 + i;
+        } else {
+          stop = true;
+        }
+      }
+    }
+    if (idxM == maxSteps - 1 || idxM == 0) {
+      *result = fcr;
+      return CTAS_OK;
+    }
+    vax = v2;
+    err = e2;
+  }
+  const float rf = improve_me(0.25f, vax, err, idxM, cdir, score1, &rc);
+  if (rc) return rc;
+  *result = cent + rf;
+  return CTAS_OK;
+}

@@ -1,0 +1,853 @@
+// The C ABI of libctas_hip.so: CTrec-equivalent handles (lct.cpp:487-685),
+// device-resident and host-API variants, hipGraph capture, uneven batches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ctas_internal.hpp"
+
+namespace ctas {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int persistent_workgroups() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 1024;
+  return std::max(1, p.multiProcessorCount) * 4;
+}
+
+namespace {
+
+std::string fmt_float(float v) {
+  char b[64];
+  std::snprintf(b, sizeof b, "%g", static_cast<double>(v));
+  return b;
+}
+
+int check_center(int w, float c) {
+  // lct.cpp:614-616 / 651-653: abs(center) >= ish(1)/2 (integer half) throws.
+  if (!(std::fabs(c) < static_cast<float>(w / 2)))
+    return fail(CTAS_E_CENTER, "Rotation center " + fmt_float(c) +
+                                   " is outside the sinogram width " + std::to_string(w) + ".");
+  return CTAS_OK;
+}
+
+template <typename T>
+int grow(T *&ptr, size_t &cap, size_t bytes) {
+  if (bytes <= cap && ptr) return CTAS_OK;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  void *p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  ptr = static_cast<T *>(p);
+  cap = bytes;
+  return CTAS_OK;
+}
+
+int ensure_slices(ctas_fbp *h, int S) {
+  if (S <= h->mm_cap && h->d_minmax) return CTAS_OK;
+  if (h->d_minmax) (void)hipFree(h->d_minmax);
+  if (h->d_status) (void)hipFree(h->d_status);
+  h->d_minmax = nullptr;
+  h->d_status = nullptr;
+  h->mm_cap = 0;
+  if (hipMalloc(&h->d_minmax, sizeof(float2) * kMinMaxBlocks * std::max(S, 1)) != hipSuccess ||
+      hipMalloc(&h->d_status, sizeof(int) * std::max(S, 1)) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation of per-slice status failed");
+  h->mm_cap = S;
+  return CTAS_OK;
+}
+
+int choose_kind(const ctas_fbp *h);
+int choose_ac(const ctas_fbp *h);
+
+// Upload the item table unless it is identical to the resident one. The
+// stream is drained first, since a running kernel may still read the table.
+int upload_items(ctas_fbp *h, const std::vector<Item> &items, hipStream_t s, bool *changed,
+                 std::vector<Item> &resident) {
+  *changed = false;
+  if (resident.size() == items.size() && h->d_items &&
+      std::memcmp(resident.data(), items.data(), sizeof(Item) * items.size()) == 0)
+    return CTAS_OK;
+  *changed = true;
+  CTAS_HIP_TRY(hipStreamSynchronize(s));
+  if (static_cast<int>(items.size()) > h->items_cap) {
+    if (h->d_items) (void)hipFree(h->d_items);
+    h->d_items = nullptr;
+    h->items_cap = 0;
+    if (hipMalloc(&h->d_items, sizeof(Item) * items.size()) != hipSuccess)
+      return fail(CTAS_E_NOMEM, "device allocation of the item table failed");
+    h->items_cap = static_cast<int>(items.size());
+  }
+  CTAS_HIP_TRY(hipMemcpy(h->d_items, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice));
+  resident = items;
+  return CTAS_OK;
+}
+
+struct HandleExtra {
+  std::vector<Item> resident;
+};
+std::mutex g_extra_mu;
+std::map<const ctas_fbp *, std::unique_ptr<HandleExtra>> g_extra;
+
+HandleExtra &extra(const ctas_fbp *h) {
+  std::lock_guard<std::mutex> lk(g_extra_mu);
+  auto &p = g_extra[h];
+  if (!p) p.reset(new HandleExtra);
+  return *p;
+}
+void drop_extra(const ctas_fbp *h) {
+  std::lock_guard<std::mutex> lk(g_extra_mu);
+  g_extra.erase(h);
+}
+
+Item base_item(const ctas_fbp *h, int kind) {
+  Item it{};
+  it.sincos = h->d_sincos;
+  it.sincos64 = use_fp64_angles(h->flags) ? h->d_sincos64 : nullptr;
+  it.W = h->d_W;
+  it.tw = h->d_tw;
+  it.recCof = h->recCof;
+  it.w = h->w;
+  it.N = h->N;
+  it.Z = h->Z;
+  it.logZ = h->logZ;
+  it.tiles_x = (h->w + tile_cols(kind) - 1) / tile_cols(kind);
+  it.tiles = it.tiles_x * ((h->w + tile_rows(kind) - 1) / tile_rows(kind));
+  it.kind = kind;
+  return it;
+}
+
+hipStream_t pick_stream(const ctas_fbp *h, void *stream) {
+  return stream ? static_cast<hipStream_t>(stream) : h->stream;
+}
+
+// Enqueue prepare (+ optional backprojection) for S slices of device data.
+int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
+                   const float *centers, int S, bool do_bp, hipStream_t s) {
+  if (S <= 0) return fail(CTAS_E_ARG, "number of slices must be positive");
+  if (do_bp)
+    for (int i = 0; i < S; ++i)
+      if (int rc = check_center(h->w, centers[i])) return rc;
+  const int st = store_of(h->flags);
+  const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
+  if (int rc = grow(h->d_filt, h->filt_cap, slice_elems * S * store_bytes(st))) return rc;
+  if (int rc = ensure_slices(h, S)) return rc;
+  const int kind = choose_kind(h);
+  const int pairs = (h->N + 1) / 2;
+  std::vector<Item> items(S);
+  for (int i = 0; i < S; ++i) {
+    Item it = base_item(h, kind);
+    it.sino = d_sino + slice_elems * i;
+    it.filt = static_cast<unsigned char *>(h->d_filt) + slice_elems * i * store_bytes(st);
+    it.filt_f32 = d_f32 ? d_f32 + slice_elems * i : nullptr;
+    it.out = d_out ? d_out + static_cast<size_t>(h->w) * h->w * i : nullptr;
+    it.minmax = h->d_minmax + static_cast<size_t>(kMinMaxBlocks) * i;
+    it.status = h->d_status + i;
+    it.center = do_bp ? centers[i] : 0.0f;
+    it.tile_base = it.tiles * i;
+    it.pair_base = pairs * i;
+    items[i] = it;
+  }
+  bool changed = false;
+  if (int rc = upload_items(h, items, s, &changed, extra(h).resident)) return rc;
+  if (h->contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(h->d_items, S, s));
+  CTAS_HIP_TRY(launch_filter(h->d_items, S, pairs * S, h->logZ, h->contrast, st, s));
+  if (do_bp)
+    CTAS_HIP_TRY(launch_backproject(h->d_items, S, items[0].tiles * S, kind, choose_ac(h),
+                                    mode_of(h->flags), st, s));
+  h->last_S = S;
+  return CTAS_OK;
+}
+
+// Backproject n centres from the filtered slices already on device: all
+// from slice `slice` (per_slice == false, CTrec::repeat) or centre i from
+// slice i (per_slice == true).
+int enqueue_repeat(ctas_fbp *h, int slice, float *d_out, const float *centers, int n, hipStream_t s,
+                   bool per_slice = false) {
+  if (n <= 0) return fail(CTAS_E_ARG, "number of centres must be positive");
+  if (slice < 0 || slice >= h->kept_slices || (per_slice && n > h->kept_slices))
+    return fail(CTAS_E_STATE, "no filtered sinogram on the device: call ctas_fbp_sino first");
+  for (int i = 0; i < n; ++i)
+    if (int rc = check_center(h->w, centers[i])) return rc;
+  const int st = store_of(h->flags);
+  const int kind = choose_kind(h);
+  const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
+  std::vector<Item> items(n);
+  for (int i = 0; i < n; ++i) {
+    Item it = base_item(h, kind);
+    it.filt = static_cast<unsigned char *>(h->d_filt) + slice_elems * (per_slice ? i : slice) * store_bytes(st);
+    it.out = d_out + static_cast<size_t>(h->w) * h->w * i;
+    it.center = centers[i];
+    it.tile_base = it.tiles * i;
+    it.pair_base = 0;
+    items[i] = it;
+  }
+  bool changed = false;
+  if (int rc = upload_items(h, items, s, &changed, extra(h).resident)) return rc;
+  CTAS_HIP_TRY(launch_backproject(h->d_items, n, items[0].tiles * n, kind, choose_ac(h),
+                                  mode_of(h->flags), st, s));
+  return CTAS_OK;
+}
+
+int read_status(ctas_fbp *h, int S) {
+  if (h->contrast != CTAS_CONTRAST_ABS) return CTAS_OK;
+  std::vector<int> st(S);
+  CTAS_HIP_TRY(hipMemcpy(st.data(), h->d_status, sizeof(int) * S, hipMemcpyDeviceToHost));
+  for (int i = 0; i < S; ++i)
+    if (st[i] != 0) return fail(CTAS_E_DATA, "Bad absorption data (maximum is sub-zero).");
+  return CTAS_OK;
+}
+
+// ---- autotuner ------------------------------------------------------------
+struct TuneKey {
+  int wclass, nclass, mode, store, dev;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(wclass, nclass, mode, store, dev) < std::tie(o.wclass, o.nclass, o.mode, o.store, o.dev);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tune;
+
+int width_class(int w) {
+  int c = 1;
+  while (c < w) c <<= 1;
+  return c;
+}
+TuneKey tune_key(const ctas_fbp *h) {
+  return TuneKey{width_class(h->w), width_class(h->N), mode_of(h->flags), store_of(h->flags), h->device};
+}
+
+int choose_kind(const ctas_fbp *h) {
+  if (h->tile_kind != TK_AUTO) return h->tile_kind;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(tune_key(h));
+    if (it != g_tune.end()) return it->second.first;
+  }
+  return h->w >= 1024 ? TK_128x128 : TK_64x64;
+}
+int choose_ac(const ctas_fbp *h) {
+  if (h->angles_per_pass) return h->angles_per_pass;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(tune_key(h));
+    if (it != g_tune.end()) return it->second.second;
+  }
+  return 32;
+}
+
+// ---- shared resources of uneven batches -------------------------------------
+struct GeomRes {
+  float *d_W = nullptr;
+  float2 *d_sincos = nullptr;
+  double2 *d_sincos64 = nullptr;
+  float2 *d_tw = nullptr;
+};
+std::mutex g_geo_mu;
+std::map<std::tuple<int, int, int, int, float, float, int>, GeomRes> g_geo;  // (dev, w, N, filter, param, anarc, contrast)
+
+int make_twiddles(int Z, float2 **d_tw) {
+  std::vector<float2> tw(std::max(Z / 2, 1));
+  for (int m = 0; m < Z / 2; ++m) {
+    const double a = -2.0 * M_PI * m / Z;
+    tw[m] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  if (hipMalloc(d_tw, sizeof(float2) * tw.size()) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "twiddle allocation failed");
+  CTAS_HIP_TRY(hipMemcpy(*d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+  return CTAS_OK;
+}
+
+int geometry_resources(int w, int N, int filter, float param, float anarc, int contrast, GeomRes *out) {
+  int dev = 0;
+  CTAS_HIP_TRY(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, w, N, filter, param, anarc, contrast);
+  std::lock_guard<std::mutex> lk(g_geo_mu);
+  auto f = g_geo.find(key);
+  if (f != g_geo.end()) {
+    *out = f->second;
+    return CTAS_OK;
+  }
+  std::vector<float> W;
+  if (int rc = filter_window(filter, param, w, contrast, W)) return rc;
+  std::vector<float> sc(2 * static_cast<size_t>(N));
+  angle_table(N, anarc, sc.data());
+  GeomRes r;
+  std::vector<double> sc64(2 * static_cast<size_t>(N));
+  angle_table64(N, anarc, sc64.data());
+  if (hipMalloc(&r.d_W, sizeof(float) * W.size()) != hipSuccess ||
+      hipMalloc(&r.d_sincos, sizeof(float2) * N) != hipSuccess ||
+      hipMalloc(&r.d_sincos64, sizeof(double2) * N) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "geometry allocation failed");
+  CTAS_HIP_TRY(hipMemcpy(r.d_sincos64, sc64.data(), sizeof(double) * sc64.size(), hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(r.d_W, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(r.d_sincos, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
+  if (int rc = make_twiddles(zidth(w), &r.d_tw)) return rc;
+  g_geo[key] = r;
+  *out = r;
+  return CTAS_OK;
+}
+
+struct BatchWork {
+  void *d_filt = nullptr;  size_t filt_cap = 0;
+  float2 *d_mm = nullptr;  size_t mm_cap = 0;
+  int *d_status = nullptr; size_t st_cap = 0;
+  Item *d_items = nullptr; size_t it_cap = 0;
+  unsigned *d_counter = nullptr; size_t ctr_cap = 0;
+};
+std::mutex g_batch_mu;
+std::map<int, BatchWork> g_batch;
+
+}  // namespace
+}  // namespace ctas
+
+using namespace ctas;
+
+extern "C" {
+
+const char *ctas_last_error(void) { return g_last_error.c_str(); }
+
+const char *ctas_version(void) { return "ctas-mi355x 0.1.0 (gfx950)"; }
+
+int ctas_device_count(int *n) {
+  if (!n) return fail(CTAS_E_ARG, "null argument");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return CTAS_OK;
+}
+
+int ctas_zidth(int w) { return zidth(w); }
+float ctas_arc_rad(float anarc, int points) { return arc_rad(anarc, points); }
+float ctas_step_rad(float anarc, int points) { return step_rad(anarc, points); }
+
+int ctas_filter_window(int filter, float param, int w, int contrast, float *W) {
+  if (!W) return fail(CTAS_E_ARG, "null argument");
+  std::vector<float> v;
+  if (int rc = filter_window(filter, param, w, contrast, v)) return rc;
+  std::memcpy(W, v.data(), sizeof(float) * v.size());
+  return CTAS_OK;
+}
+
+int ctas_angles(int n, float anarc, float *sincos) {
+  if (!sincos || n <= 0) return fail(CTAS_E_ARG, "bad angle table arguments");
+  angle_table(n, anarc, sincos);
+  return CTAS_OK;
+}
+
+int ctas_filter_parse(const char *spec, int *filter, float *param) {
+  if (!spec || !filter || !param) return fail(CTAS_E_ARG, "null argument");
+  std::string in(spec), name = in;
+  float p = 0.0f;
+  const size_t colon = in.find(':');
+  if (colon != std::string::npos) {
+    name = in.substr(0, colon);
+    const std::string ps = in.substr(colon + 1);
+    char *end = nullptr;
+    p = std::strtof(ps.c_str(), &end);
+    if (ps.empty() || (end && *end)) return fail(CTAS_E_ARG, "could not parse filter parameter \"" + ps + "\"");
+  }
+  std::string up = name;
+  for (char &ch : up) ch = static_cast<char>(std::toupper(static_cast<unsigned char>(ch)));
+  int f;
+  // Filter::fromName (lct.cpp:71-86); "BARLETT" selects RAMP there.
+  if (up == "NONE") f = CTAS_FILTER_NONE;
+  else if (up.empty() || up == "RAMP") f = CTAS_FILTER_RAMP;
+  else if (up == "BARLETT") f = CTAS_FILTER_RAMP;
+  else if (up == "WELCH") f = CTAS_FILTER_WELCH;
+  else if (up == "PARZEN") f = CTAS_FILTER_PARZEN;
+  else if (up == "HANN") f = CTAS_FILTER_HANN;
+  else if (up == "HAMMING") f = CTAS_FILTER_HAMMING;
+  else if (up == "BLACKMAN") f = CTAS_FILTER_BLACKMAN;
+  else if (up == "LANCKZOS") f = CTAS_FILTER_LANCKZOS;
+  else if (up == "KAISER") f = CTAS_FILTER_KAISER;
+  else if (up == "GAUSS") f = CTAS_FILTER_GAUSS;
+  else return fail(CTAS_E_ARG, "The string \"" + name + "\" does not describe any known filter.");
+  // Filter ctor checks (ct.h:148-151)
+  if (f == CTAS_FILTER_KAISER && p < 0.0f)
+    return fail(CTAS_E_ARG, "Alpha parameter of Kaiser filter must be non-negative (set " + fmt_float(p) + "). ");
+  if (f == CTAS_FILTER_GAUSS && p <= 0.0f)
+    return fail(CTAS_E_ARG, "Sigma parameter of Guass filter must be strictly positive (set " + fmt_float(p) + "). ");
+  *filter = f;
+  *param = p;
+  return CTAS_OK;
+}
+
+int ctas_contrast_parse(const char *name, int *contrast) {
+  if (!name || !contrast) return fail(CTAS_E_ARG, "null argument");
+  std::string up(name);
+  for (char &ch : up) ch = static_cast<char>(std::toupper(static_cast<unsigned char>(ch)));
+  if (up == "A" || up == "ABS" || up == "ABSORPTION") *contrast = CTAS_CONTRAST_ABS;
+  else if (up == "R" || up == "REF" || up == "REFRACTION") *contrast = CTAS_CONTRAST_REF;
+  else if (up == "P" || up == "PHS" || up == "PHASE") *contrast = CTAS_CONTRAST_PHS;
+  else if (up == "F" || up == "FLT" || up == "FILTERED") *contrast = CTAS_CONTRAST_FLT;
+  else return fail(CTAS_E_ARG, std::string("The string \"") + name + "\" does not describe any known contrast.");
+  return CTAS_OK;
+}
+
+int ctas_fbp_create(ctas_fbp **ph, int w, int n_angles, int contrast, int filter, float filter_param,
+                    float anarc, int flags) {
+  if (!ph) return fail(CTAS_E_ARG, "null handle pointer");
+  *ph = nullptr;
+  if (w < 2) return fail(CTAS_E_ARG, "Number of pixels in the CT reconstruction less than 2.");
+  if (n_angles <= 0) return fail(CTAS_E_ARG, "Zero projections in sinogram.");
+  if (contrast < CTAS_CONTRAST_ABS || contrast > CTAS_CONTRAST_FLT)
+    return fail(CTAS_E_ARG, "unknown contrast code " + std::to_string(contrast));
+  if ((flags & ~(CTAS_INTERP_MASK | CTAS_STORE_MASK)) != 0 || (flags & CTAS_INTERP_MASK) > CTAS_INTERP_LINEAR ||
+      (flags & CTAS_STORE_MASK) > CTAS_STORE_BF16)
+    return fail(CTAS_E_ARG, "unknown flags " + std::to_string(flags));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(CTAS_E_DEVICE, "no HIP device is available");
+  std::unique_ptr<ctas_fbp> h(new ctas_fbp);
+  CTAS_HIP_TRY(hipGetDevice(&h->device));
+  h->w = w;
+  h->N = n_angles;
+  h->Z = zidth(w);
+  h->logZ = 0;
+  while ((1 << h->logZ) < h->Z) ++h->logZ;
+  hipDeviceProp_t prop;
+  CTAS_HIP_TRY(hipGetDeviceProperties(&prop, h->device));
+  if (sizeof(float2) * static_cast<size_t>(h->Z) > prop.sharedMemPerBlock)
+    return fail(CTAS_E_ARG, "width " + std::to_string(w) + " needs a padded line of " +
+                                std::to_string(h->Z) + " points, more than one workgroup's LDS holds");
+  h->contrast = contrast;
+  h->filter = filter;
+  h->fparam = filter_param;
+  h->anarc = anarc;
+  h->flags = flags;
+  if (int rc = filter_window(filter, filter_param, w, contrast, h->W)) return rc;
+  h->sincos.resize(2 * static_cast<size_t>(n_angles));
+  angle_table(n_angles, anarc, h->sincos.data());
+  CTAS_HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  if (hipMalloc(&h->d_W, sizeof(float) * h->Z) != hipSuccess ||
+      hipMalloc(&h->d_sincos, sizeof(float2) * n_angles) != hipSuccess ||
+      hipMalloc(&h->d_sincos64, sizeof(double2) * n_angles) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation failed");
+  {
+    std::vector<double> sc64(2 * static_cast<size_t>(n_angles));
+    angle_table64(n_angles, anarc, sc64.data());
+    CTAS_HIP_TRY(hipMemcpy(h->d_sincos64, sc64.data(), sizeof(double) * sc64.size(), hipMemcpyHostToDevice));
+  }
+  CTAS_HIP_TRY(hipMemcpy(h->d_W, h->W.data(), sizeof(float) * h->Z, hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(h->d_sincos, h->sincos.data(), sizeof(float) * h->sincos.size(), hipMemcpyHostToDevice));
+  if (int rc = make_twiddles(h->Z, &h->d_tw)) return rc;
+  *ph = h.release();
+  return CTAS_OK;
+}
+
+int ctas_fbp_clone(const ctas_fbp *src, ctas_fbp **ph) {
+  if (!src || !ph) return fail(CTAS_E_ARG, "null argument");
+  ctas_fbp *h = nullptr;
+  if (int rc = ctas_fbp_create(&h, src->w, src->N, src->contrast, src->filter, src->fparam, src->anarc, src->flags))
+    return rc;
+  h->recCof = src->recCof;
+  h->tile_kind = src->tile_kind;
+  h->angles_per_pass = src->angles_per_pass;
+  if (src->sincos != h->sincos) ctas_fbp_set_angles(h, src->sincos.data());
+  *ph = h;
+  return CTAS_OK;
+}
+
+int ctas_fbp_destroy(ctas_fbp *h) {
+  if (!h) return CTAS_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  void *ptrs[] = {h->d_W, h->d_sincos, h->d_sincos64, h->d_tw, h->d_filt, h->d_io, h->d_out, h->d_minmax,
+                  h->d_status, h->d_items, h->d_centres, h->d_partial, h->d_tmp, h->d_scores};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  drop_extra(h);
+  delete h;
+  return CTAS_OK;
+}
+
+int ctas_fbp_info(const ctas_fbp *h, int *w, int *n, int *z, int *flags) {
+  if (!h) return fail(CTAS_E_ARG, "null handle");
+  if (w) *w = h->w;
+  if (n) *n = h->N;
+  if (z) *z = h->Z;
+  if (flags) *flags = h->flags;
+  return CTAS_OK;
+}
+
+int ctas_fbp_set_physics(ctas_fbp *h, float pixel_size, float lambda) {
+  if (!h) return fail(CTAS_E_ARG, "null handle");
+  // lct.cpp:558-566: float recCof scaled by double factors.
+  float r = 1.0f;
+  if (lambda > 0.0f) r = static_cast<float>(r * (lambda / (2 * M_PI)));
+  if (pixel_size > 0.0f) r = static_cast<float>(r * (M_PI / pixel_size));
+  h->recCof = r;
+  return CTAS_OK;
+}
+
+int ctas_fbp_get_reccof(const ctas_fbp *h, float *r) {
+  if (!h || !r) return fail(CTAS_E_ARG, "null argument");
+  *r = h->recCof;
+  return CTAS_OK;
+}
+
+int ctas_fbp_set_angles(ctas_fbp *h, const float *sincos) {
+  if (!h || !sincos) return fail(CTAS_E_ARG, "null argument");
+  CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
+  std::memcpy(h->sincos.data(), sincos, sizeof(float) * h->sincos.size());
+  CTAS_HIP_TRY(hipMemcpy(h->d_sincos, h->sincos.data(), sizeof(float) * h->sincos.size(), hipMemcpyHostToDevice));
+  // a caller-supplied table replaces the FP64 one too (promoted exactly)
+  std::vector<double> sc64(h->sincos.begin(), h->sincos.end());
+  CTAS_HIP_TRY(hipMemcpy(h->d_sincos64, sc64.data(), sizeof(double) * sc64.size(), hipMemcpyHostToDevice));
+  return CTAS_OK;
+}
+
+int ctas_fbp_get_angles(const ctas_fbp *h, float *sincos) {
+  if (!h || !sincos) return fail(CTAS_E_ARG, "null argument");
+  std::memcpy(sincos, h->sincos.data(), sizeof(float) * h->sincos.size());
+  return CTAS_OK;
+}
+
+int ctas_fbp_get_window(const ctas_fbp *h, float *W) {
+  if (!h || !W) return fail(CTAS_E_ARG, "null argument");
+  std::memcpy(W, h->W.data(), sizeof(float) * h->W.size());
+  return CTAS_OK;
+}
+
+int ctas_fbp_set_tiles(ctas_fbp *h, int kind, int ac) {
+  if (!h) return fail(CTAS_E_ARG, "null handle");
+  if (kind < TK_AUTO || kind > TK_128x128) return fail(CTAS_E_ARG, "unknown tile kind");
+  if (ac != 0 && ac != 16 && ac != 32) return fail(CTAS_E_ARG, "angles per pass must be 0, 16 or 32");
+  h->tile_kind = kind;
+  h->angles_per_pass = ac;
+  return CTAS_OK;
+}
+
+int ctas_fbp_get_tiles(const ctas_fbp *h, int *kind, int *ac) {
+  if (!h) return fail(CTAS_E_ARG, "null handle");
+  if (kind) *kind = choose_kind(h);
+  if (ac) *ac = choose_ac(h);
+  return CTAS_OK;
+}
+
+// Host-API work: upload S slices, prepare (and backproject), download.
+static int host_run(ctas_fbp *h, float *sino, float *out, const float *centers, int S, bool do_bp) {
+  if (!h || !sino || S <= 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (do_bp && (!out || !centers)) return fail(CTAS_E_ARG, "bad arguments");
+  const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
+  const size_t in_bytes = sizeof(float) * slice_elems * S;
+  const bool f32_copy = store_of(h->flags) != ST_F32;
+  if (int rc = grow(h->d_io, h->io_cap, in_bytes * (f32_copy ? 2 : 1))) return rc;
+  if (do_bp)
+    if (int rc = grow(h->d_out, h->out_cap, sizeof(float) * h->w * h->w * S)) return rc;
+  hipStream_t s = h->stream;
+  CTAS_HIP_TRY(hipStreamSynchronize(s));
+  CTAS_HIP_TRY(hipMemcpy(h->d_io, sino, in_bytes, hipMemcpyHostToDevice));
+  float *d_f32 = f32_copy ? h->d_io + slice_elems * S : nullptr;
+  if (int rc = enqueue_slices(h, h->d_io, do_bp ? h->d_out : nullptr, d_f32, centers, S, do_bp, s)) return rc;
+  CTAS_HIP_TRY(hipStreamSynchronize(s));
+  if (int rc = read_status(h, S)) return rc;
+  // The reference filters the caller's sinogram in place.
+  if (h->contrast != CTAS_CONTRAST_FLT) {
+    const void *src = f32_copy ? static_cast<const void *>(d_f32) : h->d_filt;
+    CTAS_HIP_TRY(hipMemcpy(sino, src, in_bytes, hipMemcpyDeviceToHost));
+  }
+  if (do_bp) CTAS_HIP_TRY(hipMemcpy(out, h->d_out, sizeof(float) * h->w * h->w * S, hipMemcpyDeviceToHost));
+  h->kept_slices = S;
+  return CTAS_OK;
+}
+
+int ctas_fbp_prepare(ctas_fbp *h, float *sino, int S) { return host_run(h, sino, nullptr, nullptr, S, false); }
+
+int ctas_fbp_sino(ctas_fbp *h, float *sino) { return host_run(h, sino, nullptr, nullptr, 1, false); }
+
+int ctas_fbp_reconstruct(ctas_fbp *h, float *sino, float *out, const float *centers, int S) {
+  return host_run(h, sino, out, centers, S, true);
+}
+
+int ctas_fbp_repeat(ctas_fbp *h, float *out, const float *centers, int n) {
+  if (!h || !out || !centers || n <= 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (int rc = grow(h->d_out, h->out_cap, sizeof(float) * h->w * h->w * n)) return rc;
+  if (int rc = enqueue_repeat(h, 0, h->d_out, centers, n, h->stream)) return rc;
+  CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
+  CTAS_HIP_TRY(hipMemcpy(out, h->d_out, sizeof(float) * h->w * h->w * n, hipMemcpyDeviceToHost));
+  return CTAS_OK;
+}
+
+int ctas_fbp_reconstruct_dev(ctas_fbp *h, const float *d_sino, float *d_out, const float *centers, int S,
+                             void *stream) {
+  if (!h || !d_sino || !d_out || !centers) return fail(CTAS_E_ARG, "bad arguments");
+  const int rc = enqueue_slices(h, d_sino, d_out, nullptr, centers, S, true, pick_stream(h, stream));
+  if (rc == CTAS_OK) h->kept_slices = S;
+  return rc;
+}
+
+int ctas_fbp_repeat_dev(ctas_fbp *h, int slice, float *d_out, const float *centers, int n, void *stream) {
+  if (!h || !d_out || !centers) return fail(CTAS_E_ARG, "bad arguments");
+  return enqueue_repeat(h, slice, d_out, centers, n, pick_stream(h, stream));
+}
+
+int ctas_fbp_prepare_dev(ctas_fbp *h, const float *d_sino, int S, void *stream) {
+  if (!h || !d_sino) return fail(CTAS_E_ARG, "bad arguments");
+  const int rc = enqueue_slices(h, d_sino, nullptr, nullptr, nullptr, S, false, pick_stream(h, stream));
+  if (rc == CTAS_OK) h->kept_slices = S;
+  return rc;
+}
+
+int ctas_fbp_backproject_dev(ctas_fbp *h, float *d_out, const float *centers, int S, void *stream) {
+  if (!h || !d_out || !centers) return fail(CTAS_E_ARG, "bad arguments");
+  return enqueue_repeat(h, 0, d_out, centers, S, pick_stream(h, stream), true);
+}
+
+int ctas_fbp_status(ctas_fbp *h, int S, int *status) {
+  if (!h || !status || S <= 0 || S > h->mm_cap) return fail(CTAS_E_ARG, "bad arguments");
+  CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->contrast != CTAS_CONTRAST_ABS) {
+    for (int i = 0; i < S; ++i) status[i] = 0;
+    return CTAS_OK;
+  }
+  CTAS_HIP_TRY(hipMemcpy(status, h->d_status, sizeof(int) * S, hipMemcpyDeviceToHost));
+  return CTAS_OK;
+}
+
+int ctas_fbp_graph_capture(ctas_fbp *h, const float *d_sino, float *d_out, const float *centers, int S,
+                           void *stream) {
+  if (!h || !d_sino || !d_out || !centers) return fail(CTAS_E_ARG, "bad arguments");
+  hipStream_t s = pick_stream(h, stream);
+  if (h->graph_exec) {
+    (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+  }
+  if (h->graph) {
+    (void)hipGraphDestroy(h->graph);
+    h->graph = nullptr;
+  }
+  // Make the item table resident before capture so that no copy is recorded.
+  CTAS_HIP_TRY(hipStreamSynchronize(s));
+  extra(h).resident.clear();
+  {
+    // first enqueue uploads the table; it also runs the work once
+    if (int rc = enqueue_slices(h, d_sino, d_out, nullptr, centers, S, true, s)) return rc;
+    CTAS_HIP_TRY(hipStreamSynchronize(s));
+  }
+  CTAS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_slices(h, d_sino, d_out, nullptr, centers, S, true, s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(CTAS_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h->graph = g;
+  CTAS_HIP_TRY(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+  h->kept_slices = S;
+  return CTAS_OK;
+}
+
+int ctas_fbp_graph_launch(ctas_fbp *h, void *stream) {
+  if (!h) return fail(CTAS_E_ARG, "null handle");
+  if (!h->graph_exec) return fail(CTAS_E_STATE, "no captured graph: call ctas_fbp_graph_capture first");
+  CTAS_HIP_TRY(hipGraphLaunch(h->graph_exec, pick_stream(h, stream)));
+  return CTAS_OK;
+}
+
+int ctas_fbp_autotune(ctas_fbp *h, int S) {
+  if (!h || S <= 0) return fail(CTAS_E_ARG, "bad arguments");
+  const int keep_kind = h->tile_kind, keep_ac = h->angles_per_pass;
+  const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
+  float *d_in = nullptr, *d_o = nullptr;
+  if (hipMalloc(&d_in, sizeof(float) * slice_elems * S) != hipSuccess ||
+      hipMalloc(&d_o, sizeof(float) * h->w * h->w * S) != hipSuccess) {
+    if (d_in) (void)hipFree(d_in);
+    return fail(CTAS_E_NOMEM, "autotune allocation failed");
+  }
+  // Transmission-like data in (0.5, 1]: a constant is enough to time the
+  // gather, whose cost does not depend on the values.
+  std::vector<float> ones(slice_elems * S, 0.75f);
+  CTAS_HIP_TRY(hipMemcpy(d_in, ones.data(), sizeof(float) * ones.size(), hipMemcpyHostToDevice));
+  std::vector<float> centers(S, 0.0f);
+  hipEvent_t e0, e1;
+  CTAS_HIP_TRY(hipEventCreate(&e0));
+  CTAS_HIP_TRY(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bk = TK_128x128, bac = 32;
+  const int kinds[] = {TK_64x64, TK_128x64, TK_128x128};
+  const int acs[] = {16, 32};
+  int rc = CTAS_OK;
+  for (int k : kinds) {
+    for (int ac : acs) {
+      h->tile_kind = k;
+      h->angles_per_pass = ac;
+      rc = enqueue_slices(h, d_in, d_o, nullptr, centers.data(), S, true, h->stream);
+      if (rc) break;
+      CTAS_HIP_TRY(hipEventRecord(e0, h->stream));
+      for (int r = 0; r < 3 && !rc; ++r) rc = enqueue_slices(h, d_in, d_o, nullptr, centers.data(), S, true, h->stream);
+      CTAS_HIP_TRY(hipEventRecord(e1, h->stream));
+      CTAS_HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.f;
+      CTAS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bk = k;
+        bac = ac;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_in);
+  (void)hipFree(d_o);
+  h->tile_kind = keep_kind;
+  h->angles_per_pass = keep_ac;
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[tune_key(h)] = {bk, bac};
+  return CTAS_OK;
+}
+
+int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int filter, float filter_param,
+                          float anarc, int flags, void *stream) {
+  if (!items || n <= 0) return fail(CTAS_E_ARG, "bad arguments");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
+  int dev = 0;
+  CTAS_HIP_TRY(hipGetDevice(&dev));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int st = store_of(flags), mode = mode_of(flags);
+  std::vector<Item> its(n);
+  size_t filt_bytes = 0;
+  int max_logZ = 0, pairs = 0, tiles = 0;
+  // Work order: costliest sinograms first, so that the persistent launch
+  // ends with the small tiles.
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return static_cast<double>(items[a].w) * items[a].w * items[a].n_angles >
+           static_cast<double>(items[b].w) * items[b].w * items[b].n_angles;
+  });
+  std::vector<size_t> filt_off(n);
+  for (int oi = 0; oi < n; ++oi) {
+    const ctas_batch_item &bi = items[order[oi]];
+    if (bi.w < 2 || bi.n_angles <= 0 || !bi.d_sino || !bi.d_out)
+      return fail(CTAS_E_ARG, "bad batch item " + std::to_string(order[oi]));
+    if (int rc = check_center(bi.w, bi.center)) return rc;
+    GeomRes g;
+    if (int rc = geometry_resources(bi.w, bi.n_angles, filter, filter_param, anarc, contrast, &g)) return rc;
+    Item it{};
+    it.sino = bi.d_sino;
+    it.out = bi.d_out;
+    it.sincos = g.d_sincos;
+    it.sincos64 = use_fp64_angles(flags) ? g.d_sincos64 : nullptr;
+    it.W = g.d_W;
+    it.tw = g.d_tw;
+    it.center = bi.center;
+    it.recCof = 1.0f;
+    it.w = bi.w;
+    it.N = bi.n_angles;
+    it.Z = zidth(bi.w);
+    while ((1 << it.logZ) < it.Z) ++it.logZ;
+    max_logZ = std::max(max_logZ, it.logZ);
+    int kind;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto f = g_tune.find(TuneKey{width_class(bi.w), width_class(bi.n_angles), mode, st, dev});
+      kind = f != g_tune.end() ? f->second.first : (bi.w >= 1024 ? TK_128x128 : TK_64x64);
+    }
+    it.kind = kind;
+    it.tiles_x = (bi.w + tile_cols(kind) - 1) / tile_cols(kind);
+    it.tiles = it.tiles_x * ((bi.w + tile_rows(kind) - 1) / tile_rows(kind));
+    it.tile_base = tiles;
+    it.pair_base = pairs;
+    tiles += it.tiles;
+    pairs += (bi.n_angles + 1) / 2;
+    filt_off[oi] = filt_bytes;
+    filt_bytes += (static_cast<size_t>(bi.n_angles) * bi.w * store_bytes(st) + 255) & ~static_cast<size_t>(255);
+    its[oi] = it;
+  }
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  BatchWork &bw = g_batch[dev];
+  CTAS_HIP_TRY(hipStreamSynchronize(s));
+  if (int rc = grow(bw.d_filt, bw.filt_cap, filt_bytes)) return rc;
+  if (int rc = grow(bw.d_mm, bw.mm_cap, sizeof(float2) * kMinMaxBlocks * n)) return rc;
+  if (int rc = grow(bw.d_status, bw.st_cap, sizeof(int) * n)) return rc;
+  if (int rc = grow(bw.d_items, bw.it_cap, sizeof(Item) * n)) return rc;
+  if (int rc = grow(bw.d_counter, bw.ctr_cap, sizeof(unsigned) * 64)) return rc;
+  for (int oi = 0; oi < n; ++oi) {
+    its[oi].filt = static_cast<unsigned char *>(bw.d_filt) + filt_off[oi];
+    its[oi].minmax = bw.d_mm + static_cast<size_t>(kMinMaxBlocks) * oi;
+    its[oi].status = bw.d_status + oi;
+  }
+  CTAS_HIP_TRY(hipMemcpy(bw.d_items, its.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
+  if (contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(bw.d_items, n, s));
+  CTAS_HIP_TRY(launch_filter(bw.d_items, n, pairs, max_logZ, contrast, st, s));
+  CTAS_HIP_TRY(launch_backproject_persistent(bw.d_items, n, tiles, mode, st, bw.d_counter,
+                                             persistent_workgroups(), s));
+  CTAS_HIP_TRY(hipStreamSynchronize(s));
+  if (contrast == CTAS_CONTRAST_ABS) {
+    std::vector<int> stv(n);
+    CTAS_HIP_TRY(hipMemcpy(stv.data(), bw.d_status, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+      if (stv[i]) return fail(CTAS_E_DATA, "Bad absorption data (maximum is sub-zero).");
+  }
+  return CTAS_OK;
+}
+
+int ctas_ring_filter(float *sino, int N, int w, int box) {
+  if (!sino || N <= 0 || w <= 0 || box < 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (box == 0) return CTAS_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
+  float *d = nullptr, *avg = nullptr;
+  if (hipMalloc(&d, sizeof(float) * N * w) != hipSuccess || hipMalloc(&avg, sizeof(float) * w) != hipSuccess) {
+    if (d) (void)hipFree(d);
+    return fail(CTAS_E_NOMEM, "allocation failed");
+  }
+  int rc = CTAS_OK;
+  if (hipMemcpy(d, sino, sizeof(float) * N * w, hipMemcpyHostToDevice) != hipSuccess ||
+      launch_ring(d, avg, 1, N, w, box, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(sino, d, sizeof(float) * N * w, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(CTAS_E_DEVICE, "ring filter failed on the device");
+  (void)hipFree(d);
+  (void)hipFree(avg);
+  return rc;
+}
+
+int ctas_ring_filter_dev(float *d_sino, int S, int N, int w, int box, void *stream) {
+  if (!d_sino || S <= 0 || N <= 0 || w <= 0 || box < 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (box == 0) return CTAS_OK;
+  // The column means need S*w floats of scratch; it is freed after the
+  // stream drains.
+  float *avg = nullptr;
+  if (hipMalloc(&avg, sizeof(float) * S * w) != hipSuccess) return fail(CTAS_E_NOMEM, "allocation failed");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const hipError_t e = launch_ring(d_sino, avg, S, N, w, box, s);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(avg);
+  if (e != hipSuccess) return fail(CTAS_E_DEVICE, std::string("ring filter: ") + hipGetErrorString(e));
+  return CTAS_OK;
+}
+
+}  // extern "C"

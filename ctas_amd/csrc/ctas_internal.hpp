@@ -1,0 +1,142 @@
+// Internal declarations shared by the translation units of libctas_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ctas_hip.h"
+
+namespace ctas {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define CTAS_HIP_TRY(expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::ctas::fail(CTAS_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- host geometry (window.cpp) -------------------------------------------
+int zidth(int w);
+float arc_rad(float anarc, int points);
+float step_rad(float anarc, int points);
+void angle_table(int n, float anarc, float *sincos);
+void angle_table64(int n, float anarc, double *sincos);
+// FP64 per-angle constants are used unless the mode is the parity mode
+// (CTAS sampling with float32 storage).
+inline bool use_fp64_angles(int flags) {
+  return (flags & CTAS_INTERP_MASK) != CTAS_INTERP_CTAS || (flags & CTAS_STORE_MASK) != CTAS_STORE_FP32;
+}
+int filter_window(int filter, float param, int w, int contrast, std::vector<float> &W);
+
+// ---- device descriptors -----------------------------------------------------
+// One sinogram -> one slice. The backprojector walks tiles of every item;
+// the filter walks row pairs of every item that needs filtering.
+struct Item {
+  const float *sino;      // raw sinogram [N][w] (filter input)
+  void *filt;             // filtered sinogram [N][w], storage dtype
+  float *filt_f32;        // optional float32 copy of the filtered rows (host API)
+  float *out;             // slice [w][w]
+  const float2 *sincos;   // [N] {sin, cos}: the reference's float table
+  const double2 *sincos64; // [N] FP64 {sin, cos} of i*arc/N; when set, the
+                           // per-angle constants are formed in FP64 and
+                           // rounded once (LINEAR and FP16/BF16 modes)
+  const float *W;         // [Z] window
+  const float2 *tw;       // [Z/2] twiddles exp(-2 pi i m / Z)
+  float2 *minmax;         // [kMinMaxBlocks] partial (min, max) of the raw sinogram
+  int *status;            // deAbs status (0 / CTAS_E_DATA)
+  float center;
+  float recCof;
+  int w, N, Z, logZ;
+  int tiles_x, tiles;     // backprojection tiles of this item
+  int tile_base;          // prefix sum of tiles over the items
+  int pair_base;          // prefix sum of row pairs over the items
+  int kind;               // tile kind used for this item (persistent kernel)
+  int pad_;
+};
+
+constexpr int kMinMaxBlocks = 64;
+
+enum Store { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
+enum Mode { MODE_CTAS = 0, MODE_LINEAR = 1 };
+
+inline int store_of(int flags) {
+  const int s = flags & CTAS_STORE_MASK;
+  return s == CTAS_STORE_FP16 ? ST_F16 : s == CTAS_STORE_BF16 ? ST_BF16 : ST_F32;
+}
+inline int mode_of(int flags) { return (flags & CTAS_INTERP_MASK) == CTAS_INTERP_LINEAR ? MODE_LINEAR : MODE_CTAS; }
+inline size_t store_bytes(int st) { return st == ST_F32 ? 4 : 2; }
+
+// Tile kinds: rows x cols of output per 256-thread workgroup.
+enum TileKind { TK_AUTO = 0, TK_64x64 = 1, TK_128x64 = 2, TK_128x128 = 3 };
+inline int tile_rows(int kind) { return kind == TK_64x64 ? 64 : 128; }
+inline int tile_cols(int kind) { return kind == TK_128x128 ? 128 : 64; }
+
+// ---- launchers (filter.hip, backproject.hip) ------------------------------
+// Per-item (min, max) partials of the raw sinograms (ABS contrast only).
+hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s);
+// deAbs + padded FFT filtering + REF prefix sum for all row pairs.
+hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int max_logZ,
+                         int contrast, int store, hipStream_t s);
+// Backprojection of all tiles of all items (grid-mapped).
+hipError_t launch_backproject(const Item *d_items, int n_items, int total_tiles, int kind,
+                              int angles_per_pass, int mode, int store, hipStream_t s);
+// Same, one persistent launch with a work counter; items may mix tile kinds.
+hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int total_tiles,
+                                         int mode, int store, unsigned *d_counter,
+                                         int n_workgroups, hipStream_t s);
+// Ring filter (RingFilter::apply) on S sinograms in place.
+hipError_t launch_ring(float *d_sino, float *d_avg, int S, int N, int w, int box, hipStream_t s);
+// Axis-score kernels (axis.hip).
+// AverageGradient::apply scores of n slices [n][w][w] reconstructed at the
+// given centres; d_tmp holds n*w*w floats, d_partial n*kScoreBlocks*2.
+hipError_t launch_ag_scores(float *d_slices, int n, int w, const float *d_centres, float sigma,
+                            float *d_tmp, double *d_partial, float *d_scores, hipStream_t s);
+constexpr int kScoreBlocks = 128;
+// Phantom projector (phantom.hip).
+hipError_t launch_phantom(int N, int w, double c, int n_ell, const double *d_ell, double N0,
+                          uint64_t seed, const float *d_gain, float *d_out, hipStream_t s);
+
+int persistent_workgroups();
+
+}  // namespace ctas
+
+// Handle (CTrec state). Defined here so that the C ABI units share it.
+struct ctas_fbp {
+  int device = 0;
+  int w = 0, N = 0, Z = 0, logZ = 0;
+  int contrast = 0, filter = 0, flags = 0;
+  float fparam = 0.f, anarc = 180.f;
+  float recCof = 1.f;
+  std::vector<float> W;        // window [Z]
+  std::vector<float> sincos;   // angle table [2N]
+  float *d_W = nullptr;
+  float2 *d_sincos = nullptr;
+  double2 *d_sincos64 = nullptr;
+  float2 *d_tw = nullptr;
+  hipStream_t stream = nullptr;
+
+  // workspaces (grown on demand)
+  void *d_filt = nullptr;  size_t filt_cap = 0;      // bytes
+  float *d_io = nullptr;   size_t io_cap = 0;        // bytes, host-API staging
+  float *d_out = nullptr;  size_t out_cap = 0;       // bytes
+  float2 *d_minmax = nullptr; int *d_status = nullptr; int mm_cap = 0;  // slices
+  ctas::Item *d_items = nullptr; int items_cap = 0;
+  float *d_centres = nullptr; int centres_cap = 0;
+  double *d_partial = nullptr; float *d_tmp = nullptr; size_t score_cap = 0;
+  float *d_scores = nullptr;
+  int kept_slices = 0;         // filtered slices held for repeat()
+  int last_S = 0;
+
+  int tile_kind = ctas::TK_AUTO;
+  int angles_per_pass = 0;
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};

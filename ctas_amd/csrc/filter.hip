@@ -1,0 +1,375 @@
+// Sinogram preparation on the device: the deAbs clamp rule and -log, the
+// zero-padded FFT convolution of each detector row with the filter window,
+// and the REF prefix sum (CTrec::prepare_sino, lct.cpp:569-597).
+//
+// Layout: one workgroup filters two rows of one sinogram as the real and
+// imaginary parts of one complex line of zidth points held in LDS. The window
+// is real and even, so the two filtered rows separate exactly. The line is
+// loaded in bit-reversed order, transformed by a decimation-in-time FFT whose
+// radix-2 stages are fused in pairs (four points per thread per pass),
+// multiplied by the window in natural order, and transformed back by a
+// decimation-in-frequency FFT with conjugate twiddles, which leaves the
+// result in bit-reversed order again; the centre w samples are read out of
+// their bit-reversed positions. HBM traffic per row: w*4 bytes in, w*s out.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "ctas_internal.hpp"
+
+namespace ctas {
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+template <int ST>
+__device__ __forceinline__ void store_val(void *base, size_t i, float v) {
+  if constexpr (ST == ST_F32) {
+    static_cast<float *>(base)[i] = v;
+  } else if constexpr (ST == ST_F16) {
+    static_cast<_Float16 *>(base)[i] = static_cast<_Float16>(v);
+  } else {
+    static_cast<uint16_t *>(base)[i] = f32_to_bf16_rne(v);
+  }
+}
+
+__device__ __forceinline__ int find_item_by_pair(const Item *items, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].pair_base <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Partial (min, max) of every raw sinogram: grid (kMinMaxBlocks, n_items).
+__global__ __launch_bounds__(256) void minmax_kernel(const Item *__restrict__ items) {
+  const Item &it = items[blockIdx.y];
+  const float *p = it.sino;
+  const size_t n = static_cast<size_t>(it.N) * it.w;
+  float mn = FLT_MAX, mx = -FLT_MAX;
+  const size_t stride = static_cast<size_t>(blockDim.x) * gridDim.x;
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+    const float4 *p4 = reinterpret_cast<const float4 *>(p);
+    const size_t n4 = n >> 2;
+    for (size_t k = i; k < n4; k += stride) {
+      const float4 v = p4[k];
+      mn = fminf(mn, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+      mx = fmaxf(mx, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+    }
+    for (size_t k = (n4 << 2) + i; k < n; k += stride) {
+      mn = fminf(mn, p[k]);
+      mx = fmaxf(mx, p[k]);
+    }
+  } else {
+    for (size_t k = i; k < n; k += stride) {
+      mn = fminf(mn, p[k]);
+      mx = fmaxf(mx, p[k]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = fminf(mn, __shfl_xor(mn, o));
+    mx = fmaxf(mx, __shfl_xor(mx, o));
+  }
+  __shared__ float2 part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = make_float2(mn, mx);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float2 r = part[0];
+    for (int k = 1; k < 4; ++k) {
+      r.x = fminf(r.x, part[k].x);
+      r.y = fmaxf(r.y, part[k].y);
+    }
+    it.minmax[blockIdx.x] = r;
+  }
+}
+
+template <int ST>
+__global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ items, int n_items,
+                                                     int contrast) {
+  extern __shared__ float2 buf[];
+  __shared__ float s_mina;
+  __shared__ int s_clamp;
+  __shared__ float2 s_tot[256];
+
+  const int b = blockIdx.x;
+  const Item &it = items[find_item_by_pair(items, n_items, b)];
+  const int pair = b - it.pair_base;
+  const int w = it.w, N = it.N, Z = it.Z, logZ = it.logZ;
+  const int s0 = (Z - w) >> 1;
+  const int t0 = 2 * pair, t1 = t0 + 1;
+  const bool has1 = t1 < N;
+  const float *__restrict__ r0 = it.sino + static_cast<size_t>(t0) * w;
+  const float *__restrict__ r1 = r0 + w;
+  const float2 *__restrict__ tw = it.tw;
+  const int tid = threadIdx.x;
+
+  // deAbs (physical.world.cpp:88-100): clamp at max/1e6 only when the
+  // sinogram's minimum is not positive, then -log.
+  const bool absorb = (contrast == CTAS_CONTRAST_ABS);
+  if (absorb) {
+    if (tid < 64) {
+      const float2 m = it.minmax[tid];
+      float mn = m.x, mx = m.y;
+      for (int o = 32; o > 0; o >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, o));
+        mx = fmaxf(mx, __shfl_xor(mx, o));
+      }
+      if (tid == 0) {
+        const int clamp = (mn <= 0.0f);
+        const float mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
+        s_clamp = clamp;
+        s_mina = mina;
+        if (pair == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
+      }
+    }
+    __syncthreads();
+  }
+  const bool clamp = absorb && s_clamp;
+  const float mina = absorb ? s_mina : 0.0f;
+  auto deabs = [&](float v) {
+    if (clamp && v <= mina) v = mina;
+    return -logf(v);
+  };
+
+  // Padded line z[n], n in [0, Z): row samples at s0..s0+w-1 and, from the
+  // inclusive copy range of lct.cpp:580-584, the next row's first sample at
+  // s0+w (0 after the last row). Stored at bit-reversed positions.
+  for (int n = tid; n < Z; n += 256) {
+    const int x = n - s0;
+    float a = 0.0f, c = 0.0f;
+    if (x >= 0 && x < w) {
+      a = r0[x];
+      if (absorb) a = deabs(a);
+      if (has1) {
+        c = r1[x];
+        if (absorb) c = deabs(c);
+      }
+    } else if (x == w) {
+      if (t0 + 1 < N) {
+        a = r0[w];
+        if (absorb) a = deabs(a);
+      }
+      if (has1 && t1 + 1 < N) {
+        c = r1[w];
+        if (absorb) c = deabs(c);
+      }
+    }
+    buf[__brev(static_cast<unsigned>(n)) >> (32 - logZ)] = make_float2(a, c);
+  }
+  __syncthreads();
+
+  // Forward DIT, stages s = 1..logZ (len 2^s), fused in pairs.
+  for (int s = 1; s <= logZ;) {
+    if (s < logZ) {
+      const int h = 1 << (s - 1);
+      const int sh1 = logZ - s, sh2 = logZ - s - 1;
+      for (int g = tid; g < (Z >> 2); g += 256) {
+        const int i = g & (h - 1);
+        const int base = ((g >> (s - 1)) << (s + 1)) + i;
+        float2 x0 = buf[base], x1 = buf[base + h], x2 = buf[base + 2 * h], x3 = buf[base + 3 * h];
+        const float2 w1 = tw[i << sh1];
+        float2 y = cmul(x1, w1);
+        x1 = csub(x0, y);
+        x0 = cadd(x0, y);
+        y = cmul(x3, w1);
+        x3 = csub(x2, y);
+        x2 = cadd(x2, y);
+        const float2 w2 = tw[i << sh2];
+        const float2 w3 = tw[(i + h) << sh2];
+        y = cmul(x2, w2);
+        x2 = csub(x0, y);
+        x0 = cadd(x0, y);
+        y = cmul(x3, w3);
+        x3 = csub(x1, y);
+        x1 = cadd(x1, y);
+        buf[base] = x0;
+        buf[base + h] = x1;
+        buf[base + 2 * h] = x2;
+        buf[base + 3 * h] = x3;
+      }
+      s += 2;
+    } else {
+      const int h = 1 << (s - 1);
+      const int sh1 = logZ - s;
+      for (int g = tid; g < (Z >> 1); g += 256) {
+        const int i = g & (h - 1);
+        const int base = ((g >> (s - 1)) << s) + i;
+        const float2 x0 = buf[base];
+        const float2 y = cmul(buf[base + h], tw[i << sh1]);
+        buf[base] = cadd(x0, y);
+        buf[base + h] = csub(x0, y);
+      }
+      s += 1;
+    }
+    __syncthreads();
+  }
+
+  // Spectrum (natural order) times the real window.
+  for (int k = tid; k < Z; k += 256) {
+    const float wk = it.W[k];
+    float2 v = buf[k];
+    v.x *= wk;
+    v.y *= wk;
+    buf[k] = v;
+  }
+  __syncthreads();
+
+  // Inverse DIF, stages len = 2^s for s = logZ..1, fused in pairs; conjugate
+  // twiddles; output in bit-reversed order.
+  for (int s = logZ; s >= 1;) {
+    if (s >= 2) {
+      const int h = 1 << (s - 2);  // len 4h, then 2h
+      const int sh4 = logZ - s, sh2 = logZ - s + 1;
+      for (int g = tid; g < (Z >> 2); g += 256) {
+        const int i = g & (h - 1);
+        const int base = ((g >> (s - 2)) << s) + i;
+        float2 x0 = buf[base], x1 = buf[base + h], x2 = buf[base + 2 * h], x3 = buf[base + 3 * h];
+        float2 d = csub(x0, x2);
+        x0 = cadd(x0, x2);
+        x2 = cmulc(d, tw[i << sh4]);
+        d = csub(x1, x3);
+        x1 = cadd(x1, x3);
+        x3 = cmulc(d, tw[(i + h) << sh4]);
+        const float2 w2 = tw[i << sh2];
+        d = csub(x0, x1);
+        x0 = cadd(x0, x1);
+        x1 = cmulc(d, w2);
+        d = csub(x2, x3);
+        x2 = cadd(x2, x3);
+        x3 = cmulc(d, w2);
+        buf[base] = x0;
+        buf[base + h] = x1;
+        buf[base + 2 * h] = x2;
+        buf[base + 3 * h] = x3;
+      }
+      s -= 2;
+    } else {
+      const int h = 1;  // len 2, twiddle 1
+      for (int g = tid; g < (Z >> 1); g += 256) {
+        const int base = g << 1;
+        const float2 x0 = buf[base], x1 = buf[base + h];
+        buf[base] = cadd(x0, x1);
+        buf[base + h] = csub(x0, x1);
+      }
+      s -= 1;
+    }
+    __syncthreads();
+  }
+
+  const float invZ = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
+  const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
+  const unsigned rsh = 32 - logZ;
+  if (contrast != CTAS_CONTRAST_REF) {
+    for (int x = tid; x < w; x += 256) {
+      const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+      const float a = v.x * invZ, c = v.y * invZ;
+      store_val<ST>(it.filt, row0 + x, a);
+      if (it.filt_f32) it.filt_f32[row0 + x] = a;
+      if (has1) {
+        store_val<ST>(it.filt, row1 + x, c);
+        if (it.filt_f32) it.filt_f32[row1 + x] = c;
+      }
+    }
+    return;
+  }
+  // REF (lct.cpp:590-595): prefix sum along each filtered row; contiguous
+  // chunks per thread, then a scan of the chunk totals.
+  const int cs = (w + 255) / 256;
+  const int xb = tid * cs, xe = min(w, xb + cs);
+  float2 acc = make_float2(0.0f, 0.0f);
+  for (int x = xb; x < xe; ++x) {
+    const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+    acc.x += v.x * invZ;
+    acc.y += v.y * invZ;
+  }
+  s_tot[tid] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    float2 run = make_float2(0.0f, 0.0f);
+    for (int k = 0; k < 256; ++k) {
+      const float2 t = s_tot[k];
+      s_tot[k] = run;
+      run.x += t.x;
+      run.y += t.y;
+    }
+  }
+  __syncthreads();
+  float2 run = s_tot[tid];
+  for (int x = xb; x < xe; ++x) {
+    const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+    run.x += v.x * invZ;
+    run.y += v.y * invZ;
+    store_val<ST>(it.filt, row0 + x, run.x);
+    if (it.filt_f32) it.filt_f32[row0 + x] = run.x;
+    if (has1) {
+      store_val<ST>(it.filt, row1 + x, run.y);
+      if (it.filt_f32) it.filt_f32[row1 + x] = run.y;
+    }
+  }
+}
+
+// FLT contrast: no filtering, only the storage conversion (and float copy).
+template <int ST>
+__global__ __launch_bounds__(256) void passthrough_kernel(const Item *__restrict__ items,
+                                                          int n_items) {
+  const int b = blockIdx.x;
+  const Item &it = items[find_item_by_pair(items, n_items, b)];
+  const int pair = b - it.pair_base;
+  const int w = it.w, N = it.N;
+  const int t0 = 2 * pair;
+  const int rows = (t0 + 1 < N) ? 2 : 1;
+  const size_t base = static_cast<size_t>(t0) * w;
+  for (int x = threadIdx.x; x < rows * w; x += 256) {
+    const float v = it.sino[base + x];
+    store_val<ST>(it.filt, base + x, v);
+    if (it.filt_f32) it.filt_f32[base + x] = v;
+  }
+  if (pair == 0 && threadIdx.x == 0) *it.status = 0;
+}
+
+}  // namespace
+
+hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s) {
+  dim3 grid(kMinMaxBlocks, n_items);
+  hipLaunchKernelGGL(minmax_kernel, grid, dim3(256), 0, s, d_items);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int max_logZ,
+                         int contrast, int store, hipStream_t s) {
+  if (total_pairs <= 0) return hipSuccess;
+  if (contrast == CTAS_CONTRAST_FLT) {
+    switch (store) {
+      case ST_F32: hipLaunchKernelGGL(passthrough_kernel<ST_F32>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items); break;
+      case ST_F16: hipLaunchKernelGGL(passthrough_kernel<ST_F16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items); break;
+      default: hipLaunchKernelGGL(passthrough_kernel<ST_BF16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items); break;
+    }
+    return hipGetLastError();
+  }
+  const size_t lds = sizeof(float2) << max_logZ;
+  switch (store) {
+    case ST_F32: hipLaunchKernelGGL(filter_kernel<ST_F32>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
+    case ST_F16: hipLaunchKernelGGL(filter_kernel<ST_F16>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
+    default: hipLaunchKernelGGL(filter_kernel<ST_BF16>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ctas

@@ -1,0 +1,191 @@
+/*
+ * ctas_hip.h - C ABI of the MI355X filtered-backprojection engine for CTAS.
+ *
+ * One handle per reconstruction geometry, mirroring the reference's CTrec
+ * (src/common/ct.h:235-298). All functions return 0 on success and a
+ * negative CTAS_E_* code on failure; ctas_last_error() then holds a
+ * thread-local message. Host pointers are row-major float32: sinograms
+ * [S][N][w] (N angles x w detector pixels per slice), slices [S][w][w]
+ * (row j, column i). Functions with a _dev suffix take device pointers and a
+ * hipStream_t passed as void* (NULL = the handle's own stream) and return
+ * without synchronising. A handle is used by one thread at a time, as a
+ * CTrec is; clones share nothing mutable.
+ */
+#ifndef CTAS_HIP_H
+#define CTAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ctas_fbp ctas_fbp;
+
+/* error codes */
+#define CTAS_OK 0
+#define CTAS_E_ARG -1      /* bad argument (reference: throw_error)          */
+#define CTAS_E_SHAPE -2    /* array shape does not match the handle         */
+#define CTAS_E_CENTER -3   /* |c| >= w/2 (lct.cpp:614-616, 651-653)          */
+#define CTAS_E_DATA -4     /* deAbs: maximum not positive (physical.world.cpp:94) */
+#define CTAS_E_DEVICE -5   /* HIP runtime error or no device                */
+#define CTAS_E_NOMEM -6    /* device allocation failed                      */
+#define CTAS_E_STATE -7    /* call order (e.g. repeat before sino)           */
+#define CTAS_E_AXIS -8     /* axis search failed (lct.cpp:1136-1210 throws)  */
+
+/* contrast codes = Contrast::Cntype (physical.world.h:22-27) */
+#define CTAS_CONTRAST_ABS 0
+#define CTAS_CONTRAST_REF 1
+#define CTAS_CONTRAST_PHS 2
+#define CTAS_CONTRAST_FLT 3
+
+/* filter codes = Filter::Ftype (ct.h:95-146) */
+#define CTAS_FILTER_NONE 0
+#define CTAS_FILTER_RAMP 1
+#define CTAS_FILTER_BARLETT 2
+#define CTAS_FILTER_WELCH 3
+#define CTAS_FILTER_PARZEN 4
+#define CTAS_FILTER_HANN 5
+#define CTAS_FILTER_HAMMING 6
+#define CTAS_FILTER_BLACKMAN 7
+#define CTAS_FILTER_LANCKZOS 8
+#define CTAS_FILTER_KAISER 9
+#define CTAS_FILTER_GAUSS 10
+
+/* flags for ctas_fbp_create */
+#define CTAS_INTERP_CTAS 0x0   /* floor sampling in the reference's float arithmetic (parity) */
+#define CTAS_INTERP_LINEAR 0x1 /* linear interpolation between bin centres           */
+#define CTAS_INTERP_MASK 0x3
+#define CTAS_STORE_FP32 0x00   /* filtered sinogram stored as float32             */
+#define CTAS_STORE_FP16 0x10   /* ... as IEEE half, accumulated in float32         */
+#define CTAS_STORE_BF16 0x20   /* ... as bfloat16, accumulated in float32          */
+#define CTAS_STORE_MASK 0x30
+
+/* Thread-local message of the last failure on this thread. */
+const char *ctas_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int ctas_device_count(int *n);
+/* Library version string. */
+const char *ctas_version(void);
+
+/* ---- geometry helpers (no device needed) ---------------------------- */
+/* zidth = 2^ceil(log2(2w-1)) (lct.cpp:490) */
+int ctas_zidth(int w);
+/* arcO / angO (lct.cpp:470-484): arc or step in degrees -> radians */
+float ctas_arc_rad(float anarc, int points);
+float ctas_step_rad(float anarc, int points);
+/* Filter::fill + CTrec gain/REF rule (lct.cpp:122-209, 505-508): W[Z] */
+int ctas_filter_window(int filter, float param, int w, int contrast, float *W);
+/* Filter::fromName (lct.cpp:71-86) and _conversion "NAME[:param]"
+ * (lct.cpp:236-248): name -> code and parameter. */
+int ctas_filter_parse(const char *spec, int *filter, float *param);
+/* Contrast(const std::string&) (physical.world.cpp:19-27). */
+int ctas_contrast_parse(const char *name, int *contrast);
+/* The float angle table of lct.cpp:512-517 ({sin, cos} per angle). */
+int ctas_angles(int n, float anarc, float *sincos);
+
+/* ---- reconstruction handle (CTrec) ------------------------------------ */
+/* CTrec(sinoshape=(n_angles, w), contrast, anarc, filter)
+ * (lct.cpp:487-526). Throws there for w < 2 or n_angles == 0. */
+int ctas_fbp_create(ctas_fbp **h, int w, int n_angles, int contrast, int filter,
+                    float filter_param, float anarc, int flags);
+/* CTrec(CTrec&) (lct.cpp:529-545): same geometry, own device buffers. */
+int ctas_fbp_clone(const ctas_fbp *src, ctas_fbp **h);
+/* ~CTrec (lct.cpp:549-555) */
+int ctas_fbp_destroy(ctas_fbp *h);
+int ctas_fbp_info(const ctas_fbp *h, int *w, int *n_angles, int *zidth, int *flags);
+/* CTrec::setPhysics (lct.cpp:558-566): pixel size and wavelength in m. */
+int ctas_fbp_set_physics(ctas_fbp *h, float pixel_size, float lambda);
+int ctas_fbp_get_reccof(const ctas_fbp *h, float *recCof);
+/* Replace / read the {sin, cos} float table (2*N floats). */
+int ctas_fbp_set_angles(ctas_fbp *h, const float *sincos);
+int ctas_fbp_get_angles(const ctas_fbp *h, float *sincos);
+int ctas_fbp_get_window(const ctas_fbp *h, float *W);
+
+/* CTrec::prepare_sino on S host slices, in place (lct.cpp:569-597). */
+int ctas_fbp_prepare(ctas_fbp *h, float *sino, int S);
+/* CTrec::sino (lct.cpp:600-609): prepare in place and keep the filtered
+ * sinogram on the device for ctas_fbp_repeat / ctas_axis_scores. */
+int ctas_fbp_sino(ctas_fbp *h, float *sino);
+/* CTrec::repeat (lct.cpp:612-646) for n_centers centres at once:
+ * out[n_centers][w][w]. */
+int ctas_fbp_repeat(ctas_fbp *h, float *out, const float *centers, int n_centers);
+/* CTrec::reconstruct (lct.cpp:649-685) for S slices, one centre per slice:
+ * sino[S][N][w] is filtered in place, out[S][w][w]. */
+int ctas_fbp_reconstruct(ctas_fbp *h, float *sino, float *out, const float *centers, int S);
+
+/* Device-resident variants (inputs already in HBM; the input sinogram is not
+ * modified; filtered rows go to the handle's storage-dtype buffer). */
+int ctas_fbp_reconstruct_dev(ctas_fbp *h, const float *d_sino, float *d_out,
+                             const float *centers, int S, void *stream);
+/* Backproject the device-resident filtered sinogram kept by ctas_fbp_sino
+ * (or by the last _dev reconstruct, slice `slice`) at n centres. */
+int ctas_fbp_repeat_dev(ctas_fbp *h, int slice, float *d_out, const float *centers,
+                        int n_centers, void *stream);
+/* The two stages of reconstruct_dev on their own: prepare S device slices
+ * into the handle's filtered buffer (CTrec::prepare_sino, lct.cpp:569-597),
+ * and backproject those S filtered slices, centre i for slice i (the kernel
+ * launch of ForCLdev::reconstruct, lct.cpp:406-409). */
+int ctas_fbp_prepare_dev(ctas_fbp *h, const float *d_sino, int S, void *stream);
+int ctas_fbp_backproject_dev(ctas_fbp *h, float *d_out, const float *centers, int S, void *stream);
+/* Capture reconstruct_dev(d_sino, d_out, centers, S) into a hipGraph owned
+ * by the handle, then replay it. No memset nodes are recorded. */
+int ctas_fbp_graph_capture(ctas_fbp *h, const float *d_sino, float *d_out,
+                           const float *centers, int S, void *stream);
+int ctas_fbp_graph_launch(ctas_fbp *h, void *stream);
+/* Last per-slice deAbs status of a _dev call (0 ok, CTAS_E_DATA bad data);
+ * synchronises the handle's stream. */
+int ctas_fbp_status(ctas_fbp *h, int S, int *status);
+
+/* Tile choice for this handle's width class: rows x cols of output per
+ * workgroup and angles staged per LDS pass; 0 = let the autotuner decide. */
+int ctas_fbp_set_tiles(ctas_fbp *h, int tile_kind, int angles_per_pass);
+int ctas_fbp_get_tiles(const ctas_fbp *h, int *tile_kind, int *angles_per_pass);
+/* Time the candidate tiles on the current device for this handle's shape
+ * and keep the fastest (cached per width class in the process). */
+int ctas_fbp_autotune(ctas_fbp *h, int S);
+
+/* ---- uneven batches in one launch ------------------------------------- */
+typedef struct {
+  int w, n_angles;        /* shape of this sinogram                       */
+  float center;           /* centre offset c                              */
+  const float *d_sino;    /* device [N][w] float32 (unfiltered)           */
+  float *d_out;           /* device [w][w] float32                        */
+} ctas_batch_item;
+/* Filters and backprojects n items of different widths and angle counts
+ * with one persistent backprojection launch (contrast, filter, arc and
+ * flags shared). */
+int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int filter,
+                          float filter_param, float anarc, int flags, void *stream);
+
+/* ---- ring filter (RingFilter::apply, lct.cpp:295-327) ----------------- */
+int ctas_ring_filter(float *sino, int N, int w, int box);
+int ctas_ring_filter_dev(float *d_sino, int S, int N, int w, int box, void *stream);
+
+/* ---- rotation axis (lct.cpp:804-1326, axis.cpp:116-208) -------------- */
+/* AverageGradient::apply for n trial axis positions axisR (pixels from the
+ * left edge) on the sinogram kept by ctas_fbp_sino of an ABS/RAMP handle. */
+int ctas_axis_scores(ctas_fbp *h, const float *axisR, int n, float *scores);
+/* raxis(sino, anarc, maxDev, algo) (lct.cpp:1133-1212): axis position. */
+int ctas_raxis(const float *sino, int N, int w, float anarc, float maxDev, int algo,
+               float *result);
+/* raxis(proj0, proj180, maxDev) (lct.cpp:1217-1294). */
+int ctas_raxis_pair(const float *p0, const float *p180, int rows, int w, float maxDev,
+                    float *result);
+
+/* ---- synthetic ellipse phantoms ---------------------------------------- */
+/* n_ell random ellipses (x0, y0, a, b, phi, mu per pixel) for width w: centres
+ * uniform in the inner 80% of the disc, semi-axes uniform in
+ * [5, 200]*axis_scale px, phi uniform in [0, pi), densities uniform in
+ * [-0.5, 1] scaled by 2/w per pixel. Seeded (SplitMix64). */
+int ctas_phantom_ellipses(int n_ell, int w, double axis_scale, uint64_t seed, double *ell);
+/* Analytic line integrals at bin centres (FP64 on the device); with N0 > 0
+ * the noisy transmission counts/N0; gain (optional, [w]) multiplies it. */
+int ctas_phantom_sino_dev(int N, int w, double c, int n_ell, const double *ell, double N0,
+                          uint64_t seed, const float *d_gain, float *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""GPU parity of the HIP engine against the CPU oracle, through the C ABI.
+
+Bars: bit-exact where the arithmetic is the reference's float arithmetic on
+identical inputs (CTAS-mode backprojection of a given filtered sinogram, ring
+filter); 1e-4 relative RMS for the full CTAS path (the FFT filter differs from
+the oracle's FP64 FFT at ~1e-7); LINEAR mode against the FP64 oracle in the
+same mode; FP16/BF16 storage at 2e-3 relative RMS against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import ctas_amd as ca
+import oracle
+from tests.conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if ca.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def _phantom(N, w, c=0.0, n_ell=10, N0=1e5, seed=1, scale=1.0):
+    ell = oracle.random_ellipses(n_ell, w, scale, seed)
+    return oracle.phantom_sino(N, w, c, ell, N0, seed)
+
+
+@pytest.mark.parametrize("kind", [ca.TILE_64x64, ca.TILE_128x64, ca.TILE_128x128])
+@pytest.mark.parametrize("c", [0.0, 3.37, -7.5])
+def test_bp_ctas_bit_exact(kind, c):
+    """FLT contrast: the backprojection alone, bit-identical to the oracle."""
+    N, w = 180, 256
+    rng = np.random.default_rng(7)
+    P = rng.standard_normal((N, w)).astype(np.float32)
+    rec = ca.CTrec(w, N, ca.FLT, "RAMP")
+    rec.set_tiles(kind, 16 if kind == ca.TILE_64x64 else 32)
+    _, out = rec.reconstruct(P, c)
+    ref = oracle.bp_ctas(P, oracle.angles(N), c)
+    assert np.array_equal(out[0], ref), f"max diff {np.abs(out[0] - ref).max()}"
+
+
+def test_bp_ctas_odd_width_and_angles():
+    N, w = 77, 201
+    rng = np.random.default_rng(3)
+    P = rng.standard_normal((N, w)).astype(np.float32)
+    rec = ca.CTrec(w, N, ca.FLT, "RAMP")
+    _, out = rec.reconstruct(P, 1.25)
+    assert np.array_equal(out[0], oracle.bp_ctas(P, oracle.angles(N), 1.25))
+
+
+@pytest.mark.parametrize("contrast", [ca.ABS, ca.PHS, ca.REF])
+@pytest.mark.parametrize("filt", ["RAMP", "HANN", "HAMMING", "PARZEN", "KAISER:2.5",
+                                  "GAUSS:0.4", "LANCKZOS", "BLACKMAN", "WELCH", "NONE"])
+def test_prepare_matches_oracle(contrast, filt):
+    N, w = 64, 300
+    P = _phantom(N, w, n_ell=6, seed=5)
+    if contrast != ca.ABS:
+        P = -np.log(np.maximum(P, 1e-3)).astype(np.float32)
+    rec = ca.CTrec(w, N, contrast, filt)
+    got = rec.prepare(P)
+    f, p = ca.parse_filter(filt)
+    W = oracle.filter_window(f, p, w, contrast)
+    assert np.array_equal(W, rec.window()) or np.abs(W - rec.window()).max() <= 1e-6 * np.abs(W).max()
+    ref = oracle.prepare(P, W, contrast)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 2e-5 * scale
+
+
+def test_config1_end_to_end():
+    """512 px x 720 angles, 10 ellipses, Poisson 1e5, RAMP, c = 0 (config 1)."""
+    N, w = 720, 512
+    P = _phantom(N, w, 0.0, 10, 1e5, seed=11)
+    rec = ca.CTrec(w, N, ca.ABS, "RAMP")
+    filt, out = rec.reconstruct(P, 0.0)
+    ref_f, ref = oracle.fbp_batch(P, [0.0], ca.ABS, 1, 0.0, 180.0)
+    assert rel_rms(filt, ref_f[0]) <= 1e-6
+    assert rel_rms(out[0], ref[0]) <= 1e-4
+
+
+def test_hann_offset_centre_batch():
+    """Config-2 geometry at reduced size: HANN, c = +3.37, several slices."""
+    N, w, S = 360, 512, 3
+    P = np.stack([_phantom(N, w, 3.37, 20, 1e5, seed=20 + s) for s in range(S)])
+    rec = ca.CTrec(w, N, ca.ABS, "HANN")
+    _, out = rec.reconstruct(P, [3.37] * S)
+    _, ref = oracle.fbp_batch(P, [3.37] * S, ca.ABS, 5, 0.0, 180.0, threads=3)
+    for s in range(S):
+        assert rel_rms(out[s], ref[s]) <= 1e-4
+
+
+def test_linear_mode_vs_f64():
+    N, w = 240, 256
+    P = _phantom(N, w, 2.0, 8, 0.0, seed=9)  # line integrals
+    rec = ca.CTrec(w, N, ca.FLT, "RAMP", interp=ca.INTERP_LINEAR)
+    _, out = rec.reconstruct(P, 2.0)
+    ref = oracle.bp_f64(P, 2.0, np.pi, 1.0, linear=True)
+    assert rel_rms(out[0], ref) <= 1e-4
+
+
+@pytest.mark.parametrize("store", [ca.STORE_FP16, ca.STORE_BF16])
+@pytest.mark.parametrize("interp", [ca.INTERP_LINEAR, ca.INTERP_CTAS])
+def test_reduced_precision_storage(store, interp):
+    N, w = 360, 384
+    P = _phantom(N, w, 0.0, 12, 1e5, seed=13)
+    rec = ca.CTrec(w, N, ca.ABS, "HANN", interp=interp, store=store)
+    _, out = rec.reconstruct(P, 0.0)
+    W = oracle.filter_window(5, 0.0, w, ca.ABS)
+    Pf = oracle.prepare(P, W, ca.ABS)
+    ref = oracle.bp_f64(Pf, 0.0, np.pi, 1.0, linear=(interp == ca.INTERP_LINEAR))
+    err = rel_rms(out[0], ref)
+    print(f"store={store:#x} interp={interp} rel_rms={err:.3e}")
+    assert err <= 2e-3
+
+
+def test_repeat_matches_reconstruct():
+    N, w = 180, 256
+    P = _phantom(N, w, 0.0, 6, 1e5, seed=3)
+    rec = ca.CTrec(w, N, ca.ABS, "RAMP")
+    rec.sino(P)
+    cs = np.array([-4.5, 0.0, 2.25, 10.0], np.float32)
+    many = rec.repeat(cs)
+    for k, c in enumerate(cs):
+        _, one = ca.CTrec(w, N, ca.ABS, "RAMP").reconstruct(P, c)
+        assert np.array_equal(many[k], one[0])
+
+
+def test_recof_physics():
+    N, w = 90, 128
+    P = np.full((N, w), 0.25, np.float32)
+    rec = ca.CTrec(w, N, ca.FLT, "RAMP")
+    rec.set_physics(1e-6, 1e-10)
+    _, out = rec.reconstruct(P, 0.0)
+    r = rec.rec_cof()
+    ref = oracle.bp_ctas(P, oracle.angles(N), 0.0, r)
+    assert np.array_equal(out[0], ref)
+    inside = ref != 0
+    assert np.allclose(out[0][inside], 0.25 * r, rtol=1e-6)
+
+
+def test_bad_absorption_data():
+    rec = ca.CTrec(64, 16, ca.ABS, "RAMP")
+    with pytest.raises(ca.CtasError) as e:
+        rec.reconstruct(-np.ones((16, 64), np.float32), 0.0)
+    assert e.value.code == ca.CTAS_E_DATA
+
+
+def test_centre_out_of_range():
+    rec = ca.CTrec(64, 16, ca.ABS, "RAMP")
+    with pytest.raises(ca.CtasError) as e:
+        rec.reconstruct(np.ones((16, 64), np.float32), 32.0)
+    assert e.value.code == ca.CTAS_E_CENTER
+
+
+def test_device_graph_and_streams(torch_cuda):
+    torch = torch_cuda
+    N, w, S = 180, 256, 4
+    P = np.stack([_phantom(N, w, 1.5, 8, 1e5, seed=40 + s) for s in range(S)])
+    d_in = torch.from_numpy(P).cuda()
+    d_out = torch.zeros((S, w, w), dtype=torch.float32, device="cuda")
+    rec = ca.CTrec(w, N, ca.ABS, "HANN")
+    rec.reconstruct_dev(d_in, d_out, [1.5] * S)
+    torch.cuda.synchronize()
+    eager = d_out.cpu().numpy().copy()
+    _, host = ca.CTrec(w, N, ca.ABS, "HANN").reconstruct(P, [1.5] * S)
+    assert np.array_equal(eager, host)
+    rec.graph_capture(d_in, d_out, [1.5] * S)
+    for rep in range(3):
+        d_out.zero_()
+        torch.cuda.synchronize()
+        rec.graph_launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(d_out.cpu().numpy(), eager), f"replay {rep}"
+    assert (rec.status(S) == 0).all()
+
+
+def test_uneven_batch_persistent(torch_cuda):
+    torch = torch_cuda
+    shapes = [(360, 512), (90, 768), (180, 256), (240, 1024), (45, 300)]
+    items, refs = [], []
+    for k, (N, w) in enumerate(shapes):
+        c = [0.0, 1.5, -2.0, 3.37, 0.5][k]
+        P = _phantom(N, w, c, 6, 1e5, seed=60 + k)
+        d_in = torch.from_numpy(P).cuda()
+        d_out = torch.zeros((w, w), dtype=torch.float32, device="cuda")
+        items.append((d_in, d_out, c))
+        _, ref = ca.CTrec(w, N, ca.ABS, "RAMP").reconstruct(P, c)
+        refs.append(ref[0])
+    ca.batch_uneven(items, ca.ABS, 1, 0.0, 180.0, 0)
+    torch.cuda.synchronize()
+    for (d_in, d_out, c), ref in zip(items, refs):
+        assert np.array_equal(d_out.cpu().numpy(), ref)
+
+
+def test_ring_filter_bit_exact():
+    rng = np.random.default_rng(2)
+    P = rng.random((50, 97)).astype(np.float32)
+    for box in (1, 3, 8):
+        assert np.array_equal(ca.ring_filter(P, box), oracle.ring_filter(P, box))
+
+
+def _dots(w, n=8, seed=0):
+    rng = np.random.default_rng(seed)
+    return np.array([[x, y, 2.0, 2.0, 0.0, 2.0 / w] for x, y in rng.uniform(-0.31 * w, 0.31 * w, (n, 2))])
+
+
+def test_axis_scores_vs_oracle():
+    N, w = 361, 192     # sigma = w/(N-1) < 1: no smoothing
+    P = _phantom(N, w, 0.0, 8, 1e5, seed=70)
+    for N2, w2, P2 in ((N, w, P), (241, 256, _phantom(241, 256, 1.0, 8, 1e5, seed=72))):
+        step = 180.0 / (N2 - 1)
+        rec = ca.CTrec(w2, N2, ca.ABS, "RAMP", anarc=step)
+        rec.sino(P2)
+        axes = np.array([0.5 * (w2 - 1) + d for d in (-6.0, -1.5, 0.0, 2.25, 5.0)], np.float32)
+        got = rec.axis_scores(axes)
+        W = oracle.filter_window(1, 0.0, w2, ca.ABS)
+        Pf = oracle.prepare(P2, W, ca.ABS)
+        sigma = np.float32(oracle.angO(step, N2) * np.float32(w2) / np.pi)
+        ref = np.array([oracle.ag_score(Pf, oracle.angles(N2, step), float(sigma), float(a)) for a in axes])
+        assert np.allclose(got, ref, rtol=2e-4), (N2, w2, got, ref)
+
+
+@pytest.mark.parametrize("c0", [3.0, -4.0])
+def test_raxis_matches_oracle(c0):
+    N, w = 241, 128
+    P = np.exp(-oracle.phantom_sino(N, w, c0, _dots(w), 0.0, 0)).astype(np.float32)
+    step = 180.0 / (N - 1)
+    for algo in (0, 1, 3):
+        got = ca.raxis(P, step, 0.25, algo)
+        ref = oracle.raxis(P, step, 0.25, algo)
+        assert abs(got - ref) <= 0.25, (algo, got, ref)
+    assert abs(ca.raxis(P, step, 0.25, 1) - 0.5 * (w - 1) - c0) <= 0.25
