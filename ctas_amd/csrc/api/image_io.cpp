@@ -1,0 +1,215 @@
+#include "image_io.h"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace ctas {
+namespace {
+
+bool ends_with(const std::string &s, const std::string &e) {
+  if (s.size() < e.size()) return false;
+  std::string t = s.substr(s.size() - e.size());
+  for (char &c : t) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+  return t == e;
+}
+
+std::vector<unsigned char> slurp(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw_error("image", "Could not open file \"" + path + "\".");
+  return std::vector<unsigned char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+struct Reader {
+  const std::vector<unsigned char> &b;
+  bool be;
+  uint32_t u16(size_t o) const {
+    if (o + 2 > b.size()) throw_error("image", "truncated TIFF");
+    return be ? (b[o] << 8 | b[o + 1]) : (b[o] | b[o + 1] << 8);
+  }
+  uint32_t u32(size_t o) const {
+    if (o + 4 > b.size()) throw_error("image", "truncated TIFF");
+    return be ? (uint32_t(b[o]) << 24 | uint32_t(b[o + 1]) << 16 | uint32_t(b[o + 2]) << 8 | b[o + 3])
+              : (b[o] | uint32_t(b[o + 1]) << 8 | uint32_t(b[o + 2]) << 16 | uint32_t(b[o + 3]) << 24);
+  }
+};
+
+Map read_tiff(const std::string &path) {
+  const std::vector<unsigned char> b = slurp(path);
+  if (b.size() < 8 || !((b[0] == 'I' && b[1] == 'I') || (b[0] == 'M' && b[1] == 'M')))
+    throw_error("image", "\"" + path + "\" is not a TIFF file.");
+  Reader r{b, b[0] == 'M'};
+  if (r.u16(2) != 42) throw_error("image", "\"" + path + "\": unsupported TIFF variant.");
+  const size_t ifd = r.u32(4);
+  const uint32_t n = r.u16(ifd);
+  uint32_t width = 0, height = 0, bps = 0, comp = 1, spp = 1, fmt = 1, rps = 0;
+  std::vector<uint32_t> offs, counts;
+  auto values = [&](size_t e, std::vector<uint32_t> &out) {
+    const uint32_t type = r.u16(e + 2), cnt = r.u32(e + 4);
+    const size_t sz = type == 3 ? 2 : 4;
+    size_t at = cnt * sz <= 4 ? e + 8 : r.u32(e + 8);
+    out.resize(cnt);
+    for (uint32_t k = 0; k < cnt; ++k) out[k] = sz == 2 ? r.u16(at + 2 * k) : r.u32(at + 4 * k);
+  };
+  for (uint32_t k = 0; k < n; ++k) {
+    const size_t e = ifd + 2 + 12 * k;
+    const uint32_t tag = r.u16(e);
+    std::vector<uint32_t> v;
+    values(e, v);
+    if (v.empty()) continue;
+    switch (tag) {
+      case 256: width = v[0]; break;
+      case 257: height = v[0]; break;
+      case 258: bps = v[0]; break;
+      case 259: comp = v[0]; break;
+      case 273: offs = v; break;
+      case 277: spp = v[0]; break;
+      case 278: rps = v[0]; break;
+      case 279: counts = v; break;
+      case 339: fmt = v[0]; break;
+      default: break;
+    }
+  }
+  if (bps != 32 || fmt != 3 || comp != 1 || spp != 1)
+    throw_error("image", "\"" + path + "\": only uncompressed single-channel float32 TIFF is supported.");
+  (void)rps;
+  Map img(height, width);
+  const size_t need = static_cast<size_t>(width) * height * 4;
+  std::vector<unsigned char> raw;
+  raw.reserve(need);
+  for (size_t s = 0; s < offs.size() && raw.size() < need; ++s) {
+    const size_t len = s < counts.size() ? counts[s] : need - raw.size();
+    if (offs[s] + len > b.size()) throw_error("image", "\"" + path + "\": truncated strip.");
+    raw.insert(raw.end(), b.begin() + offs[s], b.begin() + offs[s] + len);
+  }
+  if (raw.size() < need) throw_error("image", "\"" + path + "\": missing image data.");
+  for (size_t i = 0; i < static_cast<size_t>(width) * height; ++i) {
+    uint32_t u = r.be ? (uint32_t(raw[4 * i]) << 24 | uint32_t(raw[4 * i + 1]) << 16 | uint32_t(raw[4 * i + 2]) << 8 | raw[4 * i + 3])
+                      : (raw[4 * i] | uint32_t(raw[4 * i + 1]) << 8 | uint32_t(raw[4 * i + 2]) << 16 | uint32_t(raw[4 * i + 3]) << 24);
+    std::memcpy(img.data() + i, &u, 4);
+  }
+  return img;
+}
+
+std::vector<Map> read_npy(const std::string &path) {
+  const std::vector<unsigned char> b = slurp(path);
+  if (b.size() < 10 || std::memcmp(b.data(), "\x93NUMPY", 6) != 0) throw_error("image", "\"" + path + "\" is not a .npy file.");
+  const int major = b[6];
+  size_t hlen, hoff;
+  if (major == 1) {
+    hlen = b[8] | (b[9] << 8);
+    hoff = 10;
+  } else {
+    hlen = b[8] | (b[9] << 8) | (size_t(b[10]) << 16) | (size_t(b[11]) << 24);
+    hoff = 12;
+  }
+  const std::string h(reinterpret_cast<const char *>(b.data()) + hoff, hlen);
+  if (h.find("'<f4'") == std::string::npos) throw_error("image", "\"" + path + "\": only little-endian float32 .npy is supported.");
+  if (h.find("'fortran_order': True") != std::string::npos) throw_error("image", "\"" + path + "\": Fortran order is not supported.");
+  const size_t a = h.find('('), z = h.find(')');
+  std::vector<long> dims;
+  std::stringstream ss(h.substr(a + 1, z - a - 1));
+  std::string tok;
+  while (std::getline(ss, tok, ','))
+    if (tok.find_first_not_of(" ") != std::string::npos) dims.push_back(std::stol(tok));
+  if (dims.size() == 2) dims.insert(dims.begin(), 1);
+  if (dims.size() != 3) throw_error("image", "\"" + path + "\": expected a 2-D or 3-D array.");
+  const size_t plane = static_cast<size_t>(dims[1]) * dims[2];
+  const size_t data = hoff + hlen;
+  if (data + plane * dims[0] * 4 > b.size()) throw_error("image", "\"" + path + "\": truncated data.");
+  std::vector<Map> out(dims[0]);
+  for (long s = 0; s < dims[0]; ++s) {
+    out[s].resize(dims[1], dims[2]);
+    std::memcpy(out[s].data(), b.data() + data + plane * s * 4, plane * 4);
+  }
+  return out;
+}
+
+void put16(std::vector<unsigned char> &v, uint32_t x) {
+  v.push_back(x & 255);
+  v.push_back((x >> 8) & 255);
+}
+void put32(std::vector<unsigned char> &v, uint32_t x) {
+  for (int k = 0; k < 4; ++k) v.push_back((x >> (8 * k)) & 255);
+}
+
+}  // namespace
+
+std::vector<Map> read_images(const std::string &path) {
+  if (ends_with(path, ".npy")) return read_npy(path);
+  return {read_tiff(path)};
+}
+
+void write_tiff(const std::string &path, const Map &img) {
+  const uint32_t w = static_cast<uint32_t>(img.columns()), h = static_cast<uint32_t>(img.rows());
+  std::vector<unsigned char> v;
+  v.insert(v.end(), {'I', 'I'});
+  put16(v, 42);
+  put32(v, 8);
+  const uint16_t n = 10;
+  const uint32_t data_off = 8 + 2 + 12 * n + 4;
+  put16(v, n);
+  auto entry = [&](uint16_t tag, uint16_t type, uint32_t cnt, uint32_t val) {
+    put16(v, tag);
+    put16(v, type);
+    put32(v, cnt);
+    if (type == 3 && cnt == 1) {
+      put16(v, val);
+      put16(v, 0);
+    } else {
+      put32(v, val);
+    }
+  };
+  entry(256, 4, 1, w);
+  entry(257, 4, 1, h);
+  entry(258, 3, 1, 32);
+  entry(259, 3, 1, 1);
+  entry(262, 3, 1, 1);
+  entry(273, 4, 1, data_off);
+  entry(277, 3, 1, 1);
+  entry(278, 4, 1, h);
+  entry(279, 4, 1, w * h * 4);
+  entry(339, 3, 1, 3);
+  put32(v, 0);
+  const size_t at = v.size();
+  v.resize(at + static_cast<size_t>(w) * h * 4);
+  std::memcpy(v.data() + at, img.data(), static_cast<size_t>(w) * h * 4);
+  std::ofstream f(path, std::ios::binary);
+  if (!f) throw_error("image", "Could not write file \"" + path + "\".");
+  f.write(reinterpret_cast<const char *>(v.data()), static_cast<std::streamsize>(v.size()));
+}
+
+void write_npy(const std::string &path, const std::vector<Map> &stack) {
+  if (stack.empty()) throw_error("image", "nothing to write");
+  const long r = stack[0].rows(), c = stack[0].columns();
+  std::string h = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(stack.size()) + ", " +
+                  std::to_string(r) + ", " + std::to_string(c) + "), }";
+  while ((10 + h.size() + 1) % 64) h += ' ';
+  h += '\n';
+  std::ofstream f(path, std::ios::binary);
+  if (!f) throw_error("image", "Could not write file \"" + path + "\".");
+  f.write("\x93NUMPY\x01\x00", 8);
+  const unsigned char hl[2] = {static_cast<unsigned char>(h.size() & 255), static_cast<unsigned char>(h.size() >> 8)};
+  f.write(reinterpret_cast<const char *>(hl), 2);
+  f.write(h.data(), static_cast<std::streamsize>(h.size()));
+  for (const Map &m : stack) f.write(reinterpret_cast<const char *>(m.data()), static_cast<std::streamsize>(m.size() * 4));
+}
+
+std::string mask_name(const std::string &mask, long index, long count) {
+  if (count <= 1 && mask.find('@') == std::string::npos) return mask;
+  int digits = 1;
+  for (long c = count - 1; c >= 10; c /= 10) ++digits;
+  char num[32];
+  std::snprintf(num, sizeof num, "%0*ld", digits, index);
+  const size_t at = mask.rfind('@');
+  if (at != std::string::npos) return mask.substr(0, at) + num + mask.substr(at + 1);
+  const size_t slash = mask.rfind('/');
+  const size_t dot = mask.rfind('.');
+  if (dot != std::string::npos && (slash == std::string::npos || dot > slash))
+    return mask.substr(0, dot) + "-" + num + mask.substr(dot);
+  return mask + "-" + num;
+}
+
+}  // namespace ctas

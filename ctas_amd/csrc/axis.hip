@@ -93,7 +93,7 @@ __device__ __forceinline__ float sobel_term(const float *cs, int h, int w, int y
 
 __global__ __launch_bounds__(256) void sobel_score_kernel(const float *__restrict__ slices,
                                                           const float *__restrict__ centres, int w,
-                                                          double *__restrict__ partial) {
+                                                          int smoothed, double *__restrict__ partial) {
   const int k = blockIdx.y;
   const float c = centres[k];
   const int r = static_cast<int>(ceilf(fabsf(c)));
@@ -124,7 +124,9 @@ __global__ __launch_bounds__(256) void sobel_score_kernel(const float *__restric
       dV = sobel_term(cs, h, w, yy, x, 1, 1, 1.0f, dV);
       const float m = sqrtf(dH * dH + dV * dV);
       sm += m;
-      cnt += (m > 0.0f) ? 1.0 : 0.0;
+      // #(cS > 0) of lct.cpp:912: the reference's float-FFT smoothing leaves
+      // round-off in every pixel, so then every derivative pixel counts.
+      cnt += (smoothed || m > 0.0f) ? 1.0 : 0.0;
     }
   }
   __shared__ double ps[256], pc[256];
@@ -165,7 +167,8 @@ hipError_t launch_ag_scores(float *d_slices, int n, int w, const float *d_centre
     hipLaunchKernelGGL(smooth_rows_kernel, dim3(w, n), dim3(256), sizeof(float) * w, s, d_slices, d_tmp, w, taps);
     hipLaunchKernelGGL(smooth_cols_kernel, dim3(w, n), dim3(256), 0, s, d_tmp, d_slices, w, taps);
   }
-  hipLaunchKernelGGL(sobel_score_kernel, dim3(kScoreBlocks, n), dim3(256), 0, s, d_slices, d_centres, w, d_partial);
+  hipLaunchKernelGGL(sobel_score_kernel, dim3(kScoreBlocks, n), dim3(256), 0, s, d_slices, d_centres, w,
+                     sigma > 1.0f ? 1 : 0, d_partial);
   hipLaunchKernelGGL(score_finish_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d_partial, d_scores, n);
   return hipGetLastError();
 }

@@ -454,7 +454,9 @@ void orc_ring_filter(float *P, int N, int w, int box) {
  * sig22); it is evaluated here separably in FP64 (the FFT of the reference
  * computes the same sum in float). The gradient magnitude is read from the
  * top rows of the full derivative arrays, reproducing the shape mismatch of
- * lct.cpp:905. work may be NULL. */
+ * lct.cpp:905. work may be NULL. Parity note: the reference's positive-value
+ * count depends on FFTW round-off; the restatement counts every pixel of the
+ * derivative rows when smoothing (see the count below). */
 double orc_ag_score(const float *Pf, int N, int w, const float *sincos,
                     float sigma, float axisR, float *work) {
   (void)work;
@@ -520,6 +522,11 @@ double orc_ag_score(const float *Pf, int N, int w, const float *sincos,
       }
     }
   }
+  /* The count #(cS > 0) of lct.cpp:912: after the reference's float FFT
+   * smoothing every value carries round-off noise, so every pixel of the rows
+   * that received derivatives (full-array rows >= r) counts; without
+   * smoothing, exact zeros (the masked corners) do not. */
+  const int smoothed = sigma > 1;
   double sm = 0.0;
   long nm = 0;
   for (int y = 1; y < h - 1; y++) {
@@ -527,7 +534,7 @@ double orc_ag_score(const float *Pf, int N, int w, const float *sincos,
       const float a = dH[(size_t)y * w + x], b = dV[(size_t)y * w + x];
       const float m = sqrtf(a * a + b * b);
       sm += m;
-      nm += (m > 0.0f);
+      nm += smoothed ? (y >= r) : (m > 0.0f);
     }
   }
   free(dH);

