@@ -54,6 +54,14 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise CtasError(CTAS_E_DEVICE, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    # torch bundles its own libamdhip64.so.7. Loading it first makes the
+    # engine's libamdhip64.so.7 dependency resolve to that same copy, so the
+    # process has one HIP runtime; loaded the other way round, torch would
+    # bring up a second runtime and report no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     vp, i, f, d = C.c_void_p, C.c_int, C.c_float, C.c_double
     fp = C.POINTER(C.c_float)

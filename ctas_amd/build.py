@@ -96,6 +96,17 @@ def build_cpp_api(force: bool = True) -> Path:
     return lib
 
 
+def build_diag(force: bool = True) -> Path:
+    """ctas-peak-probe: measures clock, LDS bandwidth and VALU rate on the
+    device (the peaks of the backprojection roofline)."""
+    src = CSRC / "diag" / "ctas_peak_probe.cpp"
+    exe = PKG / "bin" / "ctas-peak-probe"
+    exe.parent.mkdir(exist_ok=True)
+    if force or _needs(exe, [src]):
+        _run([HIPCC, *HIP_FLAGS, "-x", "hip", str(src), "-o", str(exe)])
+    return exe
+
+
 def build_oracle() -> None:
     """The CPU oracle (checker for the tests and the CPU-baseline timing)."""
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
@@ -104,6 +115,7 @@ def build_oracle() -> None:
 def build_all(force: bool = True) -> None:
     build_engine(force=force)
     build_cpp_api(force=force)
+    build_diag(force=force)
     build_oracle()
 
 

@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "ctas_internal.hpp"
@@ -264,17 +265,220 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
   }
 }
 
+// One output tile of one item, FP16/BF16 storage with CTAS (floor)
+// sampling in the precision path (FP64 angle constants). The LDS windows
+// hold the 16-bit samples as stored. The ray coordinate of each pixel is a
+// 16.16 fixed-point integer: per angle, the tile's base term (T0 - lo + the
+// window slot offset, from FP64) plus a row term and a column term, each
+// advanced by an integer increment, so that each sample costs one integer
+// add and one shift (the byte offset is (u >> 16) << 1). Two angles are
+// summed into a pixel's float accumulator with one dot2 (f16 or bf16 pair
+// times (1, 1)). Accuracy: the fixed-point coordinate is within ~1e-4 px of
+// the FP64 one, finer than the float arithmetic of the parity path.
+template <int ST>
+struct Pair16;
+template <>
+struct Pair16<ST_F16> {
+  typedef _Float16 v2 __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ float dot(v2 a, float acc) {
+    const v2 one = {static_cast<_Float16>(1.0f), static_cast<_Float16>(1.0f)};
+    return __builtin_amdgcn_fdot2(a, one, acc, false);
+  }
+  static __device__ __forceinline__ v2 make(uint16_t lo, uint16_t hi) {
+    return v2{__builtin_bit_cast(_Float16, lo), __builtin_bit_cast(_Float16, hi)};
+  }
+};
+template <>
+struct Pair16<ST_BF16> {
+  typedef __bf16 v2 __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ float dot(v2 a, float acc) {
+    const v2 one = {static_cast<__bf16>(1.0f), static_cast<__bf16>(1.0f)};
+    return __builtin_amdgcn_fdot2_f32_bf16(a, one, acc, false);
+  }
+  static __device__ __forceinline__ v2 make(uint16_t lo, uint16_t hi) {
+    return v2{__builtin_bit_cast(__bf16, lo), __builtin_bit_cast(__bf16, hi)};
+  }
+};
+
+template <int KIND, int AC>
+constexpr size_t lds_bytes_fx() {
+  return sizeof(uint16_t) * AC * KindGeo<KIND>::L + sizeof(int4) * AC + sizeof(int) * AC;
+}
+
+template <int KIND, int AC, int ST>
+__device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned char *lds) {
+  using G = KindGeo<KIND>;
+  using P = Pair16<ST>;
+  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT, L = G::L;
+  static_assert(AC % 2 == 0, "angles are summed in pairs");
+  uint16_t *win = reinterpret_cast<uint16_t *>(lds);
+  int4 *prm = reinterpret_cast<int4 *>(lds + sizeof(uint16_t) * AC * L);
+  int *los = reinterpret_cast<int *>(lds + sizeof(uint16_t) * AC * L + sizeof(int4) * AC);
+
+  const int w = it.w, N = it.N;
+  const int hp = w / 2;
+  const float c = it.center;
+  const int ty0 = (tile / it.tiles_x) * RT;
+  const int tx0 = (tile % it.tiles_x) * CT;
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  float *__restrict__ out = it.out;
+  {
+    const float acent = fabsf(c);
+    const float ax = static_cast<float>(hp), ay = static_cast<float>(hp) + c;
+    const float dx = fmaxf(fmaxf(static_cast<float>(tx0) - ax, ax - static_cast<float>(tx0 + CT - 1)), 0.0f);
+    const float dy = fmaxf(fmaxf(static_cast<float>(ty0) - ay, ay - static_cast<float>(ty0 + RT - 1)), 0.0f);
+    const float rmax = static_cast<float>(hp) + acent + 2.0f;
+    const bool outside = (dx * dx + dy * dy > rmax * rmax) ||
+                         (static_cast<float>(tx0 + CT - 1) <= acent) ||
+                         (static_cast<float>(tx0) >= static_cast<float>(w) - acent - 1.0f);
+    if (outside) {
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        const int row = ty0 + ty + 16 * r;
+#pragma unroll
+        for (int q = 0; q < TC; ++q) {
+          const int col = tx0 + tx + 16 * q;
+          if (row < w && col < w) out[static_cast<size_t>(row) * w + col] = 0.0f;
+        }
+      }
+      return;
+    }
+  }
+  float acc[TR][TC];
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TC; ++q) acc[r][q] = 0.0f;
+  const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
+  const long long total = static_cast<long long>(N) * w;
+  const float fty = static_cast<float>(ty), ftx = static_cast<float>(tx);
+
+  for (int a0 = 0; a0 < N; a0 += AC) {
+    const int na = min(AC, N - a0);
+    const int np = (na + 1) & ~1;  // slots used, padded to a pair
+    __syncthreads();
+    if (tid < np) {
+      // an odd pass ends with an all-zero slot that reuses the last angle's
+      // geometry
+      const int ang = a0 + min(tid, na - 1);
+      const double2 sd = it.sincos64[ang];
+      const double cd = static_cast<double>(c);
+      const double T0 = cd + hp * ((1.0 - sd.x) - sd.y) + sd.x * (ty0 - cd) + sd.y * tx0;
+      const double tc = T0 + sd.x * (0.5 * (RT - 1)) + sd.y * (0.5 * (CT - 1));
+      const int lo = static_cast<int>(floor(tc)) - G::HALF - 2;
+      los[tid] = (tid < na) ? lo : INT_MIN;
+      const int bt = static_cast<int>(llrint((T0 - lo + static_cast<double>(tid) * L) * 65536.0));
+      const float Ss = static_cast<float>(sd.x * 65536.0), Cs = static_cast<float>(sd.y * 65536.0);
+      prm[tid] = make_int4(bt, __float_as_int(Ss), __float_as_int(Cs), 0);
+    }
+    __syncthreads();
+    for (int e = tid; e < np * L; e += 256) {
+      const int a = e / L;
+      const int o = e - a * L;
+      const int lo = los[a];
+      uint16_t v = 0;
+      if (lo != INT_MIN) {
+        const long long flat = static_cast<long long>(a0 + a) * w + lo + o;
+        if (flat >= 0 && flat < total) v = filt[flat];
+      }
+      win[e] = v;
+    }
+    __syncthreads();
+    const unsigned char *wb = reinterpret_cast<const unsigned char *>(win);
+    for (int a = 0; a < np; a += 2) {
+      int A0[TR], A1[TR], C0[TC], C1[TC];
+      {
+        const int4 p = prm[a];
+        const float Ss = __int_as_float(p.y), Cs = __int_as_float(p.z);
+        const int dS = static_cast<int>(Ss * 16.0f), dC = static_cast<int>(Cs * 16.0f);
+        A0[0] = p.x + static_cast<int>(Ss * fty);
+#pragma unroll
+        for (int r = 1; r < TR; ++r) A0[r] = A0[r - 1] + dS;
+        C0[0] = static_cast<int>(Cs * ftx);
+#pragma unroll
+        for (int q = 1; q < TC; ++q) C0[q] = C0[q - 1] + dC;
+      }
+      {
+        const int4 p = prm[a + 1];
+        const float Ss = __int_as_float(p.y), Cs = __int_as_float(p.z);
+        const int dS = static_cast<int>(Ss * 16.0f), dC = static_cast<int>(Cs * 16.0f);
+        A1[0] = p.x + static_cast<int>(Ss * fty);
+#pragma unroll
+        for (int r = 1; r < TR; ++r) A1[r] = A1[r - 1] + dS;
+        C1[0] = static_cast<int>(Cs * ftx);
+#pragma unroll
+        for (int q = 1; q < TC; ++q) C1[q] = C1[q - 1] + dC;
+      }
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        typename P::v2 v[TC];
+#pragma unroll
+        for (int q = 0; q < TC; ++q) {
+          const unsigned u0 = static_cast<unsigned>(A0[r] + C0[q]);
+          const unsigned u1 = static_cast<unsigned>(A1[r] + C1[q]);
+          const uint16_t x0 = *reinterpret_cast<const uint16_t *>(wb + ((u0 >> 16) << 1));
+          const uint16_t x1 = *reinterpret_cast<const uint16_t *>(wb + ((u1 >> 16) << 1));
+          v[q] = P::make(x0, x1);
+        }
+#pragma unroll
+        for (int q = 0; q < TC; ++q) acc[r][q] = P::dot(v[q], acc[r][q]);
+      }
+    }
+  }
+
+  const float acent = fabsf(c);
+  const float up = static_cast<float>(hp) + c, dn = static_cast<float>(hp) - c;
+  const float recCof = it.recCof;
+#pragma unroll
+  for (int r = 0; r < TR; ++r) {
+    const int row = ty0 + ty + 16 * r;
+    if (row >= w) continue;
+    float dy = static_cast<float>(row - hp) - c;
+    dy *= dy;
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+      const int col = tx0 + tx + 16 * q;
+      if (col >= w) continue;
+      int ip2 = col - hp;
+      ip2 *= ip2;
+      const float d2 = dy + static_cast<float>(ip2);
+      const float fr = static_cast<float>(row), fc = static_cast<float>(col);
+      const bool masked = (fc <= acent) || (fc >= static_cast<float>(w) - acent - 1.0f) ||
+                          (fr <= up && d2 >= up * up) || (fr >= up && d2 >= dn * dn);
+      float v = acc[r][q];
+      if (recCof != 0.0f) v *= recCof;
+      v = v / static_cast<float>(N);
+      out[static_cast<size_t>(row) * w + col] = masked ? 0.0f : v;
+    }
+  }
+}
+
 // Grid-mapped launch: one workgroup per tile; blocks are remapped so that the
 // round-robin dispatch over the 8 XCDs gives each XCD a contiguous range of
 // tiles (neighbouring tiles share sinogram windows in that XCD's L2).
 template <int KIND, int AC, int MODE, int ST>
+constexpr bool use_fx() {
+  return MODE == MODE_CTAS && ST != ST_F32;
+}
+
+template <int KIND, int AC, int MODE, int ST>
+constexpr size_t grid_lds() {
+  if constexpr (use_fx<KIND, AC, MODE, ST>()) return lds_bytes_fx<KIND, AC>();
+  return lds_bytes<KIND, AC, MODE>();
+}
+
+template <int KIND, int AC, int MODE, int ST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KindGeo<KIND>::WAVES, KindGeo<KIND>::WAVES))) void bp_grid_kernel(const Item *__restrict__ items, int n_items) {
-  __shared__ __align__(16) unsigned char lds[lds_bytes<KIND, AC, MODE>()];
+  __shared__ __align__(16) unsigned char lds[grid_lds<KIND, AC, MODE, ST>()];
   int b = blockIdx.x;
   const int q = gridDim.x >> 3;
   if (b < (q << 3)) b = (b & 7) * q + (b >> 3);
   const Item &it = items[find_item_by_tile(items, n_items, b)];
-  bp_tile<KIND, AC, MODE, ST>(it, b - it.tile_base, lds);
+  if constexpr (use_fx<KIND, AC, MODE, ST>())
+    bp_tile_fx<KIND, AC, ST>(it, b - it.tile_base, lds);
+  else
+    bp_tile<KIND, AC, MODE, ST>(it, b - it.tile_base, lds);
 }
 
 template <int A, int B>
