@@ -72,6 +72,62 @@ def time_prepare_kernel(torch, rec, d_sino, stream, reps=3):
     return ev0.elapsed_time(ev1) / reps
 
 
+def bench_axis(torch, np, ca, d_sino, w, N):
+    """Axis finding on 2048 x 1800 sinograms: the reference search (raxis,
+    algorithm 1: 129-trial sweep, walk, bisection) on one slice, and the
+    81-centre sweep (+-20 px, step 0.5) over 8 slices with scores."""
+    print("[bench] axis", file=sys.stderr, flush=True)
+    step = 180.0 / (N - 1)
+    host = d_sino[:8].cpu().numpy()
+    ca.raxis(host[0], step, 0.25, 1)  # warm
+    t0 = time.perf_counter()
+    r = ca.raxis(host[0], step, 0.25, 1)
+    raxis_ms = (time.perf_counter() - t0) * 1e3
+    cent = 0.5 * (w - 1)
+    offs = cent + np.arange(-20.0, 20.01, 0.5, dtype=np.float32)
+    rec = ca.CTrec(w, N, ca.ABS, "RAMP", anarc=step)
+    rec.sino(host[0])
+    rec.axis_scores(offs[:4])  # warm
+    t0 = time.perf_counter()
+    best = []
+    for s in range(host.shape[0]):
+        rec.sino(host[s])
+        sc = rec.axis_scores(offs)
+        best.append(float(offs[int(np.argmin(sc))] - cent))
+    sweep_ms = (time.perf_counter() - t0) * 1e3
+    return {"raxis_sweep_ms": round(raxis_ms, 2), "raxis_offset": round(r - cent, 3),
+            "sweep81x8_ms": round(sweep_ms, 1), "sweep81_ms_per_slice": round(sweep_ms / host.shape[0], 1),
+            "sweep81_best_offsets": best, "true_offset": 3.37}
+
+
+def bench_uneven(torch, np, ca, n_items=96, seed=7):
+    """96 sinograms of widths {512..4096} x angles {360..1800} in one
+    persistent backprojection launch (plus their filter launch)."""
+    print("[bench] uneven batch", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed)
+    widths = rng.choice([512, 768, 1024, 1536, 2048, 3072, 4096], n_items)
+    angles = rng.choice([360, 900, 1440, 1800], n_items)
+    items, updates = [], 0
+    for k in range(n_items):
+        w, N = int(widths[k]), int(angles[k])
+        d = torch.empty((N, w), dtype=torch.float32, device="cuda")
+        ca.phantom_sino_dev(d, ca.phantom_ellipses(20, w, 4.0 * w / 2048, 500 + k), c=1.5, N0=1e5, seed=500 + k)
+        o = torch.empty((w, w), dtype=torch.float32, device="cuda")
+        items.append((d, o, 1.5))
+        updates += w * w * N
+    torch.cuda.synchronize()
+    ca.batch_uneven(items, ca.ABS, 5, 0.0, 180.0, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 2
+    for _ in range(reps):
+        ca.batch_uneven(items, ca.ABS, 5, 0.0, 180.0, 0)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    return {"items": n_items, "voxel_updates": updates, "ms": round(ms, 2),
+            "value": round(updates / (ms / 1e3) / 1e9, 1), "unit": "Gvoxel-updates/s"}
+
+
 def cpu_baseline(d_sino, w, N, c, threads):
     """The oracle's -O3 -ffast-math build (the reference's flags), one slice
     per thread as CTAS's RecInThread does, on 2 x threads slices."""
@@ -100,6 +156,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-axis", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); slices are "
@@ -182,6 +240,10 @@ def main() -> int:
         var["fp32_linear"] = {"value": round(updates / (lms / 1e3) / 1e9, 2), "ms_per_step": round(lms, 3),
                               "bp_kernel_ms": round(lbp, 3)}
         result["variants"] = var
+    if not args.no_axis:
+        result["axis"] = bench_axis(torch, np, ca, d_sino, w, N)
+    if not args.no_uneven:
+        result["uneven_batch"] = bench_uneven(torch, np, ca)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))

@@ -468,8 +468,14 @@ constexpr size_t grid_lds() {
   return lds_bytes<KIND, AC, MODE>();
 }
 
+// Waves per SIMD each variant's register budget is sized for.
 template <int KIND, int AC, int MODE, int ST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KindGeo<KIND>::WAVES, KindGeo<KIND>::WAVES))) void bp_grid_kernel(const Item *__restrict__ items, int n_items) {
+constexpr int grid_waves() {
+  return KindGeo<KIND>::WAVES;
+}
+
+template <int KIND, int AC, int MODE, int ST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grid_waves<KIND, AC, MODE, ST>(), grid_waves<KIND, AC, MODE, ST>()))) void bp_grid_kernel(const Item *__restrict__ items, int n_items) {
   __shared__ __align__(16) unsigned char lds[grid_lds<KIND, AC, MODE, ST>()];
   int b = blockIdx.x;
   const int q = gridDim.x >> 3;

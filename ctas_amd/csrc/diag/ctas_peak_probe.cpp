@@ -106,9 +106,11 @@ int main() {
   std::printf("{\"device\": \"%s\", \"cus\": %d, \"clock_ghz\": %.3f, "
               "\"lds_gbs\": {\"b32\": %.0f, \"b64\": %.0f, \"b128\": %.0f}, "
               "\"lds_bytes_per_clk_per_cu\": {\"b32\": %.1f, \"b64\": %.1f, \"b128\": %.1f}, "
-              "\"valu_fp32_lane_ops_per_s\": %.4g, \"valu_lane_ops_per_clk_per_cu\": %.1f}\n",
+              "\"valu_fp32_lane_ops_per_s\": %.4g, \"valu_lane_ops_per_clk_per_cu\": %.1f, "
+              "\"lds_per_block\": %zu, \"lds_per_cu\": %zu}\n",
               p.gcnArchName, cus, ghz, gb32, gb64, gb128, gb32 * 1e9 / (ghz * 1e9 * cus),
               gb64 * 1e9 / (ghz * 1e9 * cus), gb128 * 1e9 / (ghz * 1e9 * cus), lane_ops,
-              lane_ops / (ghz * 1e9 * cus));
+              lane_ops / (ghz * 1e9 * cus), static_cast<size_t>(p.sharedMemPerBlock),
+              static_cast<size_t>(p.maxSharedMemoryPerMultiProcessor));
   return 0;
 }

@@ -103,10 +103,8 @@ __global__ __launch_bounds__(256) void minmax_kernel(const Item *__restrict__ it
 template <int ST>
 __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ items, int n_items,
                                                      int contrast) {
+  // The only LDS is the complex line (Z x 8 bytes), so Z = 8192 fits 64 KiB.
   extern __shared__ float2 buf[];
-  __shared__ float s_mina;
-  __shared__ int s_clamp;
-  __shared__ float2 s_tot[256];
 
   const int b = blockIdx.x;
   const Item &it = items[find_item_by_pair(items, n_items, b)];
@@ -121,28 +119,22 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
   const int tid = threadIdx.x;
 
   // deAbs (physical.world.cpp:88-100): clamp at max/1e6 only when the
-  // sinogram's minimum is not positive, then -log.
+  // sinogram's minimum is not positive, then -log. Every wave reduces the 64
+  // partial extrema itself (one per lane), so no LDS is needed.
   const bool absorb = (contrast == CTAS_CONTRAST_ABS);
+  bool clamp = false;
+  float mina = 0.0f;
   if (absorb) {
-    if (tid < 64) {
-      const float2 m = it.minmax[tid];
-      float mn = m.x, mx = m.y;
-      for (int o = 32; o > 0; o >>= 1) {
-        mn = fminf(mn, __shfl_xor(mn, o));
-        mx = fmaxf(mx, __shfl_xor(mx, o));
-      }
-      if (tid == 0) {
-        const int clamp = (mn <= 0.0f);
-        const float mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
-        s_clamp = clamp;
-        s_mina = mina;
-        if (pair == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
-      }
+    const float2 m = it.minmax[tid & 63];
+    float mn = m.x, mx = m.y;
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, o));
+      mx = fmaxf(mx, __shfl_xor(mx, o));
     }
-    __syncthreads();
+    clamp = (mn <= 0.0f);
+    mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
+    if (pair == 0 && tid == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
   }
-  const bool clamp = absorb && s_clamp;
-  const float mina = absorb ? s_mina : 0.0f;
   auto deabs = [&](float v) {
     if (clamp && v <= mina) v = mina;
     return -logf(v);
@@ -289,38 +281,18 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
     }
     return;
   }
-  // REF (lct.cpp:590-595): prefix sum along each filtered row; contiguous
-  // chunks per thread, then a scan of the chunk totals.
-  const int cs = (w + 255) / 256;
-  const int xb = tid * cs, xe = min(w, xb + cs);
-  float2 acc = make_float2(0.0f, 0.0f);
-  for (int x = xb; x < xe; ++x) {
-    const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
-    acc.x += v.x * invZ;
-    acc.y += v.y * invZ;
-  }
-  s_tot[tid] = acc;
-  __syncthreads();
-  if (tid == 0) {
-    float2 run = make_float2(0.0f, 0.0f);
-    for (int k = 0; k < 256; ++k) {
-      const float2 t = s_tot[k];
-      s_tot[k] = run;
-      run.x += t.x;
-      run.y += t.y;
-    }
-  }
-  __syncthreads();
-  float2 run = s_tot[tid];
-  for (int x = xb; x < xe; ++x) {
-    const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
-    run.x += v.x * invZ;
-    run.y += v.y * invZ;
-    store_val<ST>(it.filt, row0 + x, run.x);
-    if (it.filt_f32) it.filt_f32[row0 + x] = run.x;
-    if (has1) {
-      store_val<ST>(it.filt, row1 + x, run.y);
-      if (it.filt_f32) it.filt_f32[row1 + x] = run.y;
+  // REF (lct.cpp:590-595): prefix sum along each filtered row, one lane per
+  // row, in the sequential order of std::partial_sum.
+  if (tid == 0 || (tid == 64 && has1)) {
+    const bool im = tid == 64;
+    const size_t rowb = im ? row1 : row0;
+    float run = 0.0f;
+    for (int x = 0; x < w; ++x) {
+      const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+      const float f = (im ? v.y : v.x) * invZ;
+      run = (x == 0) ? f : run + f;
+      store_val<ST>(it.filt, rowb + x, run);
+      if (it.filt_f32) it.filt_f32[rowb + x] = run;
     }
   }
 }

@@ -90,6 +90,20 @@ def test_hann_offset_centre_batch():
         assert rel_rms(out[s], ref[s]) <= 1e-4
 
 
+@pytest.mark.parametrize("contrast", [ca.ABS, ca.REF])
+def test_width_4096_zidth_8192(contrast):
+    """The widest configuration (w = 4096, zidth 8192: a 64 KiB LDS line)."""
+    N, w = 24, 4096
+    P = _phantom(N, w, 5.0, 12, 1e5, seed=90, scale=4.0)
+    if contrast == ca.REF:
+        P = (-np.log(P)).astype(np.float32)
+    rec = ca.CTrec(w, N, contrast, "HANN")
+    filt, out = rec.reconstruct(P, 5.0)
+    ref_f, ref = oracle.fbp_batch(P, [5.0], contrast, 5, 0.0, 180.0)
+    assert rel_rms(filt, ref_f[0]) <= 2e-5
+    assert rel_rms(out[0], ref[0]) <= 1e-4
+
+
 def test_linear_mode_vs_f64():
     N, w = 240, 256
     P = _phantom(N, w, 2.0, 8, 0.0, seed=9)  # line integrals
