@@ -108,6 +108,26 @@ def lib() -> C.CDLL:
         "ctas_raxis_pair": (i, [fp, fp, i, i, f, fp]),
         "ctas_phantom_ellipses": (i, [i, i, d, C.c_uint64, C.POINTER(d)]),
         "ctas_phantom_sino_dev": (i, [i, i, d, i, C.POINTER(d), d, C.c_uint64, vp, vp, vp]),
+        "ctas_cnn_create": (i, [C.POINTER(vp), C.c_uint64]),
+        "ctas_cnn_destroy": (i, [vp]),
+        "ctas_cnn_num_params": (i, [vp, ip]),
+        "ctas_cnn_get_params": (i, [vp, fp]),
+        "ctas_cnn_set_params": (i, [vp, fp]),
+        "ctas_cnn_get_grads": (i, [vp, fp]),
+        "ctas_cnn_save": (i, [vp, C.c_char_p]),
+        "ctas_cnn_load": (i, [vp, C.c_char_p]),
+        "ctas_cnn_forward_dev": (i, [vp, vp, i, vp]),
+        "ctas_cnn_gradients_dev": (i, [vp, vp, vp, i, fp]),
+        "ctas_cnn_train_step_dev": (i, [vp, vp, vp, i, f, fp]),
+        "ctas_cnn_features_dev": (i, [vp, i, i, vp, vp]),
+        "ctas_axis_sample": (i, [C.c_uint64, C.c_uint64, ip, ip, C.POINTER(d), C.POINTER(d),
+                                 C.POINTER(d), ip]),
+        "ctas_cnn_make_batch": (i, [C.c_uint64, C.c_uint64, i, vp, vp, fp, ip, vp]),
+        "ctas_cnn_predict_dev": (i, [vp, vp, i, i, fp, fp]),
+        "ctas_cnn_predict": (i, [vp, fp, i, i, fp, fp]),
+        "ctas_axis_find": (i, [vp, vp, vp, f, f, f, fp, ip]),
+        "ctas_raxis_cnn": (i, [vp, fp, i, i, f, f, f, fp, ip]),
+        "ctas_cnn_gemm_dev": (i, [vp, vp, vp, i, i, i, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -364,3 +384,140 @@ def phantom_sino_dev(d_out, ell: np.ndarray, c: float = 0.0, N0: float = 1e5, se
     _check(lib().ctas_phantom_sino_dev(N, w, c, e.shape[0], e.ctypes.data_as(C.POINTER(C.c_double)),
                                        N0, seed, None if d_gain is None else _ptr(d_gain),
                                        _ptr(d_out), _stream_ptr(stream)))
+
+
+# ---- learned axis finder ------------------------------------------------------
+CNN_INPUT = (2, 180, 256)
+CNN_WEIGHTS = Path(__file__).resolve().parent / "data" / "axis_cnn.bin"
+
+
+class AxisCNN:
+    """The BF16 axis-offset network (include/ctas_hip.h, ctas_cnn_*)."""
+
+    def __init__(self, seed: int = 0, weights: str | Path | None = None):
+        self._h = C.c_void_p()
+        _check(lib().ctas_cnn_create(C.byref(self._h), seed))
+        if weights is not None:
+            self.load(weights)
+
+    def close(self) -> None:
+        if self._h:
+            lib().ctas_cnn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self) -> int:
+        n = C.c_int()
+        _check(lib().ctas_cnn_num_params(self._h, C.byref(n)))
+        return n.value
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        _check(lib().ctas_cnn_get_params(self._h, _f32p(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        q = np.ascontiguousarray(p, np.float32)
+        assert q.size == self.num_params
+        _check(lib().ctas_cnn_set_params(self._h, _f32p(q)))
+
+    def grads(self) -> np.ndarray:
+        g = np.zeros(self.num_params, np.float32)
+        _check(lib().ctas_cnn_get_grads(self._h, _f32p(g)))
+        return g
+
+    def save(self, path) -> None:
+        _check(lib().ctas_cnn_save(self._h, str(path).encode()))
+
+    def load(self, path) -> None:
+        _check(lib().ctas_cnn_load(self._h, str(path).encode()))
+
+    def forward_dev(self, d_x, d_out) -> None:
+        """d_x: uint16 (bf16 bits) / bfloat16 tensor [B][2][180][256]; d_out float32 [B][2]."""
+        _check(lib().ctas_cnn_forward_dev(self._h, d_x.data_ptr(), d_x.shape[0], _ptr(d_out)))
+
+    def gradients_dev(self, d_x, d_target) -> float:
+        loss = C.c_float()
+        _check(lib().ctas_cnn_gradients_dev(self._h, d_x.data_ptr(), _ptr(d_target), d_x.shape[0],
+                                            C.byref(loss)))
+        return loss.value
+
+    def train_step_dev(self, d_x, d_target, lr: float) -> float:
+        loss = C.c_float()
+        _check(lib().ctas_cnn_train_step_dev(self._h, d_x.data_ptr(), _ptr(d_target), d_x.shape[0],
+                                             lr, C.byref(loss)))
+        return loss.value
+
+    def predict_dev(self, d_sino) -> tuple[float, float]:
+        o, c = C.c_float(), C.c_float()
+        _check(lib().ctas_cnn_predict_dev(self._h, _ptr(d_sino), d_sino.shape[0], d_sino.shape[1],
+                                          C.byref(o), C.byref(c)))
+        return o.value, c.value
+
+    def predict(self, sino: np.ndarray) -> tuple[float, float]:
+        s = np.ascontiguousarray(sino, np.float32)
+        o, c = C.c_float(), C.c_float()
+        _check(lib().ctas_cnn_predict(self._h, _f32p(s), s.shape[0], s.shape[1], C.byref(o),
+                                      C.byref(c)))
+        return o.value, c.value
+
+
+def cnn_features_dev(d_sino, d_x, stream=None) -> None:
+    _check(lib().ctas_cnn_features_dev(_ptr(d_sino), d_sino.shape[0], d_sino.shape[1],
+                                       d_x.data_ptr(), _stream_ptr(stream)))
+
+
+def axis_sample(seed: int, index: int):
+    """(w, N, c, N0, ellipses[n][6]) of training sample `index` of `seed`."""
+    w, N, n = C.c_int(), C.c_int(), C.c_int()
+    c, n0 = C.c_double(), C.c_double()
+    ell = np.zeros((64, 6), np.float64)
+    _check(lib().ctas_axis_sample(seed, index, C.byref(w), C.byref(N), C.byref(c), C.byref(n0),
+                                  ell.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+    return w.value, N.value, c.value, n0.value, ell[:n.value].copy()
+
+
+def cnn_make_batch(seed: int, first: int, d_x, d_target, stream=None):
+    """Fills d_x [B][2][180][256] (bf16 bits) and d_target [B][2]; returns
+    (true offsets in px, widths)."""
+    B = d_x.shape[0]
+    offs = np.zeros(B, np.float32)
+    ws = np.zeros(B, np.int32)
+    _check(lib().ctas_cnn_make_batch(seed, first, B, d_x.data_ptr(), _ptr(d_target), _f32p(offs),
+                                     ws.ctypes.data_as(C.POINTER(C.c_int)), _stream_ptr(stream)))
+    return offs, ws
+
+
+def axis_find(rec: CTrec, d_sino, net: AxisCNN | None = None, half_range: float = 20.0,
+              step: float = 0.5, min_confidence: float = 0.9) -> tuple[float, dict]:
+    """Trial-centre sweep (argmin of the AverageGradient score on the grid
+    -half_range + k*step), narrowed around the network's estimate when one
+    is given. Returns (offset, info)."""
+    off = C.c_float()
+    info = (C.c_int * 4)()
+    _check(lib().ctas_axis_find(rec._h, None if net is None else net._h, _ptr(d_sino), half_range,
+                                step, min_confidence, C.byref(off), info))
+    return off.value, {"trials": info[0], "swept": bool(info[1]), "estimate": info[2] / 1000.0,
+                       "confidence": info[3] / 1000.0}
+
+
+def raxis_cnn(net: AxisCNN, sino: np.ndarray, anarc: float, max_dev: float = 0.25,
+              min_confidence: float = 0.9) -> tuple[float, bool]:
+    s = np.ascontiguousarray(sino, np.float32)
+    r, used = C.c_float(), C.c_int()
+    _check(lib().ctas_raxis_cnn(net._h, _f32p(s), s.shape[0], s.shape[1], anarc, max_dev,
+                                min_confidence, C.byref(r), C.byref(used)))
+    return r.value, bool(used.value)
+
+
+def cnn_gemm_dev(d_a, d_b, d_c, stream=None) -> None:
+    M, K = d_a.shape
+    N = d_b.shape[1]
+    _check(lib().ctas_cnn_gemm_dev(d_a.data_ptr(), d_b.data_ptr(), _ptr(d_c), M, N, K,
+                                   _stream_ptr(stream)))

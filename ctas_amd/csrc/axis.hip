@@ -191,6 +191,8 @@ int grow_buf(T *&ptr, size_t &cap, size_t bytes) {
 
 constexpr int kScoreChunk = 64;  // centres per batch (bounds the scratch slices)
 
+}  // namespace
+
 // improveMe (lct.cpp:1108-1129): bisection towards the lower neighbour.
 float improve_me(float eps, const std::vector<float> &vax, const std::vector<float> &err, int idx, int cdir,
                  const std::function<float(float)> &f, int *rc) {
@@ -215,6 +217,8 @@ float improve_me(float eps, const std::vector<float> &vax, const std::vector<flo
   }
   return rf;
 }
+
+namespace {
 
 struct HandleGuard {
   ctas_fbp *h = nullptr;

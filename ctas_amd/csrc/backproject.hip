@@ -80,6 +80,20 @@ constexpr size_t lds_bytes() {
          sizeof(int) * AC;
 }
 
+// True when every pixel of the block rows [r0, r0+nr) x cols [c0, c0+nc)
+// lies outside the reconstruction disc (radius at most hp + |c| around the
+// axis point (hp, hp + c)) or the column band, so that all are masked.
+__device__ __forceinline__ bool wave_outside(const Item &it, float c, int r0, int nr, int c0, int nc) {
+  const int hp = it.w / 2;
+  const float acent = fabsf(c);
+  const float ax = static_cast<float>(hp), ay = static_cast<float>(hp) + c;
+  const float dx = fmaxf(fmaxf(static_cast<float>(c0) - ax, ax - static_cast<float>(c0 + nc - 1)), 0.0f);
+  const float dy = fmaxf(fmaxf(static_cast<float>(r0) - ay, ay - static_cast<float>(r0 + nr - 1)), 0.0f);
+  const float rmax = static_cast<float>(hp) + acent + 2.0f;
+  return (dx * dx + dy * dy > rmax * rmax) || (static_cast<float>(c0 + nc - 1) <= acent) ||
+         (static_cast<float>(c0) >= static_cast<float>(it.w) - acent - 1.0f);
+}
+
 __device__ __forceinline__ int find_item_by_tile(const Item *items, int n, int b) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
@@ -105,7 +119,10 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
   const int ty0 = (tile / it.tiles_x) * RT;
   const int tx0 = (tile % it.tiles_x) * CT;
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
+  // wave v owns the tile rows [v 4TR, (v+1) 4TR); lane (ty4, tx) the pixels
+  // (v 4TR + ty4 + 4r, tx + 16q): one LDS read instruction covers a 4 x 16
+  // pixel block, and a wave whose rows lie outside the disc skips its work
+  const int tx = tid & 15, ty = (tid >> 6) * (4 * TR) + ((tid >> 4) & 3);
   float *__restrict__ out = it.out;
 
   // Tiles that lie wholly outside the reconstruction disc (radius at most
@@ -123,7 +140,7 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
     if (outside) {
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
-        const int row = ty0 + ty + 16 * r;
+        const int row = ty0 + ty + 4 * r;
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
           const int col = tx0 + tx + 16 * q;
@@ -136,7 +153,7 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
 
   float jc[TR], fi[TC];
 #pragma unroll
-  for (int r = 0; r < TR; ++r) jc[r] = static_cast<float>(ty0 + ty + 16 * r) - c;
+  for (int r = 0; r < TR; ++r) jc[r] = static_cast<float>(ty0 + ty + 4 * r) - c;
 #pragma unroll
   for (int q = 0; q < TC; ++q) fi[q] = static_cast<float>(tx0 + tx + 16 * q);
   float acc[TR][TC];
@@ -148,6 +165,7 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
   const float ycm = static_cast<float>(ty0) + 0.5f * (RT - 1) - c;
   const float xcm = static_cast<float>(tx0) + 0.5f * (CT - 1);
   const void *__restrict__ filt = it.filt;
+  const bool wskip = wave_outside(it, c, ty0 + (tid >> 6) * (4 * TR), 4 * TR, tx0, CT);
   const long long total = static_cast<long long>(N) * w;
 
   for (int a0 = 0; a0 < N; a0 += AC) {
@@ -194,6 +212,7 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
       }
     }
     __syncthreads();
+    if (wskip) continue;
     const unsigned char *wb0 = reinterpret_cast<const unsigned char *>(win);
 #pragma unroll 2
     for (int a = 0; a < na; ++a) {
@@ -243,7 +262,7 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
   const float recCof = it.recCof;
 #pragma unroll
   for (int r = 0; r < TR; ++r) {
-    const int row = ty0 + ty + 16 * r;
+    const int row = ty0 + ty + 4 * r;
     if (row >= w) continue;
     float dy = static_cast<float>(row - hp) - c;
     dy *= dy;
@@ -321,7 +340,10 @@ __device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned ch
   const int ty0 = (tile / it.tiles_x) * RT;
   const int tx0 = (tile % it.tiles_x) * CT;
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
+  // wave v owns the tile rows [v 4TR, (v+1) 4TR); lane (ty4, tx) the pixels
+  // (v 4TR + ty4 + 4r, tx + 16q): one LDS read instruction covers a 4 x 16
+  // pixel block, and a wave whose rows lie outside the disc skips its work
+  const int tx = tid & 15, ty = (tid >> 6) * (4 * TR) + ((tid >> 4) & 3);
   float *__restrict__ out = it.out;
   {
     const float acent = fabsf(c);
@@ -335,7 +357,7 @@ __device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned ch
     if (outside) {
 #pragma unroll
       for (int r = 0; r < TR; ++r) {
-        const int row = ty0 + ty + 16 * r;
+        const int row = ty0 + ty + 4 * r;
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
           const int col = tx0 + tx + 16 * q;
@@ -353,6 +375,7 @@ __device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned ch
   const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
   const long long total = static_cast<long long>(N) * w;
   const float fty = static_cast<float>(ty), ftx = static_cast<float>(tx);
+  const bool wskip = wave_outside(it, c, ty0 + (tid >> 6) * (4 * TR), 4 * TR, tx0, CT);
 
   for (int a0 = 0; a0 < N; a0 += AC) {
     const int na = min(AC, N - a0);
@@ -385,13 +408,14 @@ __device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned ch
       win[e] = v;
     }
     __syncthreads();
+    if (wskip) continue;
     const unsigned char *wb = reinterpret_cast<const unsigned char *>(win);
     for (int a = 0; a < np; a += 2) {
       int A0[TR], A1[TR], C0[TC], C1[TC];
       {
         const int4 p = prm[a];
         const float Ss = __int_as_float(p.y), Cs = __int_as_float(p.z);
-        const int dS = static_cast<int>(Ss * 16.0f), dC = static_cast<int>(Cs * 16.0f);
+        const int dS = static_cast<int>(Ss * 4.0f), dC = static_cast<int>(Cs * 16.0f);
         A0[0] = p.x + static_cast<int>(Ss * fty);
 #pragma unroll
         for (int r = 1; r < TR; ++r) A0[r] = A0[r - 1] + dS;
@@ -402,7 +426,7 @@ __device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned ch
       {
         const int4 p = prm[a + 1];
         const float Ss = __int_as_float(p.y), Cs = __int_as_float(p.z);
-        const int dS = static_cast<int>(Ss * 16.0f), dC = static_cast<int>(Cs * 16.0f);
+        const int dS = static_cast<int>(Ss * 4.0f), dC = static_cast<int>(Cs * 16.0f);
         A1[0] = p.x + static_cast<int>(Ss * fty);
 #pragma unroll
         for (int r = 1; r < TR; ++r) A1[r] = A1[r - 1] + dS;
@@ -432,7 +456,7 @@ __device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned ch
   const float recCof = it.recCof;
 #pragma unroll
   for (int r = 0; r < TR; ++r) {
-    const int row = ty0 + ty + 16 * r;
+    const int row = ty0 + ty + 4 * r;
     if (row >= w) continue;
     float dy = static_cast<float>(row - hp) - c;
     dy *= dy;

@@ -247,7 +247,7 @@ int choose_kind(const ctas_fbp *h) {
     auto it = g_tune.find(tune_key(h));
     if (it != g_tune.end()) return it->second.first;
   }
-  return h->w >= 1024 ? TK_128x128 : TK_64x64;
+  return TK_64x64;  // the tuned choice at 512-4096 px on MI355X
 }
 int choose_ac(const ctas_fbp *h) {
   if (h->angles_per_pass) return h->angles_per_pass;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -34,6 +35,12 @@ inline bool use_fp64_angles(int flags) {
   return (flags & CTAS_INTERP_MASK) != CTAS_INTERP_CTAS || (flags & CTAS_STORE_MASK) != CTAS_STORE_FP32;
 }
 int filter_window(int filter, float param, int w, int contrast, std::vector<float> &W);
+
+// improveMe (lct.cpp:1108-1129): bisection from the bracket around vax[idx]
+// down to eps; f evaluates the score of one trial offset and sets *rc on
+// failure (axis.hip).
+float improve_me(float eps, const std::vector<float> &vax, const std::vector<float> &err, int idx, int cdir,
+                 const std::function<float(float)> &f, int *rc);
 
 // ---- device descriptors -----------------------------------------------------
 // One sinogram -> one slice. The backprojector walks tiles of every item;

@@ -174,6 +174,68 @@ int ctas_raxis(const float *sino, int N, int w, float anarc, float maxDev, int a
 int ctas_raxis_pair(const float *p0, const float *p180, int rows, int w, float maxDev,
                     float *result);
 
+/* ---- learned axis finder (no counterpart in the reference) ------------
+ * A BF16 CNN (4 conv 3x3 + 2 FC, 245,554 parameters) on the matrix cores
+ * that estimates the centre offset of a sinogram and a confidence; it
+ * fronts the sweep of trial reconstructions of raxis (lct.cpp:1133-1212,
+ * AverageGradient lct.cpp:804-917). Input: [2][180][256] bf16 (the -log
+ * sinogram averaged over 180 angle cells x 256 detector bins, standardised,
+ * and its detector-flipped copy). Output per sample: {offset in detector
+ * bins of the input, confidence logit}. */
+typedef struct ctas_cnn ctas_cnn;
+int ctas_cnn_create(ctas_cnn **net, uint64_t seed);
+int ctas_cnn_destroy(ctas_cnn *net);
+/* Parameters: w1 b1 w2 b2 w3 b3 w4 b4 wf1 bf1 wf2 bf2 back to back (conv
+ * weights [cout][cin][3][3], fully connected [out][in]), float32. */
+int ctas_cnn_num_params(const ctas_cnn *net, int *count);
+int ctas_cnn_get_params(ctas_cnn *net, float *params);
+int ctas_cnn_set_params(ctas_cnn *net, const float *params);
+/* Gradients of the last ctas_cnn_gradients_dev / train_step call. */
+int ctas_cnn_get_grads(ctas_cnn *net, float *grads);
+int ctas_cnn_save(const ctas_cnn *net, const char *path);
+int ctas_cnn_load(ctas_cnn *net, const char *path);
+/* d_x [B][2][180][256] bf16 -> d_out [B][2] float32. */
+int ctas_cnn_forward_dev(ctas_cnn *net, const uint16_t *d_x, int B, float *d_out);
+/* d_target [B][2] = {offset (bins), tolerance (bins)}: loss = mean of
+ * Huber(offset error, 0.25) + 0.2 BCE(logit, |error| < tolerance). */
+int ctas_cnn_gradients_dev(ctas_cnn *net, const uint16_t *d_x, const float *d_target, int B,
+                           float *loss);
+/* gradients + one Adam step (beta 0.9 / 0.999, eps 1e-8) at rate lr. */
+int ctas_cnn_train_step_dev(ctas_cnn *net, const uint16_t *d_x, const float *d_target, int B,
+                            float lr, float *loss);
+/* Network input of one device sinogram [N][w] (transmission). */
+int ctas_cnn_features_dev(const float *d_sino, int N, int w, uint16_t *d_x, void *stream);
+/* Training sample `index` of stream `seed` (w, N, offset c, photons N0 and
+ * up to 60 ellipses (x0, y0, a, b, phi, mu)); returns the ellipse count. */
+int ctas_axis_sample(uint64_t seed, uint64_t index, int *w, int *n_angles, double *c, double *N0,
+                     double *ell, int *n_ell);
+/* Samples first .. first+B-1 of `seed` projected on the GPU and reduced to
+ * network inputs d_x and targets d_target ({c/bin, 1 px/bin}). */
+int ctas_cnn_make_batch(uint64_t seed, uint64_t first, int B, uint16_t *d_x, float *d_target,
+                        float *offsets_px, int *widths_px, void *stream);
+/* Estimate (px) and confidence in [0, 1] for one sinogram. */
+int ctas_cnn_predict_dev(ctas_cnn *net, const float *d_sino, int n_angles, int w, float *offset,
+                         float *confidence);
+int ctas_cnn_predict(ctas_cnn *net, const float *sino, int n_angles, int w, float *offset,
+                     float *confidence);
+/* Sweep of trial centres -half_range + k*step (AverageGradient score,
+ * argmin) on a device sinogram of h's shape (h: ABS contrast, RAMP filter).
+ * With net (and confidence >= min_confidence) only a window around the
+ * estimate is scored (5 points, widened until its minimum is interior).
+ * info[4]: trials, whole grid swept (0/1), estimate (milli-px),
+ * confidence (per mille). */
+int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, float half_range, float step,
+                   float min_confidence, float *offset, int *info);
+/* raxis algorithm 1 (lct.cpp:1133-1212) with its 8 px sweep replaced by the
+ * network's estimate (improveMe from +-4 px around it); falls back to
+ * ctas_raxis(algo 1) below min_confidence. */
+int ctas_raxis_cnn(ctas_cnn *net, const float *sino, int N, int w, float anarc, float maxDev,
+                   float min_confidence, float *result, int *used_sweep);
+/* The network's bf16 GEMM on row-major operands (verification):
+ * C[M][N] = A[M][K] B[K][N]. */
+int ctas_cnn_gemm_dev(const uint16_t *d_a, const uint16_t *d_b, float *d_c, int M, int N, int K,
+                      void *stream);
+
 /* ---- synthetic ellipse phantoms ---------------------------------------- */
 /* n_ell random ellipses (x0, y0, a, b, phi, mu per pixel) for width w: centres
  * uniform in the inner 80% of the disc, semi-axes uniform in
