@@ -1,6 +1,6 @@
 // Learned front end of the rotation-axis search: a small BF16 CNN that
 // predicts the centre offset, and a confidence, from a sinogram downsampled
-// to 180 angles x 256 detector bins plus its detector-flipped mirror.
+// to 180 angles x 256 detector bins plus its copy turned by 180 degrees.
 //
 // Network (245,554 parameters): four 3x3 convolutions (2->32, 32->32, 32->64,
 // 64->64, padding 1, ReLU, 2x2 max pooling after each) on 2x180x256, then a
@@ -385,8 +385,12 @@ __global__ void to_bf16(const float *__restrict__ p, uint16_t *__restrict__ q, i
 
 // Sinogram [N][w] (transmission) -> features [2][180][256] bf16: the mean
 // of -log(max(v, 1e-6)) over each cell of angle rows x detector columns,
-// standardised per sample; channel 1 is the detector-flipped copy. Grid
-// (1, 180): one block per angle cell, one thread per detector bin.
+// standardised per sample; channel 1 is the map turned by 180 degrees
+// (detector and angle order reversed), which puts the projections that
+// mirror each other across the 0/180 degree seam (P(theta + pi, s) =
+// P(theta, -s)) in the same rows of the two channels, shifted by twice
+// the axis offset. Grid (1, 180): one block per angle cell, one thread per
+// detector bin.
 __global__ __launch_bounds__(256) void features_kernel(const float *__restrict__ sino, int N, int w,
                                                        float *__restrict__ tmp) {
   const int a = blockIdx.y;
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(256) void standardise_kernel(const float *__restric
     const int a = i / kFeatW, x = i - a * kFeatW;
     const float v = (tmp[i] - mean) * inv;
     feat[i] = bf16_rne(v);
-    feat[n + a * kFeatW + (kFeatW - 1 - x)] = bf16_rne(v);
+    feat[n + (kFeatA - 1 - a) * kFeatW + (kFeatW - 1 - x)] = bf16_rne(v);
   }
 }
 
@@ -924,12 +928,14 @@ extern "C" int ctas_cnn_predict(ctas_cnn *n, const float *sino, int N, int w, fl
 // Sweep of trial centres offsets -half_range + k*step (k = 0 .. K-1) on the
 // device sinogram d_sino of handle h's shape (AverageGradient canon: ABS,
 // RAMP); result = the offset of the lowest score (first on ties). With a
-// network whose confidence reaches min_confidence, only a window of 5 grid
-// points around its estimate is scored, widened by 2 points towards a lower
-// edge score until the lowest score is interior (or the window reaches the
-// grid's end); past 21 points the remaining grid is scored too. info
+// network whose confidence reaches min_confidence, only a window of 3 grid
+// points around its estimate is scored, widened by one point on the side of
+// its lowest score while that is at an edge (and not the grid's end); past
+// kMaxWindow points the remaining grid is scored too. info
 // (optional, 4 ints): trials scored, 1 if the whole grid was swept,
 // network estimate (milli-px), confidence (per mille).
+constexpr int kMaxWindow = 21;
+
 extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, float half_range, float step,
                               float min_confidence, float *offset, int *info) {
   if (!h || !d_sino || !offset || !(step > 0) || !(half_range >= 0)) return fail(CTAS_E_ARG, "bad arguments");
@@ -976,18 +982,18 @@ extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, f
   if (net && conf >= min_confidence) {
     swept = 0;
     const int k0 = std::max(0, std::min(K - 1, static_cast<int>(std::lround((est + half_range) / step))));
-    int lo = std::max(0, k0 - 2), hi = std::min(K - 1, k0 + 2);
+    int lo = std::max(0, k0 - 1), hi = std::min(K - 1, k0 + 1);
     if (int rc = eval(lo, hi)) return rc;
     for (;;) {
       km = argmin(lo, hi);
-      if (hi - lo + 1 > 21) {
+      if (hi - lo + 1 > kMaxWindow) {
         swept = 1;
         break;
       }
       if (km == lo && lo > 0) {
-        lo = std::max(0, lo - 2);
+        --lo;
       } else if (km == hi && hi < K - 1) {
-        hi = std::min(K - 1, hi + 2);
+        ++hi;
       } else {
         break;
       }

@@ -180,7 +180,7 @@ int ctas_raxis_pair(const float *p0, const float *p180, int rows, int w, float m
  * fronts the sweep of trial reconstructions of raxis (lct.cpp:1133-1212,
  * AverageGradient lct.cpp:804-917). Input: [2][180][256] bf16 (the -log
  * sinogram averaged over 180 angle cells x 256 detector bins, standardised,
- * and its detector-flipped copy). Output per sample: {offset in detector
+ * and its copy turned by 180 degrees: detector and angle order reversed). Output per sample: {offset in detector
  * bins of the input, confidence logit}. */
 typedef struct ctas_cnn ctas_cnn;
 int ctas_cnn_create(ctas_cnn **net, uint64_t seed);
@@ -221,7 +221,8 @@ int ctas_cnn_predict(ctas_cnn *net, const float *sino, int n_angles, int w, floa
 /* Sweep of trial centres -half_range + k*step (AverageGradient score,
  * argmin) on a device sinogram of h's shape (h: ABS contrast, RAMP filter).
  * With net (and confidence >= min_confidence) only a window around the
- * estimate is scored (5 points, widened until its minimum is interior).
+ * estimate is scored (3 points, widened one point at a time until its
+ * minimum is interior; past 21 points the whole grid).
  * info[4]: trials, whole grid swept (0/1), estimate (milli-px),
  * confidence (per mille). */
 int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, float half_range, float step,

@@ -19,7 +19,7 @@ differences in the tests).
 
 The network input is restated too (``features``): per cell of angle rows x
 detector columns the mean of -log(max(v, 1e-6)), standardised, plus the
-detector-flipped copy.
+copy turned by 180 degrees (angle and detector order reversed).
 """
 from __future__ import annotations
 
@@ -200,4 +200,4 @@ def features(sino: np.ndarray) -> np.ndarray:
             c1 = max(c0 + 1, (x + 1) * w // FEAT_W)
             out[a, x] = L[r0:r1, c0:c1].mean()
     v = (out - out.mean()) / np.sqrt(max(out.var(), 1e-12))
-    return np.stack([v, v[:, ::-1]])
+    return np.stack([v, v[::-1, ::-1]])

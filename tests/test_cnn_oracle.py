@@ -106,7 +106,7 @@ def test_features_cells_flip_and_standardisation():
     s[0, 0] = 0.0                       # clamped at 1e-6
     f = co.features(s)
     assert f.shape == (2, 180, 256)
-    assert np.allclose(f[1], f[0][:, ::-1])
+    assert np.allclose(f[1], f[0][::-1, ::-1])
     assert abs(f[0].mean()) < 1e-12 and f[0].std() == pytest.approx(1.0)
     L = -np.log(np.maximum(s.astype(np.float64), 1e-6))
     cells = L.reshape(180, 2, 256, 2).mean(axis=(1, 3))

@@ -74,7 +74,7 @@ def test_features_match_oracle(torch_cuda):
     got = co.from_bf16_bits(x.cpu().numpy().view(np.uint16))
     ref = co.features(d.cpu().numpy())
     assert rel_rms(got, ref) < 4e-3     # bf16 storage (2^-9 relative)
-    assert np.array_equal(got[1], got[0][:, ::-1])
+    assert np.array_equal(got[1], got[0][::-1, ::-1])
 
 
 def _gpu_params(net):
