@@ -100,6 +100,28 @@ def bench_axis(torch, np, ca, d_sino, w, N):
             "sweep81_best_offsets": best, "true_offset": 3.37}
 
 
+def bench_axis_cnn(torch, np, ca, n_fixed=24, n_mixed=48):
+    """Axis finding from a device-resident sinogram: the plain sweep of 81
+    trial centres (-20 .. 20 px step 0.5, AverageGradient score) against the
+    network-fronted search (window around the CNN estimate; the whole sweep
+    when the confidence is below 0.9), on held-out engine-made phantoms:
+    n_fixed of 2048 px x 1800 angles and n_mixed of mixed shapes
+    (1024-4096 px, 900-1800 angles)."""
+    from ctas_amd import eval_axis_cnn as ev
+    if not ca.CNN_WEIGHTS.exists():
+        return {"skipped": f"no trained weights at {ca.CNN_WEIGHTS}"}
+    print("[bench] axis CNN vs sweep", file=sys.stderr, flush=True)
+    net = ca.AxisCNN(weights=ca.CNN_WEIGHTS)
+    fixed = ev.run(ca, torch, net, seed=2000003, n=n_fixed, shape=(2048, 1800),
+                   log=lambda m: print(m, file=sys.stderr, flush=True))
+    mixed = ev.run(ca, torch, net, seed=1000001, n=n_mixed,
+                   log=lambda m: print(m, file=sys.stderr, flush=True))
+    keep = ("n", "agree_with_sweep_0.5px", "exact_agreement", "sweep_within_0.5px_of_truth",
+            "cnn_estimate_median_err_px", "fallback_rate", "mean_trials_cnn", "trials_sweep",
+            "sweep_ms_mean", "cnn_ms_mean", "speedup_mean_time")
+    return {"2048x1800": {k: fixed[k] for k in keep}, "mixed_shapes": {k: mixed[k] for k in keep}}
+
+
 def bench_uneven(torch, np, ca, n_items=96, seed=7):
     """96 sinograms of widths {512..4096} x angles {360..1800} in one
     persistent backprojection launch (plus their filter launch)."""
@@ -242,6 +264,7 @@ def main() -> int:
         result["variants"] = var
     if not args.no_axis:
         result["axis"] = bench_axis(torch, np, ca, d_sino, w, N)
+        result["axis_cnn"] = bench_axis_cnn(torch, np, ca)
     if not args.no_uneven:
         result["uneven_batch"] = bench_uneven(torch, np, ca)
     if not args.no_cpu:

@@ -118,7 +118,7 @@ def lib() -> C.CDLL:
         "ctas_cnn_load": (i, [vp, C.c_char_p]),
         "ctas_cnn_forward_dev": (i, [vp, vp, i, vp]),
         "ctas_cnn_gradients_dev": (i, [vp, vp, vp, i, fp]),
-        "ctas_cnn_train_step_dev": (i, [vp, vp, vp, i, f, fp]),
+        "ctas_cnn_train_step_dev": (i, [vp, vp, vp, i, f, f, fp, fp]),
         "ctas_cnn_features_dev": (i, [vp, i, i, vp, vp]),
         "ctas_axis_sample": (i, [C.c_uint64, C.c_uint64, ip, ip, C.POINTER(d), C.POINTER(d),
                                  C.POINTER(d), ip]),
@@ -448,11 +448,14 @@ class AxisCNN:
                                             C.byref(loss)))
         return loss.value
 
-    def train_step_dev(self, d_x, d_target, lr: float) -> float:
-        loss = C.c_float()
+    def train_step_dev(self, d_x, d_target, lr: float, clip_norm: float = 0.0,
+                       with_norm: bool = False):
+        """One Adam step; returns the loss (and the gradient norm before
+        clipping when with_norm)."""
+        loss, norm = C.c_float(), C.c_float()
         _check(lib().ctas_cnn_train_step_dev(self._h, d_x.data_ptr(), _ptr(d_target), d_x.shape[0],
-                                             lr, C.byref(loss)))
-        return loss.value
+                                             lr, clip_norm, C.byref(loss), C.byref(norm)))
+        return (loss.value, norm.value) if with_norm else loss.value
 
     def predict_dev(self, d_sino) -> tuple[float, float]:
         o, c = C.c_float(), C.c_float()

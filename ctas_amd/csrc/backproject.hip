@@ -478,6 +478,175 @@ __device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned ch
   }
 }
 
+// ---- FP16/BF16 with pair words -------------------------------------------
+// LDS holds, for every window position o, the 32-bit word (v[o], v[o+1]).
+// A lane owns vertically adjacent pixel pairs (rows j, j+1): their ray
+// coordinates differ by s = sin(theta), so with floor sampling row j+1
+// reads v[k] or v[k+1] (k = floor(t_j)) for s >= 0, v[k-1] or v[k] for
+// s < 0 (the pair is then taken at k-1). One ds_read_b32 serves both pixels;
+// each pixel adds its half with a dot2 against a weight word that is 1.0 in
+// the half it takes. Coordinates are 16.16 fixed point formed from FP64
+// per-angle constants, as in bp_tile_fx.
+template <int ST> struct One16;
+template <> struct One16<ST_F16> { static constexpr unsigned LO = 0x3C00u, HI = 0x3C000000u; };
+template <> struct One16<ST_BF16> { static constexpr unsigned LO = 0x3F80u, HI = 0x3F800000u; };
+
+template <int ST>
+__device__ __forceinline__ float dot2w(unsigned a, unsigned b, float acc) {
+  using v2 = typename Pair16<ST>::v2;
+  if constexpr (ST == ST_F16)
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(v2, a), __builtin_bit_cast(v2, b), acc, false);
+  else
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(v2, a), __builtin_bit_cast(v2, b), acc, false);
+}
+
+template <int KIND, int AC>
+constexpr size_t lds_bytes_px2() {
+  return sizeof(uint32_t) * AC * KindGeo<KIND>::L + 2 * sizeof(int4) * AC + sizeof(int) * AC;
+}
+
+template <int KIND, int AC, int ST>
+__device__ __forceinline__ void bp_tile_px2(const Item &it, int tile, unsigned char *lds) {
+  using G = KindGeo<KIND>;
+  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT, L = G::L;
+  constexpr int RP = TR / 2;
+  constexpr unsigned LO = One16<ST>::LO, HI = One16<ST>::HI;
+  uint32_t *win = reinterpret_cast<uint32_t *>(lds);
+  int4 *prm = reinterpret_cast<int4 *>(lds + sizeof(uint32_t) * AC * L);
+  int *los = reinterpret_cast<int *>(lds + sizeof(uint32_t) * AC * L + 2 * sizeof(int4) * AC);
+
+  const int w = it.w, N = it.N;
+  const int hp = w / 2;
+  const float c = it.center;
+  const int ty0 = (tile / it.tiles_x) * RT;
+  const int tx0 = (tile % it.tiles_x) * CT;
+  const int tid = threadIdx.x;
+  // wave v owns the tile rows [v 4TR, (v+1) 4TR); lane (ty4, tx) the pixels
+  // (v 4TR + 2 ty4 + 8 rp + h, tx + 16 q), h = 0, 1; the coordinate of
+  // pixel (y, x) of the tile is bt + S y + C x in 16.16 fixed point
+  const int tx = tid & 15, ty = (tid >> 6) * (4 * TR) + 2 * ((tid >> 4) & 3);
+  float *__restrict__ out = it.out;
+  {
+    const float acent = fabsf(c);
+    const float ax = static_cast<float>(hp), ay = static_cast<float>(hp) + c;
+    const float dx = fmaxf(fmaxf(static_cast<float>(tx0) - ax, ax - static_cast<float>(tx0 + CT - 1)), 0.0f);
+    const float dy = fmaxf(fmaxf(static_cast<float>(ty0) - ay, ay - static_cast<float>(ty0 + RT - 1)), 0.0f);
+    const float rmax = static_cast<float>(hp) + acent + 2.0f;
+    const bool outside = (dx * dx + dy * dy > rmax * rmax) ||
+                         (static_cast<float>(tx0 + CT - 1) <= acent) ||
+                         (static_cast<float>(tx0) >= static_cast<float>(w) - acent - 1.0f);
+    if (outside) {
+#pragma unroll
+      for (int rp = 0; rp < RP; ++rp)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int row = ty0 + ty + 8 * rp + h;
+#pragma unroll
+          for (int q = 0; q < TC; ++q) {
+            const int col = tx0 + tx + 16 * q;
+            if (row < w && col < w) out[static_cast<size_t>(row) * w + col] = 0.0f;
+          }
+        }
+      return;
+    }
+  }
+  float acc[RP][2][TC];
+#pragma unroll
+  for (int rp = 0; rp < RP; ++rp)
+#pragma unroll
+    for (int q = 0; q < TC; ++q) acc[rp][0][q] = acc[rp][1][q] = 0.0f;
+  const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
+  const long long total = static_cast<long long>(N) * w;
+  const bool wskip = wave_outside(it, c, ty0 + (tid >> 6) * (4 * TR), 4 * TR, tx0, CT);
+
+  for (int a0 = 0; a0 < N; a0 += AC) {
+    const int na = min(AC, N - a0);
+    __syncthreads();
+    if (tid < na) {
+      const int ang = a0 + tid;
+      const double2 sd = it.sincos64[ang];
+      const double cd = static_cast<double>(c);
+      const double T0 = cd + hp * ((1.0 - sd.x) - sd.y) + sd.x * (ty0 - cd) + sd.y * tx0;
+      const double tc = T0 + sd.x * (0.5 * (RT - 1)) + sd.y * (0.5 * (CT - 1));
+      const int lo = static_cast<int>(floor(tc)) - G::HALF - 2;
+      los[tid] = lo;
+      const bool neg = sd.x < 0.0;
+      const int bt = static_cast<int>(llrint((T0 - lo + static_cast<double>(tid) * L) * 65536.0)) - (neg ? 65536 : 0);
+      const int S = static_cast<int>(llrint(sd.x * 65536.0)), C = static_cast<int>(llrint(sd.y * 65536.0));
+      const int thr = neg ? -S : 65536 - S;
+      prm[2 * tid] = make_int4(bt, S, C, thr | (neg ? (1 << 30) : 0));
+      prm[2 * tid + 1] = make_int4(8 * S, 16 * C, 0, 0);
+    }
+    __syncthreads();
+    for (int e = tid; e < na * L; e += 256) {
+      const int a = e / L;
+      const int o = e - a * L;
+      const long long flat = static_cast<long long>(a0 + a) * w + los[a] + o;
+      const uint32_t v0 = (flat >= 0 && flat < total) ? filt[flat] : 0u;
+      const uint32_t v1 = (flat + 1 >= 0 && flat + 1 < total) ? filt[flat + 1] : 0u;
+      win[e] = v0 | (v1 << 16);
+    }
+    __syncthreads();
+    if (wskip) continue;
+    const unsigned char *wb = reinterpret_cast<const unsigned char *>(win);
+    for (int a = 0; a < na; ++a) {
+      const int4 p = prm[2 * a], pd = prm[2 * a + 1];
+      const unsigned thr = static_cast<unsigned>(p.w & 0x3FFFFFFF);
+      const unsigned wj = (p.w >> 30) ? HI : LO;
+      int A[RP], C[TC];
+      A[0] = __mul24(p.y, ty) + p.x;
+#pragma unroll
+      for (int rp = 1; rp < RP; ++rp) A[rp] = A[rp - 1] + pd.x;
+      C[0] = __mul24(p.z, tx);
+#pragma unroll
+      for (int q = 1; q < TC; ++q) C[q] = C[q - 1] + pd.y;
+#pragma unroll
+      for (int rp = 0; rp < RP; ++rp) {
+        unsigned u[TC], P[TC];
+#pragma unroll
+        for (int q = 0; q < TC; ++q) {
+          u[q] = static_cast<unsigned>(A[rp] + C[q]);
+          P[q] = *reinterpret_cast<const uint32_t *>(wb + ((u[q] >> 16) << 2));
+        }
+#pragma unroll
+        for (int q = 0; q < TC; ++q) {
+          const unsigned wj1 = (u[q] & 0xFFFFu) >= thr ? HI : LO;
+          acc[rp][0][q] = dot2w<ST>(P[q], wj, acc[rp][0][q]);
+          acc[rp][1][q] = dot2w<ST>(P[q], wj1, acc[rp][1][q]);
+        }
+      }
+    }
+  }
+
+  const float acent = fabsf(c);
+  const float up = static_cast<float>(hp) + c, dn = static_cast<float>(hp) - c;
+  const float recCof = it.recCof;
+#pragma unroll
+  for (int rp = 0; rp < RP; ++rp)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int row = ty0 + ty + 8 * rp + h;
+      if (row >= w) continue;
+      float dy = static_cast<float>(row - hp) - c;
+      dy *= dy;
+#pragma unroll
+      for (int q = 0; q < TC; ++q) {
+        const int col = tx0 + tx + 16 * q;
+        if (col >= w) continue;
+        int ip2 = col - hp;
+        ip2 *= ip2;
+        const float d2 = dy + static_cast<float>(ip2);
+        const float fr = static_cast<float>(row), fc = static_cast<float>(col);
+        const bool masked = (fc <= acent) || (fc >= static_cast<float>(w) - acent - 1.0f) ||
+                            (fr <= up && d2 >= up * up) || (fr >= up && d2 >= dn * dn);
+        float v = acc[rp][h][q];
+        if (recCof != 0.0f) v *= recCof;
+        v = v / static_cast<float>(N);
+        out[static_cast<size_t>(row) * w + col] = masked ? 0.0f : v;
+      }
+    }
+}
+
 // Grid-mapped launch: one workgroup per tile; blocks are remapped so that the
 // round-robin dispatch over the 8 XCDs gives each XCD a contiguous range of
 // tiles (neighbouring tiles share sinogram windows in that XCD's L2).
@@ -488,7 +657,7 @@ constexpr bool use_fx() {
 
 template <int KIND, int AC, int MODE, int ST>
 constexpr size_t grid_lds() {
-  if constexpr (use_fx<KIND, AC, MODE, ST>()) return lds_bytes_fx<KIND, AC>();
+  if constexpr (use_fx<KIND, AC, MODE, ST>()) return lds_bytes_px2<KIND, AC>();
   return lds_bytes<KIND, AC, MODE>();
 }
 
@@ -506,7 +675,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grid_waves<
   if (b < (q << 3)) b = (b & 7) * q + (b >> 3);
   const Item &it = items[find_item_by_tile(items, n_items, b)];
   if constexpr (use_fx<KIND, AC, MODE, ST>())
-    bp_tile_fx<KIND, AC, ST>(it, b - it.tile_base, lds);
+    bp_tile_px2<KIND, AC, ST>(it, b - it.tile_base, lds);
   else
     bp_tile<KIND, AC, MODE, ST>(it, b - it.tile_base, lds);
 }

@@ -3,8 +3,9 @@
 // to 180 angles x 256 detector bins plus its copy turned by 180 degrees.
 //
 // Network (245,554 parameters): four 3x3 convolutions (2->32, 32->32, 32->64,
-// 64->64, padding 1, ReLU, 2x2 max pooling after each) on 2x180x256, then a
-// 11264->16 fully connected layer with ReLU and a 16->2 output (offset in
+// 64->64, padding 1, leaky ReLU (slope 1/64), 2x2 max pooling after each) on
+// 2x180x256, then a 11264->16 fully connected layer with leaky ReLU and a
+// 16->2 output (offset in
 // downsampled detector bins, confidence logit). Activations and weights are
 // bfloat16; every product runs on the matrix cores (v_mfma_f32_16x16x16_bf16,
 // float32 accumulation) through one implicit-GEMM kernel whose operands are
@@ -56,6 +57,11 @@ __device__ __forceinline__ uint16_t bf16_rne(float v) {
 __device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 
 constexpr int kFeatA = 180, kFeatW = 256;  // feature map: angles x detector bins
+// Slope of the leaky ReLU on the negative side (a power of two, so scaled
+// gradients round like the unscaled ones).
+constexpr float kLeak = 0.015625f;
+
+__device__ __forceinline__ float leaky(float v) { return v > 0.0f ? v : kLeak * v; }
 constexpr int kBT = 64;                    // GEMM block tile (M and N)
 
 // ---- generic GEMM: C[m][n] = sum_k A(m, k) B(k, n) -------------------------
@@ -157,9 +163,9 @@ struct Im2colT {  // B(n, k) = im2col(k, n): the pixel dimension is the reductio
   Im2col f;
   __device__ uint16_t operator()(int n, int k) const { return f(k, n); }
 };
-// gradient of the pre-activation: g = d(post) * (post > 0), as bf16
+// gradient of the pre-activation: g = d(post) * (post > 0 ? 1 : kLeak), as bf16
 __device__ __forceinline__ uint16_t relu_grad(const float *d, const uint16_t *post, size_t i) {
-  return bf16_f(post[i]) > 0.0f ? bf16_rne(d[i]) : static_cast<uint16_t>(0);
+  return bf16_rne(bf16_f(post[i]) > 0.0f ? d[i] : kLeak * d[i]);
 }
 struct GradA {  // A(co, n) = g[b][co][y][x], n = (b*H + y)*W + x
   const float *d;
@@ -217,13 +223,13 @@ struct ActBT {  // B(n, k) = act[n][k]
 };
 
 // ---- store functors -------------------------------------------------------------
-struct StoreConvAct {  // y[b][co][y][x] = relu(acc + bias[co]) as bf16
+struct StoreConvAct {  // y[b][co][y][x] = leaky(acc + bias[co]) as bf16
   uint16_t *y;
   const float *bias;
   int C, H, W;
   __device__ void operator()(int co, int n, float v, int) const {
     const int hw = H * W, b = n / hw, p = n - b * hw;
-    y[(static_cast<size_t>(b) * C + co) * hw + p] = bf16_rne(fmaxf(v + bias[co], 0.0f));
+    y[(static_cast<size_t>(b) * C + co) * hw + p] = bf16_rne(leaky(v + bias[co]));
   }
 };
 struct StoreDX {  // dx[b][ci][y][x] (fp32)
@@ -241,7 +247,7 @@ struct StorePartial {  // ws[split][m][n]
     ws[(static_cast<size_t>(z) * M + m) * N + n] = v;
   }
 };
-struct StoreFC {  // out[b][o] = acc + bias[o] (relu optional), bf16 and/or fp32
+struct StoreFC {  // out[b][o] = acc + bias[o] (leaky ReLU optional), bf16 and/or fp32
   uint16_t *y16;
   float *y32;
   const float *bias;
@@ -249,7 +255,7 @@ struct StoreFC {  // out[b][o] = acc + bias[o] (relu optional), bf16 and/or fp32
   bool relu;
   __device__ void operator()(int o, int b, float v, int) const {
     float r = v + bias[o];
-    if (relu) r = fmaxf(r, 0.0f);
+    if (relu) r = leaky(r);
     const size_t i = static_cast<size_t>(b) * O + o;
     if (y16) y16[i] = bf16_rne(r);
     if (y32) y32[i] = r;
@@ -312,7 +318,20 @@ __global__ void reduce_splits(const float *__restrict__ ws, float *__restrict__ 
   out[i] = s;
 }
 
-// Bias gradient: sum of the masked gradient over batch and pixels (one
+// Epilogue of a split forward product: y[b][m][p] = bf16(leaky(sum over
+// splits of ws[z][m][n] + bias[m])), n = b*HW + p.
+__global__ void finish_act(const float *__restrict__ ws, int splits, int M, int N, const float *__restrict__ bias,
+                           uint16_t *__restrict__ y, int HW) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * N) return;
+  const int m = i / N, n = i - m * N;
+  float v = 0.0f;
+  for (int z = 0; z < splits; ++z) v += ws[static_cast<size_t>(z) * M * N + i];
+  const int b = n / HW, p = n - b * HW;
+  y[(static_cast<size_t>(b) * M + m) * HW + p] = bf16_rne(leaky(v + bias[m]));
+}
+
+// Bias gradient: sum of the slope-scaled gradient over batch and pixels (one
 // block per channel, fixed-order tree reduction).
 __global__ void bias_grad(const float *__restrict__ d, const uint16_t *__restrict__ post, float *__restrict__ db,
                           int B, int C, int HW) {
@@ -321,7 +340,7 @@ __global__ void bias_grad(const float *__restrict__ d, const uint16_t *__restric
   for (int b = 0; b < B; ++b)
     for (int p = threadIdx.x; p < HW; p += blockDim.x) {
       const size_t i = (static_cast<size_t>(b) * C + co) * HW + p;
-      if (!post || bf16_f(post[i]) > 0.0f) s += d[i];
+      s += (!post || bf16_f(post[i]) > 0.0f) ? d[i] : kLeak * d[i];
     }
   __shared__ float red[256];
   red[threadIdx.x] = s;
@@ -363,12 +382,46 @@ __global__ void loss_kernel(const float *__restrict__ out, const float *__restri
   if (threadIdx.x == 0) *loss = red[0] / B;
 }
 
+// Global gradient norm: per-block sums of squares, then one block sums them
+// in a fixed order and writes the clipping factor min(1, clip / norm) (1
+// when clip <= 0) and the norm.
+constexpr int kNormBlocks = 256;
+
+__global__ __launch_bounds__(256) void sumsq_kernel(const float *__restrict__ g, int n, float *__restrict__ part) {
+  __shared__ float red[256];
+  float s = 0.0f;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kNormBlocks * 256) s += g[i] * g[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void clip_factor_kernel(const float *__restrict__ part, float clip,
+                                                          float *__restrict__ out) {
+  __shared__ float red[256];
+  red[threadIdx.x] = threadIdx.x < kNormBlocks ? part[threadIdx.x] : 0.0f;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float norm = sqrtf(red[0]);
+    out[0] = (clip > 0.0f && norm > clip) ? clip / norm : 1.0f;
+    out[1] = norm;
+  }
+}
+
 __global__ void adam_kernel(float *__restrict__ p, uint16_t *__restrict__ p16, const float *__restrict__ g,
                             float *__restrict__ m, float *__restrict__ v, int n, float lr, float b1, float b2,
-                            float c1, float c2, float eps) {
+                            float c1, float c2, float eps, const float *__restrict__ scale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float gi = g[i];
+  const float gi = g[i] * scale[0];
   const float mi = b1 * m[i] + (1.0f - b1) * gi;
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
   m[i] = mi;
@@ -470,6 +523,7 @@ struct ctas_cnn {
   float *loss = nullptr;
   float *ftmp = nullptr;
   uint16_t *xin = nullptr;  // features of one sinogram (prediction)
+  float *norm = nullptr;    // gradient-norm partials, clip factor, norm
 };
 
 namespace ctas {
@@ -523,6 +577,7 @@ int ensure_batch(ctas_cnn *n, int B) {
 
 int ensure_ws(ctas_cnn *n, size_t floats) {
   if (floats <= n->ws_cap) return CTAS_OK;
+  CTAS_HIP_TRY(hipStreamSynchronize(n->stream));
   if (n->ws) (void)hipFree(n->ws);
   n->ws = nullptr;
   if (int rc = alloc(reinterpret_cast<void **>(&n->ws), floats * 4)) return rc;
@@ -536,20 +591,49 @@ float *G32(const ctas_cnn *n, int i) { return n->g + n->prm[i].off; }
 
 int blocks_for(size_t n) { return static_cast<int>(std::min<size_t>((n + 255) / 256, 65535)); }
 
+// Splits of the reduction for a product whose C tiles alone would not fill
+// the chip (inference on a single sinogram); 1 when they do.
+int forward_splits(int M, int N, int K) {
+  const int tiles = ((M + kBT - 1) / kBT) * ((N + kBT - 1) / kBT);
+  if (tiles >= 256) return 1;
+  return std::max(1, std::min((256 + tiles - 1) / tiles, K / 64));
+}
+
+// A product with bias + ReLU + bf16 epilogue: direct store when it fills
+// the chip, else split over K into the workspace and finished in a fixed
+// order.
+template <class FA, class FB>
+int product_act(ctas_cnn *n, FA fa, FB fb, int M, int Ncols, int K, const float *bias, uint16_t *y, int HW) {
+  hipStream_t s = n->stream;
+  const int splits = forward_splits(M, Ncols, K);
+  if (splits == 1) {
+    CTAS_HIP_TRY(gemm(fa, fb, StoreConvAct{y, bias, M, 1, HW}, M, Ncols, K, s));
+    return CTAS_OK;
+  }
+  if (int rc = ensure_ws(n, static_cast<size_t>(splits) * M * Ncols)) return rc;
+  int chunk = (K + splits - 1) / splits;
+  chunk = (chunk + 15) & ~15;
+  const int real = (K + chunk - 1) / chunk;
+  CTAS_HIP_TRY(gemm(fa, fb, StorePartial{n->ws, M, Ncols}, M, Ncols, K, s, splits));
+  hipLaunchKernelGGL(finish_act, dim3((M * Ncols + 255) / 256), dim3(256), 0, s, n->ws, real, M, Ncols, bias, y, HW);
+  return CTAS_OK;
+}
+
 int forward(ctas_cnn *n, const uint16_t *x, int B) {
   hipStream_t s = n->stream;
   const uint16_t *in = x;
   for (int l = 0; l < 4; ++l) {
     const Layer &L = kConv[l];
     const int HW = L.H * L.W;
-    CTAS_HIP_TRY(gemm(DenseA{W16(n, 2 * l), L.cin * 9}, Im2col{in, L.cin, L.H, L.W},
-                      StoreConvAct{n->act[l], P32(n, 2 * l + 1), L.cout, L.H, L.W}, L.cout, B * HW, L.cin * 9, s));
+    if (int rc = product_act(n, DenseA{W16(n, 2 * l), L.cin * 9}, Im2col{in, L.cin, L.H, L.W}, L.cout, B * HW,
+                             L.cin * 9, P32(n, 2 * l + 1), n->act[l], HW))
+      return rc;
     const size_t q = static_cast<size_t>(B) * L.cout * (L.H / 2) * (L.W / 2);
     hipLaunchKernelGGL(maxpool_fwd, dim3(blocks_for(q)), dim3(256), 0, s, n->act[l], n->pool[l], B * L.cout, L.H, L.W);
     in = n->pool[l];
   }
-  CTAS_HIP_TRY(gemm(DenseA{W16(n, 8), kF}, ActB{n->pool[3], kF}, StoreFC{n->h1, nullptr, P32(n, 9), kH1, true}, kH1,
-                    B, kF, s));
+  if (int rc = product_act(n, DenseA{W16(n, 8), kF}, ActB{n->pool[3], kF}, kH1, B, kF, P32(n, 9), n->h1, 1))
+    return rc;
   CTAS_HIP_TRY(gemm(DenseA{W16(n, 10), kH1}, ActB{n->h1, kH1}, StoreFC{nullptr, n->out, P32(n, 11), kOut, false},
                     kOut, B, kH1, s));
   return CTAS_OK;
@@ -650,6 +734,7 @@ extern "C" int ctas_cnn_create(ctas_cnn **pn, uint64_t seed) {
   if (int rc = alloc(reinterpret_cast<void **>(&n->v), off * 4)) return rc;
   if (int rc = alloc(reinterpret_cast<void **>(&n->p16), off * 2)) return rc;
   if (int rc = alloc(reinterpret_cast<void **>(&n->loss), 256)) return rc;
+  if (int rc = alloc(reinterpret_cast<void **>(&n->norm), sizeof(float) * (kNormBlocks + 2))) return rc;
   if (int rc = alloc(reinterpret_cast<void **>(&n->ftmp), sizeof(float) * kFeatA * kFeatW)) return rc;
   CTAS_HIP_TRY(hipMemcpy(n->p, h.data(), off * 4, hipMemcpyHostToDevice));
   std::vector<float> z(off, 0.0f);
@@ -666,7 +751,7 @@ extern "C" int ctas_cnn_destroy(ctas_cnn *n) {
   if (!n) return CTAS_OK;
   (void)hipStreamSynchronize(n->stream);
   release_batch(n);
-  void *ptrs[] = {n->p, n->g, n->m, n->v, n->p16, n->ws, n->loss, n->ftmp, n->xin};
+  void *ptrs[] = {n->p, n->g, n->m, n->v, n->p16, n->ws, n->loss, n->ftmp, n->xin, n->norm};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   (void)hipStreamDestroy(n->stream);
@@ -737,15 +822,25 @@ extern "C" int ctas_cnn_gradients_dev(ctas_cnn *n, const uint16_t *d_x, const fl
   return CTAS_OK;
 }
 
+// Gradients + one Adam step (beta 0.9 / 0.999, eps 1e-8) at rate lr, the
+// gradient scaled down to global norm clip_norm when it is larger (no
+// clipping for clip_norm <= 0); grad_norm (optional) receives the norm
+// before clipping.
 extern "C" int ctas_cnn_train_step_dev(ctas_cnn *n, const uint16_t *d_x, const float *d_target, int B, float lr,
-                                       float *loss) {
+                                       float clip_norm, float *loss, float *grad_norm) {
   if (int rc = ctas_cnn_gradients_dev(n, d_x, d_target, B, loss)) return rc;
   ++n->step;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - std::pow(b1, static_cast<float>(n->step));
   const float c2 = 1.0f - std::pow(b2, static_cast<float>(n->step));
+  const int np = static_cast<int>(n->nparam);
+  hipLaunchKernelGGL(sumsq_kernel, dim3(kNormBlocks), dim3(256), 0, n->stream, n->g, np, n->norm);
+  hipLaunchKernelGGL(clip_factor_kernel, dim3(1), dim3(256), 0, n->stream, n->norm, clip_norm, n->norm + kNormBlocks);
   hipLaunchKernelGGL(adam_kernel, dim3((n->nparam + 255) / 256), dim3(256), 0, n->stream, n->p, n->p16, n->g, n->m,
-                     n->v, static_cast<int>(n->nparam), lr, b1, b2, c1, c2, 1e-8f);
+                     n->v, np, lr, b1, b2, c1, c2, 1e-8f, n->norm + kNormBlocks);
+  if (grad_norm) {
+    CTAS_HIP_TRY(hipMemcpyAsync(grad_norm, n->norm + kNormBlocks + 1, sizeof(float), hipMemcpyDeviceToHost, n->stream));
+  }
   CTAS_HIP_TRY(hipStreamSynchronize(n->stream));
   return CTAS_OK;
 }

@@ -30,11 +30,16 @@ import numpy as np
 
 
 def run(ca, torch, net, seed: int, n: int, half: float = 20.0, step: float = 0.5,
-        min_conf: float = 0.9, log=print) -> dict:
+        min_conf: float = 0.9, shape: tuple[int, int] | None = None, log=print) -> dict:
+    """shape=(w, N): only samples of that shape (the first n of the seed)."""
     groups = defaultdict(list)
-    for i in range(n):
+    i = got = 0
+    while got < n:
         w, N, c, N0, ell = ca.axis_sample(seed, i)
-        groups[(w, N)].append((i, c, N0, ell))
+        if shape is None or (w, N) == tuple(shape):
+            groups[(w, N)].append((i, c, N0, ell))
+            got += 1
+        i += 1
     rows = []
     t_start = last = time.perf_counter()
     for (w, N), items in sorted(groups.items()):

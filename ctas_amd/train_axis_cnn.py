@@ -65,7 +65,9 @@ def main(argv=None) -> int:
     ap.add_argument("--samples", type=int, default=200000)
     ap.add_argument("--epochs", type=float, default=3.0)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--clip", type=float, default=1.0, help="global gradient-norm clip")
+    ap.add_argument("--eval-every", type=int, default=1500)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--eval-seed", type=int, default=1000001)
     ap.add_argument("--eval", type=int, default=2048)
@@ -91,6 +93,7 @@ def main(argv=None) -> int:
     g.manual_seed(a.seed)
     perm = torch.randperm(a.samples, device="cuda", generator=g)
     pos, hist = 0, []
+    best = (float("inf"), None, 0)
     t0 = last = time.perf_counter()
     for s in range(steps):
         if pos + a.batch > a.samples:
@@ -102,15 +105,30 @@ def main(argv=None) -> int:
         tb = t.index_select(0, idx)
         lr = a.lr * (s + 1) / warm if s < warm else \
             a.lr * (0.02 + 0.98 * 0.5 * (1 + math.cos(math.pi * (s - warm) / max(1, steps - warm))))
-        hist.append(net.train_step_dev(xb, tb, lr))
+        loss, gnorm = net.train_step_dev(xb, tb, lr, a.clip, with_norm=True)
+        hist.append(loss)
+        if not np.isfinite(loss):
+            log(f"[train] non-finite loss at step {s + 1}; stopping")
+            break
+        if (s + 1) % a.eval_every == 0 or s == steps - 1:
+            r = evaluate(torch, ca, net, xe, oe, we)
+            if r["median_err_px"] < best[0]:
+                best = (r["median_err_px"], net.params(), s + 1)
+            log(f"[eval] step {s + 1}: held-out median {r['median_err_px']:.3f} px, "
+                f"p95 {r['p95_err_px']:.2f} px (best {best[0]:.3f} at {best[2]})")
         now = time.perf_counter()
         if now - last > 15 or s == steps - 1:
-            log(f"[train] step {s + 1}/{steps} lr {lr:.2e} loss {np.mean(hist[-200:]):.4f} {now - t0:.0f} s")
+            log(f"[train] step {s + 1}/{steps} lr {lr:.2e} loss {np.mean(hist[-200:]):.4f} "
+                f"|g| {gnorm:.3g} {now - t0:.0f} s")
             last = now
     train_s = time.perf_counter() - t0
+    if best[1] is not None:
+        net.set_params(best[1])
+        log(f"[train] kept the parameters of step {best[2]}")
     res = evaluate(torch, ca, net, xe, oe, we)
     res.update(samples=a.samples, epochs=a.epochs, batch=a.batch, steps=steps, train_s=round(train_s, 1),
-               final_loss=float(np.mean(hist[-200:])), tol_px=a.tol_px)
+               final_loss=float(np.mean(hist[-200:])), tol_px=a.tol_px, lr=a.lr, clip=a.clip,
+               kept_step=best[2])
     log("[eval] " + json.dumps(res))
     Path(a.out).parent.mkdir(parents=True, exist_ok=True)
     net.save(a.out)

@@ -175,8 +175,8 @@ int ctas_raxis_pair(const float *p0, const float *p180, int rows, int w, float m
                     float *result);
 
 /* ---- learned axis finder (no counterpart in the reference) ------------
- * A BF16 CNN (4 conv 3x3 + 2 FC, 245,554 parameters) on the matrix cores
- * that estimates the centre offset of a sinogram and a confidence; it
+ * A BF16 CNN (4 conv 3x3 + 2 FC, leaky ReLU, 245,554 parameters) on the
+ * matrix cores that estimates the centre offset of a sinogram and a confidence; it
  * fronts the sweep of trial reconstructions of raxis (lct.cpp:1133-1212,
  * AverageGradient lct.cpp:804-917). Input: [2][180][256] bf16 (the -log
  * sinogram averaged over 180 angle cells x 256 detector bins, standardised,
@@ -200,9 +200,11 @@ int ctas_cnn_forward_dev(ctas_cnn *net, const uint16_t *d_x, int B, float *d_out
  * Huber(offset error, 0.25) + 0.2 BCE(logit, |error| < tolerance). */
 int ctas_cnn_gradients_dev(ctas_cnn *net, const uint16_t *d_x, const float *d_target, int B,
                            float *loss);
-/* gradients + one Adam step (beta 0.9 / 0.999, eps 1e-8) at rate lr. */
+/* gradients + one Adam step (beta 0.9 / 0.999, eps 1e-8) at rate lr, with
+ * the gradient scaled to global norm clip_norm when larger (clip_norm <= 0:
+ * no clipping); grad_norm (optional) gets the norm before clipping. */
 int ctas_cnn_train_step_dev(ctas_cnn *net, const uint16_t *d_x, const float *d_target, int B,
-                            float lr, float *loss);
+                            float lr, float clip_norm, float *loss, float *grad_norm);
 /* Network input of one device sinogram [N][w] (transmission). */
 int ctas_cnn_features_dev(const float *d_sino, int N, int w, uint16_t *d_x, void *stream);
 /* Training sample `index` of stream `seed` (w, N, offset c, photons N0 and

@@ -4,9 +4,11 @@ The learned axis finder has no counterpart in the reference; this module is
 the checker for the engine's matrix-core kernels (ctas_amd/csrc/cnn.hip):
 forward pass, loss and every parameter gradient of
 
-    conv3x3(2->32) ReLU maxpool2 | conv3x3(32->32) ReLU maxpool2 |
-    conv3x3(32->64) ReLU maxpool2 | conv3x3(64->64) ReLU maxpool2 |
-    fc(11264->16) ReLU | fc(16->2)
+    conv3x3(2->32) act maxpool2 | conv3x3(32->32) act maxpool2 |
+    conv3x3(32->64) act maxpool2 | conv3x3(64->64) act maxpool2 |
+    fc(11264->16) act | fc(16->2)
+
+with act the leaky ReLU of slope 1/64
 
 on inputs [B][2][180][256], with the parameter packing of the C ABI
 (w1 b1 w2 b2 w3 b3 w4 b4 wf1 bf1 wf2 bf2, conv weights [cout][cin][3][3]).
@@ -34,6 +36,15 @@ SHAPES += [(H1, F_IN), (H1,), (OUT, H1), (OUT,)]
 SIZES = [int(np.prod(s)) for s in SHAPES]
 NUM_PARAMS = sum(SIZES)
 FEAT_A, FEAT_W = 180, 256
+LEAK = 1.0 / 64
+
+
+def _leaky(z):
+    return np.where(z > 0, z, LEAK * z)
+
+
+def _slope(post, g):
+    return np.where(post > 0, g, LEAK * g)
 
 
 def bf16(x: np.ndarray) -> np.ndarray:
@@ -120,13 +131,13 @@ def forward(p: np.ndarray, x: np.ndarray, bf16_emulation: bool = True):
         Wt, b = rnd(T[2 * l].reshape(co, ci * 9)), T[2 * l + 1]
         cols = _im2col(a)
         z = np.einsum("ok,bkn->bon", Wt, cols).reshape(-1, co, H, W) + b[None, :, None, None]
-        act = rnd(np.maximum(z, 0.0))
+        act = rnd(_leaky(z))
         pooled, idx = _pool(act)
         cache["layers"].append((a, cols, act, idx))
         a = pooled
     flat = a.reshape(a.shape[0], -1)
     W1, b1, W2, b2 = rnd(T[8]), T[9], rnd(T[10]), T[11]
-    h1 = rnd(np.maximum(flat @ W1.T + b1, 0.0))
+    h1 = rnd(_leaky(flat @ W1.T + b1))
     out = h1 @ W2.T + b2
     cache.update(flat=flat, h1=h1, W1=W1, W2=W2)
     return out, cache
@@ -159,17 +170,15 @@ def backward(p: np.ndarray, cache, gout: np.ndarray, bf16_emulation: bool = True
     grads[10] = g_op.T @ h1
     grads[11] = gout.sum(0)
     dh1 = g_op @ cache["W2"]
-    m1 = h1 > 0
-    g1 = rnd(np.where(m1, dh1, 0.0))
+    g1 = rnd(_slope(h1, dh1))
     grads[8] = g1.T @ flat
-    grads[9] = np.where(m1, dh1, 0.0).sum(0)
+    grads[9] = _slope(h1, dh1).sum(0)
     gpool = (g1 @ cache["W1"]).reshape(-1, 64, 11, 16)
     for l in range(3, -1, -1):
         ci, co, H, W = CONV[l]
         a_in, cols, act, idx = cache["layers"][l]
         gact = _unpool(gpool, idx, H, W)
-        mask = act > 0
-        gm = np.where(mask, gact, 0.0)
+        gm = _slope(act, gact)
         g = rnd(gm).reshape(-1, co, H * W)
         grads[2 * l] = np.einsum("bon,bkn->ok", g, cols).reshape(co, ci, 3, 3)
         grads[2 * l + 1] = gm.sum(axis=(0, 2, 3))
