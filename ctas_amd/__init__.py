@@ -116,14 +116,14 @@ def lib() -> C.CDLL:
         "ctas_cnn_get_grads": (i, [vp, fp]),
         "ctas_cnn_save": (i, [vp, C.c_char_p]),
         "ctas_cnn_load": (i, [vp, C.c_char_p]),
-        "ctas_cnn_forward_dev": (i, [vp, vp, i, vp]),
-        "ctas_cnn_gradients_dev": (i, [vp, vp, vp, i, fp]),
-        "ctas_cnn_train_step_dev": (i, [vp, vp, vp, i, f, f, fp, fp]),
+        "ctas_cnn_forward_dev": (i, [vp, vp, i, vp, vp]),
+        "ctas_cnn_gradients_dev": (i, [vp, vp, vp, i, fp, vp]),
+        "ctas_cnn_train_step_dev": (i, [vp, vp, vp, i, f, f, fp, fp, vp]),
         "ctas_cnn_features_dev": (i, [vp, i, i, vp, vp]),
         "ctas_axis_sample": (i, [C.c_uint64, C.c_uint64, ip, ip, C.POINTER(d), C.POINTER(d),
                                  C.POINTER(d), ip]),
         "ctas_cnn_make_batch": (i, [C.c_uint64, C.c_uint64, i, vp, vp, fp, ip, vp]),
-        "ctas_cnn_predict_dev": (i, [vp, vp, i, i, fp, fp]),
+        "ctas_cnn_predict_dev": (i, [vp, vp, i, i, fp, fp, vp]),
         "ctas_cnn_predict": (i, [vp, fp, i, i, fp, fp]),
         "ctas_axis_find": (i, [vp, vp, vp, f, f, f, fp, ip]),
         "ctas_raxis_cnn": (i, [vp, fp, i, i, f, f, f, fp, ip]),
@@ -191,6 +191,17 @@ def _stream_ptr(stream) -> int | None:
     if stream is None:
         return None
     return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+def _current_stream(stream=None) -> int | None:
+    """The given stream, else torch's current stream (None: the null stream)."""
+    if stream is not None:
+        return _stream_ptr(stream)
+    try:
+        import torch
+        return torch.cuda.current_stream().cuda_stream or None
+    except Exception:
+        return None
 
 
 class CTrec:
@@ -438,29 +449,33 @@ class AxisCNN:
     def load(self, path) -> None:
         _check(lib().ctas_cnn_load(self._h, str(path).encode()))
 
-    def forward_dev(self, d_x, d_out) -> None:
+    # The _dev calls order themselves after `stream` (default: torch's current
+    # stream) and return when their results are complete.
+    def forward_dev(self, d_x, d_out, stream=None) -> None:
         """d_x: uint16 (bf16 bits) / bfloat16 tensor [B][2][180][256]; d_out float32 [B][2]."""
-        _check(lib().ctas_cnn_forward_dev(self._h, d_x.data_ptr(), d_x.shape[0], _ptr(d_out)))
+        _check(lib().ctas_cnn_forward_dev(self._h, d_x.data_ptr(), d_x.shape[0], _ptr(d_out),
+                                          _current_stream(stream)))
 
-    def gradients_dev(self, d_x, d_target) -> float:
+    def gradients_dev(self, d_x, d_target, stream=None) -> float:
         loss = C.c_float()
         _check(lib().ctas_cnn_gradients_dev(self._h, d_x.data_ptr(), _ptr(d_target), d_x.shape[0],
-                                            C.byref(loss)))
+                                            C.byref(loss), _current_stream(stream)))
         return loss.value
 
     def train_step_dev(self, d_x, d_target, lr: float, clip_norm: float = 0.0,
-                       with_norm: bool = False):
+                       with_norm: bool = False, stream=None):
         """One Adam step; returns the loss (and the gradient norm before
         clipping when with_norm)."""
         loss, norm = C.c_float(), C.c_float()
         _check(lib().ctas_cnn_train_step_dev(self._h, d_x.data_ptr(), _ptr(d_target), d_x.shape[0],
-                                             lr, clip_norm, C.byref(loss), C.byref(norm)))
+                                             lr, clip_norm, C.byref(loss), C.byref(norm),
+                                             _current_stream(stream)))
         return (loss.value, norm.value) if with_norm else loss.value
 
-    def predict_dev(self, d_sino) -> tuple[float, float]:
+    def predict_dev(self, d_sino, stream=None) -> tuple[float, float]:
         o, c = C.c_float(), C.c_float()
         _check(lib().ctas_cnn_predict_dev(self._h, _ptr(d_sino), d_sino.shape[0], d_sino.shape[1],
-                                          C.byref(o), C.byref(c)))
+                                          C.byref(o), C.byref(c), _current_stream(stream)))
         return o.value, c.value
 
     def predict(self, sino: np.ndarray) -> tuple[float, float]:

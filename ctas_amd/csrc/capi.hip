@@ -92,7 +92,9 @@ int upload_items(ctas_fbp *h, const std::vector<Item> &items, hipStream_t s, boo
       std::memcmp(resident.data(), items.data(), sizeof(Item) * items.size()) == 0)
     return CTAS_OK;
   *changed = true;
+  // the table may still be read by work queued on the handle's stream or on s
   CTAS_HIP_TRY(hipStreamSynchronize(s));
+  CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
   if (static_cast<int>(items.size()) > h->items_cap) {
     if (h->d_items) (void)hipFree(h->d_items);
     h->d_items = nullptr;
@@ -143,6 +145,21 @@ Item base_item(const ctas_fbp *h, int kind) {
 hipStream_t pick_stream(const ctas_fbp *h, void *stream) {
   return stream ? static_cast<hipStream_t>(stream) : h->stream;
 }
+
+}  // namespace
+
+int stream_wait(hipStream_t waiter, hipStream_t producer) {
+  if (waiter == producer) return CTAS_OK;
+  hipEvent_t e = nullptr;
+  CTAS_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  hipError_t r = hipEventRecord(e, producer);
+  if (r == hipSuccess) r = hipStreamWaitEvent(waiter, e, 0);
+  (void)hipEventDestroy(e);
+  if (r != hipSuccess) return fail(CTAS_E_DEVICE, std::string("stream ordering: ") + hipGetErrorString(r));
+  return CTAS_OK;
+}
+
+namespace {
 
 // Enqueue prepare (+ optional backprojection) for S slices of device data.
 int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
@@ -598,6 +615,8 @@ int ctas_fbp_repeat(ctas_fbp *h, float *out, const float *centers, int n) {
 int ctas_fbp_reconstruct_dev(ctas_fbp *h, const float *d_sino, float *d_out, const float *centers, int S,
                              void *stream) {
   if (!h || !d_sino || !d_out || !centers) return fail(CTAS_E_ARG, "bad arguments");
+  NullStreamJoin join(h->stream, stream);
+  if (join.rc) return join.rc;
   const int rc = enqueue_slices(h, d_sino, d_out, nullptr, centers, S, true, pick_stream(h, stream));
   if (rc == CTAS_OK) h->kept_slices = S;
   return rc;
@@ -605,11 +624,15 @@ int ctas_fbp_reconstruct_dev(ctas_fbp *h, const float *d_sino, float *d_out, con
 
 int ctas_fbp_repeat_dev(ctas_fbp *h, int slice, float *d_out, const float *centers, int n, void *stream) {
   if (!h || !d_out || !centers) return fail(CTAS_E_ARG, "bad arguments");
+  NullStreamJoin join(h->stream, stream);
+  if (join.rc) return join.rc;
   return enqueue_repeat(h, slice, d_out, centers, n, pick_stream(h, stream));
 }
 
 int ctas_fbp_prepare_dev(ctas_fbp *h, const float *d_sino, int S, void *stream) {
   if (!h || !d_sino) return fail(CTAS_E_ARG, "bad arguments");
+  NullStreamJoin join(h->stream, stream);
+  if (join.rc) return join.rc;
   const int rc = enqueue_slices(h, d_sino, nullptr, nullptr, nullptr, S, false, pick_stream(h, stream));
   if (rc == CTAS_OK) h->kept_slices = S;
   return rc;
@@ -617,6 +640,8 @@ int ctas_fbp_prepare_dev(ctas_fbp *h, const float *d_sino, int S, void *stream) 
 
 int ctas_fbp_backproject_dev(ctas_fbp *h, float *d_out, const float *centers, int S, void *stream) {
   if (!h || !d_out || !centers) return fail(CTAS_E_ARG, "bad arguments");
+  NullStreamJoin join(h->stream, stream);
+  if (join.rc) return join.rc;
   return enqueue_repeat(h, 0, d_out, centers, S, pick_stream(h, stream), true);
 }
 
@@ -644,6 +669,7 @@ int ctas_fbp_graph_capture(ctas_fbp *h, const float *d_sino, float *d_out, const
     h->graph = nullptr;
   }
   // Make the item table resident before capture so that no copy is recorded.
+  if (!stream) CTAS_HIP_TRY(hipDeviceSynchronize());
   CTAS_HIP_TRY(hipStreamSynchronize(s));
   extra(h).resident.clear();
   {
@@ -669,6 +695,8 @@ int ctas_fbp_graph_capture(ctas_fbp *h, const float *d_sino, float *d_out, const
 int ctas_fbp_graph_launch(ctas_fbp *h, void *stream) {
   if (!h) return fail(CTAS_E_ARG, "null handle");
   if (!h->graph_exec) return fail(CTAS_E_STATE, "no captured graph: call ctas_fbp_graph_capture first");
+  NullStreamJoin join(h->stream, stream);
+  if (join.rc) return join.rc;
   CTAS_HIP_TRY(hipGraphLaunch(h->graph_exec, pick_stream(h, stream)));
   return CTAS_OK;
 }

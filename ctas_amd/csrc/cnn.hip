@@ -801,8 +801,12 @@ extern "C" int ctas_cnn_set_params(ctas_cnn *n, const float *params) {
   return CTAS_OK;
 }
 
-extern "C" int ctas_cnn_forward_dev(ctas_cnn *n, const uint16_t *d_x, int B, float *d_out) {
+// The _dev calls read inputs made ready on `stream` (nullptr: the null
+// stream), run on the network's own stream and return when their results
+// are complete.
+extern "C" int ctas_cnn_forward_dev(ctas_cnn *n, const uint16_t *d_x, int B, float *d_out, void *stream) {
   if (!n || !d_x || !d_out || B <= 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (int rc = stream_wait(n->stream, static_cast<hipStream_t>(stream))) return rc;
   if (int rc = ensure_batch(n, B)) return rc;
   if (int rc = forward(n, d_x, B)) return rc;
   CTAS_HIP_TRY(hipMemcpyAsync(d_out, n->out, sizeof(float) * kOut * B, hipMemcpyDeviceToDevice, n->stream));
@@ -811,8 +815,10 @@ extern "C" int ctas_cnn_forward_dev(ctas_cnn *n, const uint16_t *d_x, int B, flo
 }
 
 // Loss and parameter gradients of a batch (no update); loss on the host.
-extern "C" int ctas_cnn_gradients_dev(ctas_cnn *n, const uint16_t *d_x, const float *d_target, int B, float *loss) {
+extern "C" int ctas_cnn_gradients_dev(ctas_cnn *n, const uint16_t *d_x, const float *d_target, int B, float *loss,
+                                      void *stream) {
   if (!n || !d_x || !d_target || B <= 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (int rc = stream_wait(n->stream, static_cast<hipStream_t>(stream))) return rc;
   if (int rc = ensure_batch(n, B)) return rc;
   if (int rc = forward(n, d_x, B)) return rc;
   hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), 0, n->stream, n->out, d_target, n->gout, n->loss, B);
@@ -827,8 +833,8 @@ extern "C" int ctas_cnn_gradients_dev(ctas_cnn *n, const uint16_t *d_x, const fl
 // clipping for clip_norm <= 0); grad_norm (optional) receives the norm
 // before clipping.
 extern "C" int ctas_cnn_train_step_dev(ctas_cnn *n, const uint16_t *d_x, const float *d_target, int B, float lr,
-                                       float clip_norm, float *loss, float *grad_norm) {
-  if (int rc = ctas_cnn_gradients_dev(n, d_x, d_target, B, loss)) return rc;
+                                       float clip_norm, float *loss, float *grad_norm, void *stream) {
+  if (int rc = ctas_cnn_gradients_dev(n, d_x, d_target, B, loss, stream)) return rc;
   ++n->step;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - std::pow(b1, static_cast<float>(n->step));
@@ -993,8 +999,9 @@ extern "C" int ctas_cnn_make_batch(uint64_t seed, uint64_t first, int B, uint16_
 
 // Offset (full-resolution px) and confidence of one device sinogram.
 extern "C" int ctas_cnn_predict_dev(ctas_cnn *n, const float *d_sino, int N, int w, float *offset,
-                                    float *confidence) {
+                                    float *confidence, void *stream) {
   if (!n || !d_sino || !offset || !confidence || N < 1 || w < 2) return fail(CTAS_E_ARG, "bad arguments");
+  if (int rc = stream_wait(n->stream, static_cast<hipStream_t>(stream))) return rc;
   if (int rc = ensure_batch(n, 1)) return rc;
   if (!n->xin && alloc(reinterpret_cast<void **>(&n->xin), 2 * kFeatA * kFeatW * 2)) return CTAS_E_NOMEM;
   CTAS_HIP_TRY(launch_features(d_sino, N, w, n->ftmp, n->xin, n->stream));
@@ -1015,7 +1022,7 @@ extern "C" int ctas_cnn_predict(ctas_cnn *n, const float *sino, int N, int w, fl
   int rc = CTAS_OK;
   if (hipMemcpy(d_s, sino, sizeof(float) * N * w, hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(CTAS_E_DEVICE, "copy failed");
-  if (!rc) rc = ctas_cnn_predict_dev(n, d_s, N, w, offset, confidence);
+  if (!rc) rc = ctas_cnn_predict_dev(n, d_s, N, w, offset, confidence, nullptr);
   (void)hipFree(d_s);
   return rc;
 }
@@ -1038,7 +1045,10 @@ extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, f
   const int K = static_cast<int>(std::floor(2.0 * half_range / step + 0.5)) + 1;
   if (K > 4096) return fail(CTAS_E_ARG, "too many trial centres");
   if (std::ceil(half_range) >= 0.5f * w) return fail(CTAS_E_CENTER, "trial centres beyond half the width");
-  // filtering on the handle's stream, the network on its own stream
+  // filtering on the handle's stream, the network on its own stream, both
+  // after the work already queued on the null stream
+  NullStreamJoin join(h->stream, nullptr);
+  if (join.rc) return join.rc;
   if (int rc = ctas_fbp_prepare_dev(h, d_sino, 1, h->stream)) return rc;
   const float cent = static_cast<float>(0.5 * (w - 1));
   auto grid = [&](int k) { return static_cast<float>(-static_cast<double>(half_range) + k * static_cast<double>(step)); };
@@ -1072,7 +1082,7 @@ extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, f
   float est = 0.0f, conf = 0.0f;
   int swept = 1, km = 0;
   if (net) {
-    if (int rc = ctas_cnn_predict_dev(net, d_sino, h->N, w, &est, &conf)) return rc;
+    if (int rc = ctas_cnn_predict_dev(net, d_sino, h->N, w, &est, &conf, nullptr)) return rc;
   }
   if (net && conf >= min_confidence) {
     swept = 0;
@@ -1116,7 +1126,7 @@ extern "C" int ctas_cnn_gemm_dev(const uint16_t *d_a, const uint16_t *d_b, float
                                  void *stream) {
   if (!d_a || !d_b || !d_c || M <= 0 || N <= 0 || K <= 0) return fail(CTAS_E_ARG, "bad arguments");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  CTAS_HIP_TRY(gemm(DenseA{d_a, K}, ActBT{d_b, N}, StoreDX{d_c, 1, 1, N}, M, N, K, s));
+  CTAS_HIP_TRY(gemm(DenseA{d_a, K}, ActBT{d_b, N}, StoreDX{d_c, 1, 1, N}, M, N, K, s));  // on the caller's stream
   CTAS_HIP_TRY(hipStreamSynchronize(s));
   return CTAS_OK;
 }

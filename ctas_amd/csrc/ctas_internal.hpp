@@ -112,6 +112,27 @@ hipError_t launch_phantom(int N, int w, double c, int n_ell, const double *d_ell
 
 int persistent_workgroups();
 
+// Makes work queued next on `waiter` start after the work already queued on
+// `producer` (nullptr: the null stream, whose work is ordered with every
+// blocking stream, torch's default stream included).
+int stream_wait(hipStream_t waiter, hipStream_t producer);
+
+// A _dev call with no caller stream runs on the library's own stream as if
+// it ran on the null stream: after the work already queued there, and
+// before any queued there later.
+struct NullStreamJoin {
+  hipStream_t own;
+  bool active;
+  int rc;
+  NullStreamJoin(hipStream_t own_stream, void *caller_stream)
+      : own(own_stream), active(caller_stream == nullptr), rc(0) {
+    if (active) rc = stream_wait(own, nullptr);
+  }
+  ~NullStreamJoin() {
+    if (active) (void)stream_wait(nullptr, own);
+  }
+};
+
 }  // namespace ctas
 
 // Handle (CTrec state). Defined here so that the C ABI units share it.

@@ -194,17 +194,20 @@ int ctas_cnn_set_params(ctas_cnn *net, const float *params);
 int ctas_cnn_get_grads(ctas_cnn *net, float *grads);
 int ctas_cnn_save(const ctas_cnn *net, const char *path);
 int ctas_cnn_load(ctas_cnn *net, const char *path);
-/* d_x [B][2][180][256] bf16 -> d_out [B][2] float32. */
-int ctas_cnn_forward_dev(ctas_cnn *net, const uint16_t *d_x, int B, float *d_out);
+/* The network's _dev calls read inputs made ready on `stream` (NULL: the
+ * null stream) and return when their results are complete.
+ * d_x [B][2][180][256] bf16 -> d_out [B][2] float32. */
+int ctas_cnn_forward_dev(ctas_cnn *net, const uint16_t *d_x, int B, float *d_out, void *stream);
 /* d_target [B][2] = {offset (bins), tolerance (bins)}: loss = mean of
  * Huber(offset error, 0.25) + 0.2 BCE(logit, |error| < tolerance). */
 int ctas_cnn_gradients_dev(ctas_cnn *net, const uint16_t *d_x, const float *d_target, int B,
-                           float *loss);
+                           float *loss, void *stream);
 /* gradients + one Adam step (beta 0.9 / 0.999, eps 1e-8) at rate lr, with
  * the gradient scaled to global norm clip_norm when larger (clip_norm <= 0:
  * no clipping); grad_norm (optional) gets the norm before clipping. */
 int ctas_cnn_train_step_dev(ctas_cnn *net, const uint16_t *d_x, const float *d_target, int B,
-                            float lr, float clip_norm, float *loss, float *grad_norm);
+                            float lr, float clip_norm, float *loss, float *grad_norm,
+                            void *stream);
 /* Network input of one device sinogram [N][w] (transmission). */
 int ctas_cnn_features_dev(const float *d_sino, int N, int w, uint16_t *d_x, void *stream);
 /* Training sample `index` of stream `seed` (w, N, offset c, photons N0 and
@@ -217,7 +220,7 @@ int ctas_cnn_make_batch(uint64_t seed, uint64_t first, int B, uint16_t *d_x, flo
                         float *offsets_px, int *widths_px, void *stream);
 /* Estimate (px) and confidence in [0, 1] for one sinogram. */
 int ctas_cnn_predict_dev(ctas_cnn *net, const float *d_sino, int n_angles, int w, float *offset,
-                         float *confidence);
+                         float *confidence, void *stream);
 int ctas_cnn_predict(ctas_cnn *net, const float *sino, int n_angles, int w, float *offset,
                      float *confidence);
 /* Sweep of trial centres -half_range + k*step (AverageGradient score,
