@@ -31,6 +31,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "ctas_internal.hpp"
 
@@ -73,13 +74,6 @@ __device__ __forceinline__ float load_val(const void *base, size_t i) {
 template <int MODE> struct WinT { using type = float; };
 template <> struct WinT<MODE_LINEAR> { using type = float2; };
 
-// LDS needed by one tile kind (window + per-angle constants + lo table).
-template <int KIND, int AC, int MODE>
-constexpr size_t lds_bytes() {
-  return sizeof(typename WinT<MODE>::type) * AC * KindGeo<KIND>::L + sizeof(float4) * AC +
-         sizeof(int) * AC;
-}
-
 // True when every pixel of the block rows [r0, r0+nr) x cols [c0, c0+nc)
 // lies outside the reconstruction disc (radius at most hp + |c| around the
 // axis point (hp, hp + c)) or the column band, so that all are masked.
@@ -103,18 +97,253 @@ __device__ __forceinline__ int find_item_by_tile(const Item *items, int n, int b
   return lo;
 }
 
-// One output tile of one item.
+// Zeros of a tile that lies wholly outside the disc or the column band.
+template <int KIND>
+__device__ __forceinline__ bool tile_outside(const Item &it, int ty0, int tx0) {
+  using G = KindGeo<KIND>;
+  return wave_outside(it, it.center, ty0, G::RT, tx0, G::CT);
+}
+
+// Masks and scaling of the accumulated sums, in the reference's float
+// arithmetic (lct.cpp:710-727); row(r) and col(q) give the pixel of
+// accumulator [r][q].
+template <int NR, int NQ, class RowF, class ColF>
+__device__ __forceinline__ void store_tile(const Item &it, const float (&acc)[NR][NQ], RowF rowf, ColF colf) {
+  const int w = it.w, hp = w / 2;
+  const float c = it.center;
+  const float acent = fabsf(c);
+  const float up = static_cast<float>(hp) + c, dn = static_cast<float>(hp) - c;
+  const float recCof = it.recCof;
+  float *__restrict__ out = it.out;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int row = rowf(r);
+    if (row >= w) continue;
+    float dy = static_cast<float>(row - hp) - c;
+    dy *= dy;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int col = colf(q);
+      if (col >= w) continue;
+      int ip2 = col - hp;
+      ip2 *= ip2;
+      const float d2 = dy + static_cast<float>(ip2);
+      const float fr = static_cast<float>(row), fc = static_cast<float>(col);
+      const bool masked = (fc <= acent) || (fc >= static_cast<float>(w) - acent - 1.0f) ||
+                          (fr <= up && d2 >= up * up) || (fr >= up && d2 >= dn * dn);
+      float v = acc[r][q];
+      if (recCof != 0.0f) v *= recCof;
+      v = v / static_cast<float>(it.N);
+      out[static_cast<size_t>(row) * w + col] = masked ? 0.0f : v;
+    }
+  }
+}
+
+template <int NR, int NQ, class RowF, class ColF>
+__device__ __forceinline__ void zero_tile(const Item &it, RowF rowf, ColF colf) {
+  const int w = it.w;
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int row = rowf(r), col = colf(q);
+      if (row < w && col < w) it.out[static_cast<size_t>(row) * w + col] = 0.0f;
+    }
+}
+
+// ---- staging of the angle windows, software-pipelined ----------------------
+// The angles are taken in passes of AC. While pass p is computed from one
+// LDS buffer, the windows of pass p+1 are fetched from global memory into
+// registers and written to the other buffer afterwards, so that the fetch
+// latency hides behind the compute and each pass costs one barrier. Thread
+// t stages angle t / TPA of the pass (it forms that angle's constants
+// itself) and its window elements sub, sub + TPA, ... (sub = t % TPA).
+//
+// Window of an angle: L samples starting at lo = floor(ray coordinate of the
+// tile centre) - HALF - 2, addressed as the reference does (flat offset
+// theta*w + k of the slice; 0 outside the slice) in CTAS mode, and as
+// (q0, q1 - q0) pairs of bin-centre samples (0 outside the row) in LINEAR
+// mode.
+// Loads through a global (address space 1) pointer. The item table holds
+// generic pointers, from which the compiler would emit FLAT loads; those
+// also count against the LDS wait counter, so every LDS wait of the compute
+// loop would wait for the prefetch too.
+template <class T>
+__device__ __forceinline__ T ldg(const T *p, long long i) {
+  return ((const __attribute__((address_space(1))) T *)(p))[i];
+}
+__device__ __forceinline__ float2 ldg(const float2 *p, long long i) {
+  const float *q = reinterpret_cast<const float *>(p) + 2 * i;
+  return make_float2(ldg(q, 0), ldg(q, 1));
+}
+__device__ __forceinline__ double2 ldg(const double2 *p, long long i) {
+  const double *q = reinterpret_cast<const double *>(p) + 2 * i;
+  return make_double2(ldg(q, 0), ldg(q, 1));
+}
+
+template <int ST> struct Raw { using type = uint16_t; };
+template <> struct Raw<ST_F32> { using type = float; };
+
+template <int ST>
+__device__ __forceinline__ float raw_to_float(typename Raw<ST>::type r) {
+  if constexpr (ST == ST_F32)
+    return r;
+  else if constexpr (ST == ST_F16)
+    return static_cast<float>(__builtin_bit_cast(_Float16, r));
+  else
+    return __uint_as_float(static_cast<uint32_t>(r) << 16);
+}
+
+// The loads of a fetch are unconditional (indices clamped into the slice or
+// row, the sample zeroed at store time when it was outside), so that no
+// branch forces a wait on them before the compute of the current pass.
+template <int KIND, int AC, int MODE, int ST>
+struct StageFloat {
+  using G = KindGeo<KIND>;
+  using WT = typename WinT<MODE>::type;
+  using R = typename Raw<ST>::type;
+  static constexpr int L = G::L, TPA = 256 / AC, EPT = (L + TPA - 1) / TPA;
+  static constexpr int NV = MODE == MODE_CTAS ? 1 : 2;  // raw samples per element
+  // FP64 angle table except in the parity mode (CTAS sampling, float32).
+  static constexpr bool F64 = !(MODE == MODE_CTAS && ST == ST_F32);
+  using AT = typename std::conditional<F64, double2, float2>::type;
+  static constexpr size_t WIN = sizeof(WT) * AC * L;
+  static constexpr size_t HALFBUF = WIN + sizeof(float4) * AC;
+  // two window/constant buffers, then two slots of AC table entries
+  static constexpr size_t ANGS = 2 * HALFBUF;
+  static constexpr size_t BYTES = ANGS + 2 * sizeof(AT) * AC;
+  static_assert(NV * EPT <= 64, "one validity bit per raw sample");
+  R v[EPT][NV];
+  uint64_t ok;  // bit (NV j + i): raw sample i of element j lies inside
+  float4 prm;
+  AT ang, ang_next;  // table entries of the pass being fetched and of the one after
+
+  // Branch-free: threads past the pass's last angle repeat that angle (their
+  // results are not stored), so no branch makes the compiler wait on a load.
+  __device__ __forceinline__ void load_angle(const Item &it, int a0, int na) {
+    const int a = min(static_cast<int>(threadIdx.x) / TPA, na - 1);
+    if constexpr (F64)
+      ang_next = ldg(it.sincos64, a0 + a);
+    else
+      ang_next = ldg(it.sincos, a0 + a);
+  }
+  // The entry loaded with the previous fetch reaches the next one through
+  // LDS, so that no load stays pending across a pass.
+  __device__ __forceinline__ void put_angle(unsigned char *lds, int slot, int na) const {
+    const int tid = threadIdx.x, a = tid / TPA;
+    if (a < na && tid - a * TPA == 0) reinterpret_cast<AT *>(lds + ANGS)[slot * AC + a] = ang_next;
+  }
+  __device__ __forceinline__ void get_angle(const unsigned char *lds, int slot, int na) {
+    ang = reinterpret_cast<const AT *>(lds + ANGS)[slot * AC + min(static_cast<int>(threadIdx.x) / TPA, na - 1)];
+  }
+  __device__ __forceinline__ void take_loaded() { ang = ang_next; }
+
+  __device__ __forceinline__ void fetch(const Item &it, int a0, int na, float ycm, float xcm) {
+    const int tid = threadIdx.x, a = min(tid / TPA, na - 1), sub = tid - (tid / TPA) * TPA;
+    const int w = it.w, hp = w / 2;
+    const float c = it.center;
+    float s, co, base;
+    if constexpr (F64) {
+      // precision modes: angle constants and centre shift in FP64
+      double bd = static_cast<double>(c) + static_cast<double>(hp) * ((1.0 - ang.x) - ang.y);
+      if (MODE == MODE_LINEAR) bd += 0.5;
+      s = static_cast<float>(ang.x);
+      co = static_cast<float>(ang.y);
+      base = static_cast<float>(bd);
+    } else {
+      s = ang.x;
+      co = ang.y;
+      base = c + ((1.0f - s) - co) * static_cast<float>(hp);
+      if (MODE == MODE_LINEAR) base = base + 0.5f;
+    }
+    const float tc = base + s * ycm + co * xcm;
+    const int lo = static_cast<int>(floorf(tc)) - G::HALF - 2;
+    prm = make_float4(base, s, co, __int_as_float((a * L - lo) * static_cast<int>(sizeof(WT))));
+    const R *__restrict__ filt = static_cast<const R *>(it.filt);
+    ok = 0;
+    if constexpr (MODE == MODE_CTAS) {
+      const long long total = static_cast<long long>(it.N) * w;
+      const long long rb = static_cast<long long>(a0 + a) * w + lo;
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const long long flat = rb + sub + j * TPA;
+        const bool in = flat >= 0 && flat < total;
+        v[j][0] = ldg(filt, in ? flat : 0);
+        ok |= static_cast<uint64_t>(in) << j;
+      }
+    } else {
+      const size_t rb = static_cast<size_t>(a0 + a) * w;
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const int k = lo + sub + j * TPA;
+        const bool in1 = k >= 0 && k < w, in0 = k - 1 >= 0 && k - 1 < w;
+        v[j][1] = ldg(filt, static_cast<long long>(rb) + (in1 ? k : 0));
+        v[j][0] = ldg(filt, static_cast<long long>(rb) + (in0 ? k - 1 : 0));
+        ok |= (static_cast<uint64_t>(in0) | (static_cast<uint64_t>(in1) << 1)) << (2 * j);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(unsigned char *buf, int na) const {
+    const int tid = threadIdx.x, a = tid / TPA, sub = tid - a * TPA;
+    if (a >= na) return;
+    WT *win = reinterpret_cast<WT *>(buf);
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int o = sub + j * TPA;
+      if (o >= L) continue;
+      if constexpr (MODE == MODE_CTAS) {
+        win[a * L + o] = ((ok >> j) & 1u) ? raw_to_float<ST>(v[j][0]) : 0.0f;
+      } else {
+        const float q0 = ((ok >> (2 * j)) & 1u) ? raw_to_float<ST>(v[j][0]) : 0.0f;
+        const float q1 = ((ok >> (2 * j + 1)) & 1u) ? raw_to_float<ST>(v[j][1]) : 0.0f;
+        win[a * L + o] = make_float2(q0, q1 - q0);
+      }
+    }
+    if (sub == 0) reinterpret_cast<float4 *>(buf + WIN)[a] = prm;
+  }
+};
+
+// Drives the passes: fetch(p+1) / compute(p) / store(p+1) / barrier.
+template <class Stage, class Compute>
+__device__ __forceinline__ void run_passes(const Item &it, unsigned char *lds, int AC, float ycm, float xcm,
+                                           bool wskip, Compute compute) {
+  Stage st;
+  const int N = it.N;
+  const int P = (N + AC - 1) / AC;
+  st.load_angle(it, 0, min(AC, N));
+  st.take_loaded();
+  st.fetch(it, 0, min(AC, N), ycm, xcm);
+  st.load_angle(it, min(1, P - 1) * AC, min(AC, N - min(1, P - 1) * AC));
+  st.store(lds, min(AC, N));
+  st.put_angle(lds, 1, min(AC, N - min(1, P - 1) * AC));
+  __syncthreads();
+  for (int p = 0; p < P; ++p) {
+    const int a0 = p * AC, na = min(AC, N - a0);
+    // The fetches run in every iteration (past the end they repeat the last
+    // pass into the buffer nobody reads), so that every load issued here is
+    // consumed by this iteration's stores on every path and none stays
+    // pending into the next iteration, where the compiler would wait on it.
+    const int p1 = min(p + 1, P - 1), p2 = min(p + 2, P - 1);
+    const int na1 = min(AC, N - p1 * AC), na2 = min(AC, N - p2 * AC);
+    st.get_angle(lds, (p + 1) & 1, na1);
+    st.fetch(it, p1 * AC, na1, ycm, xcm);
+    st.load_angle(it, p2 * AC, na2);
+    if (!wskip) compute(lds + (p & 1) * Stage::HALFBUF, na);
+    st.store(lds + ((p + 1) & 1) * Stage::HALFBUF, na1);
+    st.put_angle(lds, p & 1, na2);
+    __syncthreads();
+  }
+}
+
+// One output tile of one item, float LDS windows (FP32 storage, and LINEAR
+// mode at any storage).
 template <int KIND, int AC, int MODE, int ST>
 __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char *lds) {
   using G = KindGeo<KIND>;
-  using WT = typename WinT<MODE>::type;
-  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT, L = G::L;
-  WT *win = reinterpret_cast<WT *>(lds);
-  float4 *prm = reinterpret_cast<float4 *>(lds + sizeof(WT) * AC * L);
-  int *los = reinterpret_cast<int *>(lds + sizeof(WT) * AC * L + sizeof(float4) * AC);
-
-  const int w = it.w, N = it.N;
-  const int hp = w / 2;
+  using S = StageFloat<KIND, AC, MODE, ST>;
+  using WT = typename S::WT;
+  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT;
   const float c = it.center;
   const int ty0 = (tile / it.tiles_x) * RT;
   const int tx0 = (tile % it.tiles_x) * CT;
@@ -123,101 +352,32 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
   // (v 4TR + ty4 + 4r, tx + 16q): one LDS read instruction covers a 4 x 16
   // pixel block, and a wave whose rows lie outside the disc skips its work
   const int tx = tid & 15, ty = (tid >> 6) * (4 * TR) + ((tid >> 4) & 3);
-  float *__restrict__ out = it.out;
-
-  // Tiles that lie wholly outside the reconstruction disc (radius at most
-  // hp + |c| around the axis point (hp, hp + c)) or the column band are
-  // zero, as every pixel of them is masked.
-  {
-    const float acent = fabsf(c);
-    const float ax = static_cast<float>(hp), ay = static_cast<float>(hp) + c;
-    const float dx = fmaxf(fmaxf(static_cast<float>(tx0) - ax, ax - static_cast<float>(tx0 + CT - 1)), 0.0f);
-    const float dy = fmaxf(fmaxf(static_cast<float>(ty0) - ay, ay - static_cast<float>(ty0 + RT - 1)), 0.0f);
-    const float rmax = static_cast<float>(hp) + acent + 2.0f;
-    const bool outside = (dx * dx + dy * dy > rmax * rmax) ||
-                         (static_cast<float>(tx0 + CT - 1) <= acent) ||
-                         (static_cast<float>(tx0) >= static_cast<float>(w) - acent - 1.0f);
-    if (outside) {
-#pragma unroll
-      for (int r = 0; r < TR; ++r) {
-        const int row = ty0 + ty + 4 * r;
-#pragma unroll
-        for (int q = 0; q < TC; ++q) {
-          const int col = tx0 + tx + 16 * q;
-          if (row < w && col < w) out[static_cast<size_t>(row) * w + col] = 0.0f;
-        }
-      }
-      return;
-    }
+  auto rowf = [&](int r) { return ty0 + ty + 4 * r; };
+  auto colf = [&](int q) { return tx0 + tx + 16 * q; };
+  if (tile_outside<KIND>(it, ty0, tx0)) {
+    zero_tile<TR, TC>(it, rowf, colf);
+    return;
   }
-
   float jc[TR], fi[TC];
 #pragma unroll
-  for (int r = 0; r < TR; ++r) jc[r] = static_cast<float>(ty0 + ty + 4 * r) - c;
+  for (int r = 0; r < TR; ++r) jc[r] = static_cast<float>(rowf(r)) - c;
 #pragma unroll
-  for (int q = 0; q < TC; ++q) fi[q] = static_cast<float>(tx0 + tx + 16 * q);
+  for (int q = 0; q < TC; ++q) fi[q] = static_cast<float>(colf(q));
   float acc[TR][TC];
 #pragma unroll
   for (int r = 0; r < TR; ++r)
 #pragma unroll
     for (int q = 0; q < TC; ++q) acc[r][q] = 0.0f;
-
   const float ycm = static_cast<float>(ty0) + 0.5f * (RT - 1) - c;
   const float xcm = static_cast<float>(tx0) + 0.5f * (CT - 1);
-  const void *__restrict__ filt = it.filt;
   const bool wskip = wave_outside(it, c, ty0 + (tid >> 6) * (4 * TR), 4 * TR, tx0, CT);
-  const long long total = static_cast<long long>(N) * w;
 
-  for (int a0 = 0; a0 < N; a0 += AC) {
-    const int na = min(AC, N - a0);
-    __syncthreads();  // the previous pass is done with the LDS
-    if (tid < na) {
-      float s, co, base;
-      if (it.sincos64) {
-        // precision modes: angle constants and centre shift in FP64
-        const double2 sd = it.sincos64[a0 + tid];
-        double bd = static_cast<double>(c) + static_cast<double>(hp) * ((1.0 - sd.x) - sd.y);
-        if (MODE == MODE_LINEAR) bd += 0.5;
-        s = static_cast<float>(sd.x);
-        co = static_cast<float>(sd.y);
-        base = static_cast<float>(bd);
-      } else {
-        const float2 sc = it.sincos[a0 + tid];
-        s = sc.x;
-        co = sc.y;
-        base = c + ((1.0f - s) - co) * static_cast<float>(hp);
-        if (MODE == MODE_LINEAR) base = base + 0.5f;
-      }
-      const float tc = base + s * ycm + co * xcm;
-      const int lo = static_cast<int>(floorf(tc)) - G::HALF - 2;
-      los[tid] = lo;
-      const int offb = (tid * L - lo) * static_cast<int>(sizeof(WT));
-      prm[tid] = make_float4(base, s, co, __int_as_float(offb));
-    }
-    __syncthreads();
-    // Stage the windows.
-    for (int e = tid; e < na * L; e += 256) {
-      const int a = e / L;
-      const int o = e - a * L;
-      const int k = los[a] + o;
-      const int ang = a0 + a;
-      if constexpr (MODE == MODE_CTAS) {
-        const long long flat = static_cast<long long>(ang) * w + k;
-        win[e] = (flat >= 0 && flat < total) ? load_val<ST>(filt, static_cast<size_t>(flat)) : 0.0f;
-      } else {
-        const size_t rb = static_cast<size_t>(ang) * w;
-        const float q1 = (k >= 0 && k < w) ? load_val<ST>(filt, rb + k) : 0.0f;
-        const float q0 = (k - 1 >= 0 && k - 1 < w) ? load_val<ST>(filt, rb + k - 1) : 0.0f;
-        win[e] = make_float2(q0, q1 - q0);
-      }
-    }
-    __syncthreads();
-    if (wskip) continue;
-    const unsigned char *wb0 = reinterpret_cast<const unsigned char *>(win);
+  run_passes<S>(it, lds, AC, ycm, xcm, wskip, [&](const unsigned char *buf, int na) {
+    const float4 *prm = reinterpret_cast<const float4 *>(buf + S::WIN);
 #pragma unroll 2
     for (int a = 0; a < na; ++a) {
       const float4 p = prm[a];
-      const unsigned char *wb = wb0 + __float_as_int(p.w);
+      const unsigned char *wb = buf + __float_as_int(p.w);
       float aj[TR], ci[TC];
 #pragma unroll
       for (int r = 0; r < TR; ++r) aj[r] = p.x + p.y * jc[r];
@@ -237,14 +397,14 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
 #pragma unroll
           for (int q = 0; q < TC; ++q) acc[r][q] += vv[q];
         } else {
-          float2 vv[TC];
+          WT vv[TC];
           float ff[TC];
 #pragma unroll
           for (int q = 0; q < TC; ++q) {
             const float t = aj[r] + ci[q];
             const int k = static_cast<int>(t);
             ff[q] = __builtin_amdgcn_fractf(t);
-            vv[q] = *reinterpret_cast<const float2 *>(wb + (k << 3));
+            vv[q] = *reinterpret_cast<const WT *>(wb + (k << 3));
           }
 #pragma unroll
           for (int q = 0; q < TC; ++q) {
@@ -254,345 +414,174 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
         }
       }
     }
-  }
-
-  // Masks and scaling, in the reference's float arithmetic.
-  const float acent = fabsf(c);
-  const float up = static_cast<float>(hp) + c, dn = static_cast<float>(hp) - c;
-  const float recCof = it.recCof;
-#pragma unroll
-  for (int r = 0; r < TR; ++r) {
-    const int row = ty0 + ty + 4 * r;
-    if (row >= w) continue;
-    float dy = static_cast<float>(row - hp) - c;
-    dy *= dy;
-#pragma unroll
-    for (int q = 0; q < TC; ++q) {
-      const int col = tx0 + tx + 16 * q;
-      if (col >= w) continue;
-      int ip2 = col - hp;
-      ip2 *= ip2;
-      const float d2 = dy + static_cast<float>(ip2);
-      const float fr = static_cast<float>(row), fc = static_cast<float>(col);
-      const bool masked = (fc <= acent) || (fc >= static_cast<float>(w) - acent - 1.0f) ||
-                          (fr <= up && d2 >= up * up) || (fr >= up && d2 >= dn * dn);
-      float v = acc[r][q];
-      if (recCof != 0.0f) v *= recCof;
-      v = v / static_cast<float>(N);
-      out[static_cast<size_t>(row) * w + col] = masked ? 0.0f : v;
-    }
-  }
+  });
+  store_tile<TR, TC>(it, acc, rowf, colf);
 }
 
-// One output tile of one item, FP16/BF16 storage with CTAS (floor)
-// sampling in the precision path (FP64 angle constants). The LDS windows
-// hold the 16-bit samples as stored. The ray coordinate of each pixel is a
-// 16.16 fixed-point integer: per angle, the tile's base term (T0 - lo + the
-// window slot offset, from FP64) plus a row term and a column term, each
-// advanced by an integer increment, so that each sample costs one integer
-// add and one shift (the byte offset is (u >> 16) << 1). Two angles are
-// summed into a pixel's float accumulator with one dot2 (f16 or bf16 pair
-// times (1, 1)). Accuracy: the fixed-point coordinate is within ~1e-4 px of
-// the FP64 one, finer than the float arithmetic of the parity path.
+// ---- FP16/BF16 storage, CTAS (floor) sampling ------------------------------
+// LDS holds, for every window position o, the pair (v[o], v[o+1]) as two
+// floats (converted from the stored 16-bit samples when staged). A lane owns
+// vertically adjacent pixel pairs (rows j, j+1): their ray coordinates
+// differ by s = sin(theta), so with floor sampling row j+1 reads v[k] or
+// v[k+1] (k = floor(t_j)) for s >= 0, v[k-1] or v[k] for s < 0 (the pair is
+// then taken at k-1). One 8-byte LDS read serves both pixels; half as many
+// LDS reads as the float path, and each sample costs an integer add, a
+// compare and a select instead of float arithmetic and a conversion. The
+// ray coordinate of pixel (y, x) of the tile is bt + S y + C x in 16.16
+// fixed point, bt, S and C formed per angle from FP64 constants (within
+// ~1e-4 px of the FP64 coordinate, finer than the float arithmetic of the
+// parity path).
 template <int ST>
-struct Pair16;
-template <>
-struct Pair16<ST_F16> {
-  typedef _Float16 v2 __attribute__((ext_vector_type(2)));
-  static __device__ __forceinline__ float dot(v2 a, float acc) {
-    const v2 one = {static_cast<_Float16>(1.0f), static_cast<_Float16>(1.0f)};
-    return __builtin_amdgcn_fdot2(a, one, acc, false);
-  }
-  static __device__ __forceinline__ v2 make(uint16_t lo, uint16_t hi) {
-    return v2{__builtin_bit_cast(_Float16, lo), __builtin_bit_cast(_Float16, hi)};
-  }
-};
-template <>
-struct Pair16<ST_BF16> {
-  typedef __bf16 v2 __attribute__((ext_vector_type(2)));
-  static __device__ __forceinline__ float dot(v2 a, float acc) {
-    const v2 one = {static_cast<__bf16>(1.0f), static_cast<__bf16>(1.0f)};
-    return __builtin_amdgcn_fdot2_f32_bf16(a, one, acc, false);
-  }
-  static __device__ __forceinline__ v2 make(uint16_t lo, uint16_t hi) {
-    return v2{__builtin_bit_cast(__bf16, lo), __builtin_bit_cast(__bf16, hi)};
-  }
-};
-
-template <int KIND, int AC>
-constexpr size_t lds_bytes_fx() {
-  return sizeof(uint16_t) * AC * KindGeo<KIND>::L + sizeof(int4) * AC + sizeof(int) * AC;
+__device__ __forceinline__ float half_to_float(uint32_t h) {
+  if constexpr (ST == ST_F16)
+    return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h)));
+  else
+    return __uint_as_float(h << 16);
 }
 
 template <int KIND, int AC, int ST>
-__device__ __forceinline__ void bp_tile_fx(const Item &it, int tile, unsigned char *lds) {
+struct StagePair {
   using G = KindGeo<KIND>;
-  using P = Pair16<ST>;
-  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT, L = G::L;
-  static_assert(AC % 2 == 0, "angles are summed in pairs");
-  uint16_t *win = reinterpret_cast<uint16_t *>(lds);
-  int4 *prm = reinterpret_cast<int4 *>(lds + sizeof(uint16_t) * AC * L);
-  int *los = reinterpret_cast<int *>(lds + sizeof(uint16_t) * AC * L + sizeof(int4) * AC);
+  using WT = float2;
+  static constexpr int L = G::L, TPA = 256 / AC, EPT = (L + TPA - 1) / TPA;
+  static constexpr size_t WIN = sizeof(WT) * AC * L;
+  static constexpr size_t HALFBUF = WIN + 2 * sizeof(int4) * AC;
+  static constexpr size_t ANGS = 2 * HALFBUF;
+  static constexpr size_t BYTES = ANGS + 2 * sizeof(double2) * AC;
+  static_assert(2 * EPT <= 64, "one validity bit per sample");
+  uint16_t v[EPT][2];
+  uint64_t ok;  // bit (2 j + i): sample i of pair j lies inside the slice
+  int4 prm0, prm1;
+  double2 sd, sd_next;  // table entries of the pass being fetched and of the one after
 
-  const int w = it.w, N = it.N;
-  const int hp = w / 2;
-  const float c = it.center;
-  const int ty0 = (tile / it.tiles_x) * RT;
-  const int tx0 = (tile % it.tiles_x) * CT;
-  const int tid = threadIdx.x;
-  // wave v owns the tile rows [v 4TR, (v+1) 4TR); lane (ty4, tx) the pixels
-  // (v 4TR + ty4 + 4r, tx + 16q): one LDS read instruction covers a 4 x 16
-  // pixel block, and a wave whose rows lie outside the disc skips its work
-  const int tx = tid & 15, ty = (tid >> 6) * (4 * TR) + ((tid >> 4) & 3);
-  float *__restrict__ out = it.out;
-  {
-    const float acent = fabsf(c);
-    const float ax = static_cast<float>(hp), ay = static_cast<float>(hp) + c;
-    const float dx = fmaxf(fmaxf(static_cast<float>(tx0) - ax, ax - static_cast<float>(tx0 + CT - 1)), 0.0f);
-    const float dy = fmaxf(fmaxf(static_cast<float>(ty0) - ay, ay - static_cast<float>(ty0 + RT - 1)), 0.0f);
-    const float rmax = static_cast<float>(hp) + acent + 2.0f;
-    const bool outside = (dx * dx + dy * dy > rmax * rmax) ||
-                         (static_cast<float>(tx0 + CT - 1) <= acent) ||
-                         (static_cast<float>(tx0) >= static_cast<float>(w) - acent - 1.0f);
-    if (outside) {
-#pragma unroll
-      for (int r = 0; r < TR; ++r) {
-        const int row = ty0 + ty + 4 * r;
-#pragma unroll
-        for (int q = 0; q < TC; ++q) {
-          const int col = tx0 + tx + 16 * q;
-          if (row < w && col < w) out[static_cast<size_t>(row) * w + col] = 0.0f;
-        }
-      }
-      return;
-    }
+  __device__ __forceinline__ void load_angle(const Item &it, int a0, int na) {
+    sd_next = ldg(it.sincos64, a0 + min(static_cast<int>(threadIdx.x) / TPA, na - 1));
   }
-  float acc[TR][TC];
-#pragma unroll
-  for (int r = 0; r < TR; ++r)
-#pragma unroll
-    for (int q = 0; q < TC; ++q) acc[r][q] = 0.0f;
-  const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
-  const long long total = static_cast<long long>(N) * w;
-  const float fty = static_cast<float>(ty), ftx = static_cast<float>(tx);
-  const bool wskip = wave_outside(it, c, ty0 + (tid >> 6) * (4 * TR), 4 * TR, tx0, CT);
+  __device__ __forceinline__ void put_angle(unsigned char *lds, int slot, int na) const {
+    const int tid = threadIdx.x, a = tid / TPA;
+    if (a < na && tid - a * TPA == 0) reinterpret_cast<double2 *>(lds + ANGS)[slot * AC + a] = sd_next;
+  }
+  __device__ __forceinline__ void get_angle(const unsigned char *lds, int slot, int na) {
+    sd = reinterpret_cast<const double2 *>(lds + ANGS)[slot * AC + min(static_cast<int>(threadIdx.x) / TPA, na - 1)];
+  }
+  __device__ __forceinline__ void take_loaded() { sd = sd_next; }
 
-  for (int a0 = 0; a0 < N; a0 += AC) {
-    const int na = min(AC, N - a0);
-    const int np = (na + 1) & ~1;  // slots used, padded to a pair
-    __syncthreads();
-    if (tid < np) {
-      // an odd pass ends with an all-zero slot that reuses the last angle's
-      // geometry
-      const int ang = a0 + min(tid, na - 1);
-      const double2 sd = it.sincos64[ang];
-      const double cd = static_cast<double>(c);
-      const double T0 = cd + hp * ((1.0 - sd.x) - sd.y) + sd.x * (ty0 - cd) + sd.y * tx0;
-      const double tc = T0 + sd.x * (0.5 * (RT - 1)) + sd.y * (0.5 * (CT - 1));
-      const int lo = static_cast<int>(floor(tc)) - G::HALF - 2;
-      los[tid] = (tid < na) ? lo : INT_MIN;
-      const int bt = static_cast<int>(llrint((T0 - lo + static_cast<double>(tid) * L) * 65536.0));
-      const float Ss = static_cast<float>(sd.x * 65536.0), Cs = static_cast<float>(sd.y * 65536.0);
-      prm[tid] = make_int4(bt, __float_as_int(Ss), __float_as_int(Cs), 0);
-    }
-    __syncthreads();
-    for (int e = tid; e < np * L; e += 256) {
-      const int a = e / L;
-      const int o = e - a * L;
-      const int lo = los[a];
-      uint16_t v = 0;
-      if (lo != INT_MIN) {
-        const long long flat = static_cast<long long>(a0 + a) * w + lo + o;
-        if (flat >= 0 && flat < total) v = filt[flat];
-      }
-      win[e] = v;
-    }
-    __syncthreads();
-    if (wskip) continue;
-    const unsigned char *wb = reinterpret_cast<const unsigned char *>(win);
-    for (int a = 0; a < np; a += 2) {
-      int A0[TR], A1[TR], C0[TC], C1[TC];
-      {
-        const int4 p = prm[a];
-        const float Ss = __int_as_float(p.y), Cs = __int_as_float(p.z);
-        const int dS = static_cast<int>(Ss * 4.0f), dC = static_cast<int>(Cs * 16.0f);
-        A0[0] = p.x + static_cast<int>(Ss * fty);
+  __device__ __forceinline__ void fetch(const Item &it, int a0, int na, int ty0, int tx0) {
+    const int tid = threadIdx.x, a = min(tid / TPA, na - 1), sub = tid - (tid / TPA) * TPA;
+    const int w = it.w, hp = w / 2;
+    const double cd = static_cast<double>(it.center);
+    const double T0 = cd + hp * ((1.0 - sd.x) - sd.y) + sd.x * (ty0 - cd) + sd.y * tx0;
+    const double tc = T0 + sd.x * (0.5 * (G::RT - 1)) + sd.y * (0.5 * (G::CT - 1));
+    const int lo = static_cast<int>(floor(tc)) - G::HALF - 2;
+    const bool neg = sd.x < 0.0;
+    const int bt = static_cast<int>(llrint((T0 - lo + static_cast<double>(a) * L) * 65536.0)) - (neg ? 65536 : 0);
+    const int S = static_cast<int>(llrint(sd.x * 65536.0)), C = static_cast<int>(llrint(sd.y * 65536.0));
+    const int thr = neg ? -S : 65536 - S;
+    prm0 = make_int4(bt, S, C, thr | (neg ? (1 << 30) : 0));
+    prm1 = make_int4(8 * S, 16 * C, 0, 0);
+    const long long total = static_cast<long long>(it.N) * w;
+    const long long rb = static_cast<long long>(a0 + a) * w + lo;
+    const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
+    ok = 0;
 #pragma unroll
-        for (int r = 1; r < TR; ++r) A0[r] = A0[r - 1] + dS;
-        C0[0] = static_cast<int>(Cs * ftx);
-#pragma unroll
-        for (int q = 1; q < TC; ++q) C0[q] = C0[q - 1] + dC;
-      }
-      {
-        const int4 p = prm[a + 1];
-        const float Ss = __int_as_float(p.y), Cs = __int_as_float(p.z);
-        const int dS = static_cast<int>(Ss * 4.0f), dC = static_cast<int>(Cs * 16.0f);
-        A1[0] = p.x + static_cast<int>(Ss * fty);
-#pragma unroll
-        for (int r = 1; r < TR; ++r) A1[r] = A1[r - 1] + dS;
-        C1[0] = static_cast<int>(Cs * ftx);
-#pragma unroll
-        for (int q = 1; q < TC; ++q) C1[q] = C1[q - 1] + dC;
-      }
-#pragma unroll
-      for (int r = 0; r < TR; ++r) {
-        typename P::v2 v[TC];
-#pragma unroll
-        for (int q = 0; q < TC; ++q) {
-          const unsigned u0 = static_cast<unsigned>(A0[r] + C0[q]);
-          const unsigned u1 = static_cast<unsigned>(A1[r] + C1[q]);
-          const uint16_t x0 = *reinterpret_cast<const uint16_t *>(wb + ((u0 >> 16) << 1));
-          const uint16_t x1 = *reinterpret_cast<const uint16_t *>(wb + ((u1 >> 16) << 1));
-          v[q] = P::make(x0, x1);
-        }
-#pragma unroll
-        for (int q = 0; q < TC; ++q) acc[r][q] = P::dot(v[q], acc[r][q]);
-      }
+    for (int j = 0; j < EPT; ++j) {
+      const long long flat = rb + sub + j * TPA;
+      const bool in0 = flat >= 0 && flat < total, in1 = flat + 1 >= 0 && flat + 1 < total;
+      v[j][0] = ldg(filt, in0 ? flat : 0);
+      v[j][1] = ldg(filt, in1 ? flat + 1 : 0);
+      ok |= (static_cast<uint64_t>(in0) | (static_cast<uint64_t>(in1) << 1)) << (2 * j);
     }
   }
 
-  const float acent = fabsf(c);
-  const float up = static_cast<float>(hp) + c, dn = static_cast<float>(hp) - c;
-  const float recCof = it.recCof;
+  __device__ __forceinline__ void store(unsigned char *buf, int na) const {
+    const int tid = threadIdx.x, a = tid / TPA, sub = tid - a * TPA;
+    if (a >= na) return;
+    WT *win = reinterpret_cast<WT *>(buf);
 #pragma unroll
-  for (int r = 0; r < TR; ++r) {
-    const int row = ty0 + ty + 4 * r;
-    if (row >= w) continue;
-    float dy = static_cast<float>(row - hp) - c;
-    dy *= dy;
-#pragma unroll
-    for (int q = 0; q < TC; ++q) {
-      const int col = tx0 + tx + 16 * q;
-      if (col >= w) continue;
-      int ip2 = col - hp;
-      ip2 *= ip2;
-      const float d2 = dy + static_cast<float>(ip2);
-      const float fr = static_cast<float>(row), fc = static_cast<float>(col);
-      const bool masked = (fc <= acent) || (fc >= static_cast<float>(w) - acent - 1.0f) ||
-                          (fr <= up && d2 >= up * up) || (fr >= up && d2 >= dn * dn);
-      float v = acc[r][q];
-      if (recCof != 0.0f) v *= recCof;
-      v = v / static_cast<float>(N);
-      out[static_cast<size_t>(row) * w + col] = masked ? 0.0f : v;
+    for (int j = 0; j < EPT; ++j) {
+      const int o = sub + j * TPA;
+      if (o >= L) continue;
+      const uint32_t v0 = ((ok >> (2 * j)) & 1u) ? v[j][0] : 0u;
+      const uint32_t v1 = ((ok >> (2 * j + 1)) & 1u) ? v[j][1] : 0u;
+      win[a * L + o] = make_float2(half_to_float<ST>(v0), half_to_float<ST>(v1));
+    }
+    if (sub == 0) {
+      // the buffer's offset from the start of LDS goes into the base
+      // coordinate, so that a sample's LDS address is its word index times
+      // the word size
+      int4 *prm = reinterpret_cast<int4 *>(buf + WIN);
+      int4 p0 = prm0;
+      p0.x += static_cast<int>((buf - base) / sizeof(WT)) << 16;
+      prm[2 * a] = p0;
+      prm[2 * a + 1] = prm1;
     }
   }
-}
+  const unsigned char *base = nullptr;  // start of the LDS block
+};
 
-// ---- FP16/BF16 with pair words -------------------------------------------
-// LDS holds, for every window position o, the 32-bit word (v[o], v[o+1]).
-// A lane owns vertically adjacent pixel pairs (rows j, j+1): their ray
-// coordinates differ by s = sin(theta), so with floor sampling row j+1
-// reads v[k] or v[k+1] (k = floor(t_j)) for s >= 0, v[k-1] or v[k] for
-// s < 0 (the pair is then taken at k-1). One ds_read_b32 serves both pixels;
-// each pixel adds its half with a dot2 against a weight word that is 1.0 in
-// the half it takes. Coordinates are 16.16 fixed point formed from FP64
-// per-angle constants, as in bp_tile_fx.
-template <int ST> struct One16;
-template <> struct One16<ST_F16> { static constexpr unsigned LO = 0x3C00u, HI = 0x3C000000u; };
-template <> struct One16<ST_BF16> { static constexpr unsigned LO = 0x3F80u, HI = 0x3F800000u; };
-
-template <int ST>
-__device__ __forceinline__ float dot2w(unsigned a, unsigned b, float acc) {
-  using v2 = typename Pair16<ST>::v2;
-  if constexpr (ST == ST_F16)
-    return __builtin_amdgcn_fdot2(__builtin_bit_cast(v2, a), __builtin_bit_cast(v2, b), acc, false);
-  else
-    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(v2, a), __builtin_bit_cast(v2, b), acc, false);
-}
-
-template <int KIND, int AC>
-constexpr size_t lds_bytes_px2() {
-  return sizeof(uint32_t) * AC * KindGeo<KIND>::L + 2 * sizeof(int4) * AC + sizeof(int) * AC;
+template <class Stage, class Compute>
+__device__ __forceinline__ void run_passes_fx(const Item &it, unsigned char *lds, int AC, int ty0, int tx0,
+                                              bool wskip, Compute compute) {
+  Stage st;
+  st.base = lds;
+  const int N = it.N;
+  const int P = (N + AC - 1) / AC;
+  st.load_angle(it, 0, min(AC, N));
+  st.take_loaded();
+  st.fetch(it, 0, min(AC, N), ty0, tx0);
+  st.load_angle(it, min(1, P - 1) * AC, min(AC, N - min(1, P - 1) * AC));
+  st.store(lds, min(AC, N));
+  st.put_angle(lds, 1, min(AC, N - min(1, P - 1) * AC));
+  __syncthreads();
+  for (int p = 0; p < P; ++p) {
+    const int a0 = p * AC, na = min(AC, N - a0);
+    const int p1 = min(p + 1, P - 1), p2 = min(p + 2, P - 1);
+    const int na1 = min(AC, N - p1 * AC), na2 = min(AC, N - p2 * AC);
+    st.get_angle(lds, (p + 1) & 1, na1);
+    st.fetch(it, p1 * AC, na1, ty0, tx0);
+    st.load_angle(it, p2 * AC, na2);
+    if (!wskip) compute(lds + (p & 1) * Stage::HALFBUF, na);
+    st.store(lds + ((p + 1) & 1) * Stage::HALFBUF, na1);
+    st.put_angle(lds, p & 1, na2);
+    __syncthreads();
+  }
 }
 
 template <int KIND, int AC, int ST>
 __device__ __forceinline__ void bp_tile_px2(const Item &it, int tile, unsigned char *lds) {
   using G = KindGeo<KIND>;
-  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT, L = G::L;
+  using S = StagePair<KIND, AC, ST>;
+  using W = typename S::WT;
+  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT;
   constexpr int RP = TR / 2;
-  constexpr unsigned LO = One16<ST>::LO, HI = One16<ST>::HI;
-  uint32_t *win = reinterpret_cast<uint32_t *>(lds);
-  int4 *prm = reinterpret_cast<int4 *>(lds + sizeof(uint32_t) * AC * L);
-  int *los = reinterpret_cast<int *>(lds + sizeof(uint32_t) * AC * L + 2 * sizeof(int4) * AC);
-
-  const int w = it.w, N = it.N;
-  const int hp = w / 2;
   const float c = it.center;
   const int ty0 = (tile / it.tiles_x) * RT;
   const int tx0 = (tile % it.tiles_x) * CT;
   const int tid = threadIdx.x;
   // wave v owns the tile rows [v 4TR, (v+1) 4TR); lane (ty4, tx) the pixels
-  // (v 4TR + 2 ty4 + 8 rp + h, tx + 16 q), h = 0, 1; the coordinate of
-  // pixel (y, x) of the tile is bt + S y + C x in 16.16 fixed point
+  // (v 4TR + 2 ty4 + 8 rp + h, tx + 16 q), h = 0, 1
   const int tx = tid & 15, ty = (tid >> 6) * (4 * TR) + 2 * ((tid >> 4) & 3);
-  float *__restrict__ out = it.out;
-  {
-    const float acent = fabsf(c);
-    const float ax = static_cast<float>(hp), ay = static_cast<float>(hp) + c;
-    const float dx = fmaxf(fmaxf(static_cast<float>(tx0) - ax, ax - static_cast<float>(tx0 + CT - 1)), 0.0f);
-    const float dy = fmaxf(fmaxf(static_cast<float>(ty0) - ay, ay - static_cast<float>(ty0 + RT - 1)), 0.0f);
-    const float rmax = static_cast<float>(hp) + acent + 2.0f;
-    const bool outside = (dx * dx + dy * dy > rmax * rmax) ||
-                         (static_cast<float>(tx0 + CT - 1) <= acent) ||
-                         (static_cast<float>(tx0) >= static_cast<float>(w) - acent - 1.0f);
-    if (outside) {
-#pragma unroll
-      for (int rp = 0; rp < RP; ++rp)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int row = ty0 + ty + 8 * rp + h;
-#pragma unroll
-          for (int q = 0; q < TC; ++q) {
-            const int col = tx0 + tx + 16 * q;
-            if (row < w && col < w) out[static_cast<size_t>(row) * w + col] = 0.0f;
-          }
-        }
-      return;
-    }
+  auto rowf = [&](int r) { return ty0 + ty + 8 * (r >> 1) + (r & 1); };
+  auto colf = [&](int q) { return tx0 + tx + 16 * q; };
+  if (tile_outside<KIND>(it, ty0, tx0)) {
+    zero_tile<TR, TC>(it, rowf, colf);
+    return;
   }
-  float acc[RP][2][TC];
+  float acc[TR][TC];  // row 2 rp + h
 #pragma unroll
-  for (int rp = 0; rp < RP; ++rp)
+  for (int r = 0; r < TR; ++r)
 #pragma unroll
-    for (int q = 0; q < TC; ++q) acc[rp][0][q] = acc[rp][1][q] = 0.0f;
-  const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
-  const long long total = static_cast<long long>(N) * w;
+    for (int q = 0; q < TC; ++q) acc[r][q] = 0.0f;
   const bool wskip = wave_outside(it, c, ty0 + (tid >> 6) * (4 * TR), 4 * TR, tx0, CT);
 
-  for (int a0 = 0; a0 < N; a0 += AC) {
-    const int na = min(AC, N - a0);
-    __syncthreads();
-    if (tid < na) {
-      const int ang = a0 + tid;
-      const double2 sd = it.sincos64[ang];
-      const double cd = static_cast<double>(c);
-      const double T0 = cd + hp * ((1.0 - sd.x) - sd.y) + sd.x * (ty0 - cd) + sd.y * tx0;
-      const double tc = T0 + sd.x * (0.5 * (RT - 1)) + sd.y * (0.5 * (CT - 1));
-      const int lo = static_cast<int>(floor(tc)) - G::HALF - 2;
-      los[tid] = lo;
-      const bool neg = sd.x < 0.0;
-      const int bt = static_cast<int>(llrint((T0 - lo + static_cast<double>(tid) * L) * 65536.0)) - (neg ? 65536 : 0);
-      const int S = static_cast<int>(llrint(sd.x * 65536.0)), C = static_cast<int>(llrint(sd.y * 65536.0));
-      const int thr = neg ? -S : 65536 - S;
-      prm[2 * tid] = make_int4(bt, S, C, thr | (neg ? (1 << 30) : 0));
-      prm[2 * tid + 1] = make_int4(8 * S, 16 * C, 0, 0);
-    }
-    __syncthreads();
-    for (int e = tid; e < na * L; e += 256) {
-      const int a = e / L;
-      const int o = e - a * L;
-      const long long flat = static_cast<long long>(a0 + a) * w + los[a] + o;
-      const uint32_t v0 = (flat >= 0 && flat < total) ? filt[flat] : 0u;
-      const uint32_t v1 = (flat + 1 >= 0 && flat + 1 < total) ? filt[flat + 1] : 0u;
-      win[e] = v0 | (v1 << 16);
-    }
-    __syncthreads();
-    if (wskip) continue;
-    const unsigned char *wb = reinterpret_cast<const unsigned char *>(win);
-    for (int a = 0; a < na; ++a) {
-      const int4 p = prm[2 * a], pd = prm[2 * a + 1];
+  run_passes_fx<S>(it, lds, AC, ty0, tx0, wskip, [&](const unsigned char *buf, int na) {
+    const int4 *prm = reinterpret_cast<const int4 *>(buf + S::WIN);
+    const unsigned char *lds0 = lds;  // sample addresses are relative to it
+    // one angle; HF (sin < 0, pair taken at k-1): row j takes the upper
+    // sample. Row j+1 takes the upper sample when its fraction test holds.
+    auto angle = [&](const int4 p, const int4 pd, auto hi_first) {
+      constexpr bool HF = decltype(hi_first)::value;
       const unsigned thr = static_cast<unsigned>(p.w & 0x3FFFFFFF);
-      const unsigned wj = (p.w >> 30) ? HI : LO;
       int A[RP], C[TC];
       A[0] = __mul24(p.y, ty) + p.x;
 #pragma unroll
@@ -602,49 +591,35 @@ __device__ __forceinline__ void bp_tile_px2(const Item &it, int tile, unsigned c
       for (int q = 1; q < TC; ++q) C[q] = C[q - 1] + pd.y;
 #pragma unroll
       for (int rp = 0; rp < RP; ++rp) {
-        unsigned u[TC], P[TC];
+        unsigned u[TC];
+        W P[TC];
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
           u[q] = static_cast<unsigned>(A[rp] + C[q]);
-          P[q] = *reinterpret_cast<const uint32_t *>(wb + ((u[q] >> 16) << 2));
+          P[q] = *reinterpret_cast<const W *>(lds0 + ((u[q] >> 16) * sizeof(W)));
         }
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
-          const unsigned wj1 = (u[q] & 0xFFFFu) >= thr ? HI : LO;
-          acc[rp][0][q] = dot2w<ST>(P[q], wj, acc[rp][0][q]);
-          acc[rp][1][q] = dot2w<ST>(P[q], wj1, acc[rp][1][q]);
+          const bool up = (u[q] & 0xFFFFu) >= thr;
+          acc[2 * rp][q] += HF ? P[q].y : P[q].x;
+          acc[2 * rp + 1][q] += up ? P[q].y : P[q].x;
         }
       }
-    }
-  }
-
-  const float acent = fabsf(c);
-  const float up = static_cast<float>(hp) + c, dn = static_cast<float>(hp) - c;
-  const float recCof = it.recCof;
-#pragma unroll
-  for (int rp = 0; rp < RP; ++rp)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int row = ty0 + ty + 8 * rp + h;
-      if (row >= w) continue;
-      float dy = static_cast<float>(row - hp) - c;
-      dy *= dy;
-#pragma unroll
-      for (int q = 0; q < TC; ++q) {
-        const int col = tx0 + tx + 16 * q;
-        if (col >= w) continue;
-        int ip2 = col - hp;
-        ip2 *= ip2;
-        const float d2 = dy + static_cast<float>(ip2);
-        const float fr = static_cast<float>(row), fc = static_cast<float>(col);
-        const bool masked = (fc <= acent) || (fc >= static_cast<float>(w) - acent - 1.0f) ||
-                            (fr <= up && d2 >= up * up) || (fr >= up && d2 >= dn * dn);
-        float v = acc[rp][h][q];
-        if (recCof != 0.0f) v *= recCof;
-        v = v / static_cast<float>(N);
-        out[static_cast<size_t>(row) * w + col] = masked ? 0.0f : v;
+    };
+    if (it.neg_sin) {
+      for (int a = 0; a < na; ++a) {
+        const int4 p = prm[2 * a], pd = prm[2 * a + 1];
+        if (p.w >> 30)
+          angle(p, pd, std::true_type{});
+        else
+          angle(p, pd, std::false_type{});
       }
+    } else {
+#pragma unroll 2
+      for (int a = 0; a < na; ++a) angle(prm[2 * a], prm[2 * a + 1], std::false_type{});
     }
+  });
+  store_tile<TR, TC>(it, acc, rowf, colf);
 }
 
 // Grid-mapped launch: one workgroup per tile; blocks are remapped so that the
@@ -656,9 +631,20 @@ constexpr bool use_fx() {
 }
 
 template <int KIND, int AC, int MODE, int ST>
-constexpr size_t grid_lds() {
-  if constexpr (use_fx<KIND, AC, MODE, ST>()) return lds_bytes_px2<KIND, AC>();
-  return lds_bytes<KIND, AC, MODE>();
+constexpr size_t stage_bytes() {
+  if constexpr (use_fx<KIND, AC, MODE, ST>())
+    return StagePair<KIND, AC, ST>::BYTES;
+  else
+    return StageFloat<KIND, AC, MODE, ST>::BYTES;
+}
+
+// Angles per pass actually used: the double-buffered windows of a variant
+// must fit the 64 KiB a kernel may allocate statically; the 8-byte windows
+// of the larger tiles take 16 angles per pass instead of 32.
+constexpr size_t kMaxStaticLds = 65536;
+template <int KIND, int AC, int MODE, int ST>
+constexpr int eff_ac() {
+  return stage_bytes<KIND, AC, MODE, ST>() <= kMaxStaticLds ? AC : AC / 2;
 }
 
 // Waves per SIMD each variant's register budget is sized for.
@@ -669,26 +655,27 @@ constexpr int grid_waves() {
 
 template <int KIND, int AC, int MODE, int ST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grid_waves<KIND, AC, MODE, ST>(), grid_waves<KIND, AC, MODE, ST>()))) void bp_grid_kernel(const Item *__restrict__ items, int n_items) {
-  __shared__ __align__(16) unsigned char lds[grid_lds<KIND, AC, MODE, ST>()];
+  constexpr int A = eff_ac<KIND, AC, MODE, ST>();
+  __shared__ __align__(16) unsigned char lds[stage_bytes<KIND, A, MODE, ST>()];
   int b = blockIdx.x;
   const int q = gridDim.x >> 3;
   if (b < (q << 3)) b = (b & 7) * q + (b >> 3);
   const Item &it = items[find_item_by_tile(items, n_items, b)];
-  if constexpr (use_fx<KIND, AC, MODE, ST>())
-    bp_tile_px2<KIND, AC, ST>(it, b - it.tile_base, lds);
+  if constexpr (use_fx<KIND, A, MODE, ST>())
+    bp_tile_px2<KIND, A, ST>(it, b - it.tile_base, lds);
   else
-    bp_tile<KIND, AC, MODE, ST>(it, b - it.tile_base, lds);
+    bp_tile<KIND, A, MODE, ST>(it, b - it.tile_base, lds);
 }
 
 template <int A, int B>
 struct Max { static constexpr size_t v = A > B ? A : B; };
 
-constexpr int kPersistAC = 32;
+constexpr int kPersistAC = 16;
 
-template <int MODE>
+template <int MODE, int ST>
 constexpr size_t persist_lds() {
-  return Max<Max<lds_bytes<TK_64x64, kPersistAC, MODE>(), lds_bytes<TK_128x64, kPersistAC, MODE>()>::v,
-             lds_bytes<TK_128x128, kPersistAC, MODE>()>::v;
+  return Max<Max<StageFloat<TK_64x64, kPersistAC, MODE, ST>::BYTES, StageFloat<TK_128x64, kPersistAC, MODE, ST>::BYTES>::v,
+             StageFloat<TK_128x128, kPersistAC, MODE, ST>::BYTES>::v;
 }
 
 __global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
@@ -699,7 +686,7 @@ __global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
 template <int MODE, int ST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void bp_persistent_kernel(const Item *__restrict__ items, int n_items,
                                                             int total_tiles, unsigned *counter) {
-  __shared__ __align__(16) unsigned char lds[persist_lds<MODE>()];
+  __shared__ __align__(16) unsigned char lds[persist_lds<MODE, ST>()];
   __shared__ int s_job;
   for (;;) {
     __syncthreads();

@@ -129,6 +129,7 @@ Item base_item(const ctas_fbp *h, int kind) {
   Item it{};
   it.sincos = h->d_sincos;
   it.sincos64 = use_fp64_angles(h->flags) ? h->d_sincos64 : nullptr;
+  it.neg_sin = h->neg_sin;
   it.W = h->d_W;
   it.tw = h->d_tw;
   it.recCof = h->recCof;
@@ -281,8 +282,16 @@ struct GeomRes {
   float *d_W = nullptr;
   float2 *d_sincos = nullptr;
   double2 *d_sincos64 = nullptr;
+  int neg_sin = 0;
   float2 *d_tw = nullptr;
 };
+
+// 1 when some {sin, cos} entry has a negative sine.
+int any_negative_sine(const std::vector<double> &sc64) {
+  for (size_t i = 0; i < sc64.size(); i += 2)
+    if (sc64[i] < 0.0) return 1;
+  return 0;
+}
 std::mutex g_geo_mu;
 std::map<std::tuple<int, int, int, int, float, float, int>, GeomRes> g_geo;  // (dev, w, N, filter, param, anarc, contrast)
 
@@ -315,6 +324,7 @@ int geometry_resources(int w, int N, int filter, float param, float anarc, int c
   GeomRes r;
   std::vector<double> sc64(2 * static_cast<size_t>(N));
   angle_table64(N, anarc, sc64.data());
+  r.neg_sin = any_negative_sine(sc64);
   if (hipMalloc(&r.d_W, sizeof(float) * W.size()) != hipSuccess ||
       hipMalloc(&r.d_sincos, sizeof(float2) * N) != hipSuccess ||
       hipMalloc(&r.d_sincos64, sizeof(double2) * N) != hipSuccess)
@@ -467,6 +477,7 @@ int ctas_fbp_create(ctas_fbp **ph, int w, int n_angles, int contrast, int filter
   {
     std::vector<double> sc64(2 * static_cast<size_t>(n_angles));
     angle_table64(n_angles, anarc, sc64.data());
+    h->neg_sin = any_negative_sine(sc64);
     CTAS_HIP_TRY(hipMemcpy(h->d_sincos64, sc64.data(), sizeof(double) * sc64.size(), hipMemcpyHostToDevice));
   }
   CTAS_HIP_TRY(hipMemcpy(h->d_W, h->W.data(), sizeof(float) * h->Z, hipMemcpyHostToDevice));
@@ -536,6 +547,7 @@ int ctas_fbp_set_angles(ctas_fbp *h, const float *sincos) {
   CTAS_HIP_TRY(hipMemcpy(h->d_sincos, h->sincos.data(), sizeof(float) * h->sincos.size(), hipMemcpyHostToDevice));
   // a caller-supplied table replaces the FP64 one too (promoted exactly)
   std::vector<double> sc64(h->sincos.begin(), h->sincos.end());
+  h->neg_sin = any_negative_sine(sc64);
   CTAS_HIP_TRY(hipMemcpy(h->d_sincos64, sc64.data(), sizeof(double) * sc64.size(), hipMemcpyHostToDevice));
   return CTAS_OK;
 }
@@ -789,6 +801,7 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
     it.out = bi.d_out;
     it.sincos = g.d_sincos;
     it.sincos64 = use_fp64_angles(flags) ? g.d_sincos64 : nullptr;
+    it.neg_sin = g.neg_sin;
     it.W = g.d_W;
     it.tw = g.d_tw;
     it.center = bi.center;

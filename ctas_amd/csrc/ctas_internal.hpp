@@ -65,7 +65,7 @@ struct Item {
   int tile_base;          // prefix sum of tiles over the items
   int pair_base;          // prefix sum of row pairs over the items
   int kind;               // tile kind used for this item (persistent kernel)
-  int pad_;
+  int neg_sin;             // some angle has sin < 0 (arcs past 180 degrees)
 };
 
 constexpr int kMinMaxBlocks = 64;
@@ -147,6 +147,7 @@ struct ctas_fbp {
   float *d_W = nullptr;
   float2 *d_sincos = nullptr;
   double2 *d_sincos64 = nullptr;
+  int neg_sin = 0;             // some entry of the FP64 table has sin < 0
   float2 *d_tw = nullptr;
   hipStream_t stream = nullptr;
 
