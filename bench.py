@@ -26,6 +26,19 @@ LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM bytes per launch of the backprojection of the default workload (256 x
+# 2048 px x 1800 angles, 64x64 tiles, 32 angles per pass), from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read
+# for a 1 GiB copy (profiles/fbp_fp32_ctas_2048x1800x256.md). PMC counters
+# cannot be read from inside this process, so the figure is quoted for that
+# configuration only.
+PMC_TRAFFIC = {(256, 2048, 1800, 1, 32): 79.2e9}
+
+
+def measured_traffic(S, w, N, kind, ac):
+    return PMC_TRAFFIC.get((S, w, N, kind, ac))
+
+
 def in_mask_fraction(w: int, c: float) -> float:
     import numpy as np
     hp = w // 2
@@ -245,7 +258,8 @@ def main() -> int:
                    "slices": S, "width": w, "angles": N, "filter": "HANN", "tile_kind": kind,
                    "angles_per_pass": ac, "in_mask_fraction": round(frac_in, 4)},
         "roofline": {"bound": "lds", "achieved": round(achieved, 1), "peak": LDS_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / LDS_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / LDS_PEAK_GBS, 4),
+                     "traffic": measured_traffic(S, w, N, kind, ac),
                      "kernel": "bp_grid_kernel", "kernel_ms": round(bp_ms, 3),
                      "bytes_per_update": 4, "note": "LDS sample bytes per full-grid voxel-update"},
         "stages_ms": {"prepare": round(prep_ms, 3), "backproject": round(bp_ms, 3)},
