@@ -124,7 +124,7 @@ __device__ __forceinline__ void store_tile(const Item &it, const float (&acc)[NR
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int col = colf(q);
-      if (col >= w) continue;
+      if (col < 0 || col >= w) continue;
       int ip2 = col - hp;
       ip2 *= ip2;
       const float d2 = dy + static_cast<float>(ip2);
@@ -147,7 +147,7 @@ __device__ __forceinline__ void zero_tile(const Item &it, RowF rowf, ColF colf) 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int row = rowf(r), col = colf(q);
-      if (row < w && col < w) it.out[static_cast<size_t>(row) * w + col] = 0.0f;
+      if (row < w && col >= 0 && col < w) it.out[static_cast<size_t>(row) * w + col] = 0.0f;
     }
 }
 
@@ -418,7 +418,7 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
   store_tile<TR, TC>(it, acc, rowf, colf);
 }
 
-// ---- FP16/BF16 storage, CTAS (floor) sampling ------------------------------
+// ---- FP16/BF16 storage, CTAS sampling, row pairs -----------------------------
 // LDS holds, for every window position o, the pair (v[o], v[o+1]) as two
 // floats (converted from the stored 16-bit samples when staged). A lane owns
 // vertically adjacent pixel pairs (rows j, j+1): their ray coordinates
@@ -431,6 +431,14 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
 // fixed point, bt, S and C formed per angle from FP64 constants (within
 // ~1e-4 px of the FP64 coordinate, finer than the float arithmetic of the
 // parity path).
+// The fixed-point paths index windows by floor(t); the reference reads
+// sample (long)t, truncated (lct.cpp:710-713), which for t < 0 is
+// floor(t) + 1. Window position k therefore holds the sample at trunc_pos(k).
+// Only a coordinate that is exactly a negative integer (a sample outside the
+// detector, at an angle whose constants are exact) reads one bin higher than
+// the truncation would.
+__device__ __forceinline__ int trunc_pos(int k) { return k < 0 ? k + 1 : k; }
+
 template <int ST>
 __device__ __forceinline__ float half_to_float(uint32_t h) {
   if constexpr (ST == ST_F16)
@@ -480,15 +488,16 @@ struct StagePair {
     prm0 = make_int4(bt, S, C, thr | (neg ? (1 << 30) : 0));
     prm1 = make_int4(8 * S, 16 * C, 0, 0);
     const long long total = static_cast<long long>(it.N) * w;
-    const long long rb = static_cast<long long>(a0 + a) * w + lo;
+    const long long rb = static_cast<long long>(a0 + a) * w;
     const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
     ok = 0;
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
-      const long long flat = rb + sub + j * TPA;
-      const bool in0 = flat >= 0 && flat < total, in1 = flat + 1 >= 0 && flat + 1 < total;
-      v[j][0] = ldg(filt, in0 ? flat : 0);
-      v[j][1] = ldg(filt, in1 ? flat + 1 : 0);
+      const int k = lo + sub + j * TPA;
+      const long long f0 = rb + trunc_pos(k), f1 = rb + trunc_pos(k + 1);
+      const bool in0 = f0 >= 0 && f0 < total, in1 = f1 >= 0 && f1 < total;
+      v[j][0] = ldg(filt, in0 ? f0 : 0);
+      v[j][1] = ldg(filt, in1 ? f1 : 0);
       ok |= (static_cast<uint64_t>(in0) | (static_cast<uint64_t>(in1) << 1)) << (2 * j);
     }
   }
@@ -519,12 +528,12 @@ struct StagePair {
   const unsigned char *base = nullptr;  // start of the LDS block
 };
 
+// Same pipeline for the fixed-point stages; N steps (angles or mirror slots).
 template <class Stage, class Compute>
-__device__ __forceinline__ void run_passes_fx(const Item &it, unsigned char *lds, int AC, int ty0, int tx0,
+__device__ __forceinline__ void run_passes_fx(const Item &it, int N, unsigned char *lds, int AC, int ty0, int tx0,
                                               bool wskip, Compute compute) {
   Stage st;
   st.base = lds;
-  const int N = it.N;
   const int P = (N + AC - 1) / AC;
   st.load_angle(it, 0, min(AC, N));
   st.take_loaded();
@@ -574,7 +583,7 @@ __device__ __forceinline__ void bp_tile_px2(const Item &it, int tile, unsigned c
     for (int q = 0; q < TC; ++q) acc[r][q] = 0.0f;
   const bool wskip = wave_outside(it, c, ty0 + (tid >> 6) * (4 * TR), 4 * TR, tx0, CT);
 
-  run_passes_fx<S>(it, lds, AC, ty0, tx0, wskip, [&](const unsigned char *buf, int na) {
+  run_passes_fx<S>(it, it.N, lds, AC, ty0, tx0, wskip, [&](const unsigned char *buf, int na) {
     const int4 *prm = reinterpret_cast<const int4 *>(buf + S::WIN);
     const unsigned char *lds0 = lds;  // sample addresses are relative to it
     // one angle; HF (sin < 0, pair taken at k-1): row j takes the upper
@@ -622,6 +631,289 @@ __device__ __forceinline__ void bp_tile_px2(const Item &it, int tile, unsigned c
   store_tile<TR, TC>(it, acc, rowf, colf);
 }
 
+// ---- FP16/BF16 storage, CTAS sampling, mirrored angle pairs ----------------
+// For angles a and b = pi - a (equal sines, opposite cosines), a pixel p and
+// its mirror p' about the axis column hp = w/2 (column 2hp - x) have swapped
+// ray coordinates: t_a(p) = t_b(p') and t_a(p') = t_b(p), exactly. So one
+// coordinate and one 4-byte LDS read of the word (v_a[k], v_b[k]) give two
+// updates: v_a[k] to p and v_b[k] to p'. A workgroup owns a tile of columns
+// up to hp and the mirrored tile; per slot it stages two windows of such
+// words, A around the tile's coordinates at angle a and B around the mirrored
+// tile's, and every pixel pair reads one word of each:
+//   p  += lo(A[t_a(p)])  + hi(B[t_a(p')])
+//   p' += hi(A[t_a(p)])  + lo(B[t_a(p')])
+// Angles without a partner (0 and 90 degrees of a 180-degree arc) are slots
+// with zero high halves. The column hp is its own mirror: the first sum
+// alone is its value.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// acc + the low (HI = 0) or high (HI = 1) sample of a word, converted and
+// added by one v_dot2 with weights (1, 0) or (0, 1): the products are exact
+// and the zero-weighted sample adds nothing, so the result is the rounded
+// float sum acc + sample.
+template <int ST, int HI>
+__device__ __forceinline__ float add_sample(float acc, uint32_t w) {
+  if constexpr (ST == ST_F16) {
+    const half2v wt = HI ? half2v{static_cast<_Float16>(0.0f), static_cast<_Float16>(1.0f)}
+                         : half2v{static_cast<_Float16>(1.0f), static_cast<_Float16>(0.0f)};
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, w), wt, acc, false);
+  } else {
+    // bf16 (0, 1) / (1, 0), held in a scalar register: as an inline constant
+    // the compiler encodes (1, 0) as "1.0", which the bf16 form does not read
+    // as bf16 (1, 0)
+    uint32_t wt = HI ? 0x3F800000u : 0x00003F80u;
+    asm("" : "+s"(wt));
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2v, w), __builtin_bit_cast(bf16x2v, wt), acc,
+                                           false);
+  }
+}
+
+// Lanes own vertically adjacent pixel pairs (rows j, j+1, as in the row-pair
+// path above): with floor sampling row j+1 reads word k or k+1 (k = floor
+// t_j) when s >= 0, k-1 or k when s < 0, so one 8-byte read of two words
+// serves both rows of a pixel, the second row selected by the fraction of
+// t_j against thr = 1 - s (s >= 0) or -s (s < 0). Per 8 updates (pixel and
+// mirror, two rows, two angles): two 8-byte LDS reads, two compares and
+// selects, eight dot2 accumulations.
+//
+// Coordinates: per slot, the staging threads tabulate in FP64, each term
+// rounded once to 16.16 fixed point, the row terms RA[y] = TA - loA + s y
+// (window A, word offsets and buffer included; one word lower when s < 0,
+// where the pair is taken at k-1) and RB[y] likewise for window B, for the
+// first row y of every pair, and the column terms CX[x] = c x. Pixel (y, x)
+// then reads the pair at word RA[y] + CX[x] of window A, its mirror the pair
+// at RB[y] - CX[x] of window B. A lane reads its row terms {RA, RB} and its
+// column terms as 16-byte words; the slot header holds thr and the sign.
+// Coordinates: per slot, the staging threads tabulate in FP64, each term
+// rounded once to 16.16 fixed point, the row terms RA[y] = TA - loA + s y
+// (window A, word offsets and buffer included), RB[y] likewise for window
+// B, and the column terms CX[x] = c x, for the RT rows and CT columns of
+// the tile. Pixel (y, x) then reads word RA[y] + CX[x] of window A and its
+// mirror word RB[y] - CX[x] of window B, within 2^-16 px of the FP64
+// coordinate. The tables are laid out in lattice order, so that a lane
+// reads its TR row terms and TC column terms as 16-byte words. Per 4
+// updates (pixel and mirror, two angles): two coordinates, two 4-byte LDS
+// reads, four dot2 accumulations.
+template <int KIND, int AC, int ST>
+struct StageMirror {
+  using G = KindGeo<KIND>;
+  static constexpr int L = G::L, L2 = 2 * L, TPA = 256 / AC, EPT = (L2 + TPA - 1) / TPA;
+  static constexpr int RT = G::RT, CT = G::CT, TR = G::TR, TC = G::TC;
+  static constexpr int TAB = 2 * RT + CT;  // ints per slot: RA, RB, CX
+  static constexpr int TPT = (TAB + TPA - 1) / TPA;
+  static constexpr size_t WIN = sizeof(uint32_t) * AC * L2;
+  static constexpr size_t HALFBUF = WIN + sizeof(int) * AC * TAB;
+  static constexpr size_t ANGS = 2 * HALFBUF;
+  static constexpr size_t BYTES = ANGS + 2 * sizeof(MirrorSlot) * AC;
+  static_assert(2 * EPT <= 64, "one validity bit per sample");
+  static_assert(TR % 4 == 0 && TC % 4 == 0, "16-byte table reads");
+  uint16_t v[EPT][2];
+  uint64_t ok;  // bit (2 j + i): sample i of word j lies inside the slice
+  int loA, loB;
+  MirrorSlot sl, sl_next;  // slots of the pass being fetched and of the one after
+  const unsigned char *base = nullptr;  // start of the LDS block
+
+  // lattice position of row y / column x in the tables
+  static __device__ __forceinline__ int row_index(int y) {
+    return ((y / (4 * TR)) * 4 + (y & 3)) * TR + (y % (4 * TR)) / 4;
+  }
+  static __device__ __forceinline__ int col_index(int x) { return (x & 15) * TC + x / 16; }
+
+  __device__ __forceinline__ void load_angle(const Item &it, int a0, int na) {
+    const MirrorSlot *p = it.slots + a0 + min(static_cast<int>(threadIdx.x) / TPA, na - 1);
+    const double *d = reinterpret_cast<const double *>(p);
+    const int *r = reinterpret_cast<const int *>(p) + 4;
+    sl_next.s = ldg(d, 0);
+    sl_next.c = ldg(d, 1);
+    sl_next.row_a = ldg(r, 0);
+    sl_next.row_b = ldg(r, 1);
+  }
+  __device__ __forceinline__ void put_angle(unsigned char *lds, int slot, int na) const {
+    const int tid = threadIdx.x, a = tid / TPA;
+    if (a < na && tid - a * TPA == 0) reinterpret_cast<MirrorSlot *>(lds + ANGS)[slot * AC + a] = sl_next;
+  }
+  __device__ __forceinline__ void get_angle(const unsigned char *lds, int slot, int na) {
+    sl = reinterpret_cast<const MirrorSlot *>(lds + ANGS)[slot * AC + min(static_cast<int>(threadIdx.x) / TPA, na - 1)];
+  }
+  __device__ __forceinline__ void take_loaded() { sl = sl_next; }
+
+  // ray coordinates of pixel (ty0, tx0) and of its mirror at angle row_a
+  __device__ __forceinline__ void corner(const Item &it, int ty0, int tx0, double &TA, double &TB) const {
+    const int hp = it.w / 2;
+    const double cd = static_cast<double>(it.center), s = sl.s, co = sl.c;
+    const double rowterm = cd + hp * ((1.0 - s) - co) + s * (ty0 - cd);
+    TA = rowterm + co * tx0;
+    TB = rowterm + co * (2 * hp - tx0);
+  }
+
+  // tx0: first column of the tile; the mirrored tile's columns are 2hp - x.
+  __device__ __forceinline__ void fetch(const Item &it, int a0, int na, int ty0, int tx0) {
+    (void)a0;
+    (void)na;
+    const int tid = threadIdx.x, sub = tid - (tid / TPA) * TPA;
+    const int w = it.w;
+    double TA, TB;
+    corner(it, ty0, tx0, TA, TB);
+    const double hy = sl.s * (0.5 * (RT - 1)), hx = sl.c * (0.5 * (CT - 1));
+    loA = static_cast<int>(floor(TA + hy + hx)) - G::HALF - 2;
+    loB = static_cast<int>(floor(TB + hy - hx)) - G::HALF - 2;
+    const long long total = static_cast<long long>(it.N) * w;
+    const long long ra = static_cast<long long>(sl.row_a) * w;
+    const bool has_b = sl.row_b >= 0;
+    const long long rb = static_cast<long long>(has_b ? sl.row_b : 0) * w;
+    const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
+    ok = 0;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int e = sub + j * TPA;  // word e of the slot: [0, L) window A, [L, 2L) window B
+      const long long k = trunc_pos(e < L ? loA + e : loB + (e - L));
+      const long long fa = ra + k, fb = rb + k;
+      const bool ina = fa >= 0 && fa < total, inb = has_b && fb >= 0 && fb < total;
+      v[j][0] = ldg(filt, ina ? fa : 0);
+      v[j][1] = ldg(filt, inb ? fb : 0);
+      ok |= (static_cast<uint64_t>(ina) | (static_cast<uint64_t>(inb) << 1)) << (2 * j);
+    }
+  }
+
+  __device__ __forceinline__ void store(const Item &it, unsigned char *buf, int na, int ty0, int tx0) const {
+    const int tid = threadIdx.x, a = tid / TPA, sub = tid - a * TPA;
+    if (a >= na) return;
+    uint32_t *win = reinterpret_cast<uint32_t *>(buf);
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int e = sub + j * TPA;
+      if (e >= L2) continue;
+      const uint32_t lo = ((ok >> (2 * j)) & 1u) ? v[j][0] : 0u;
+      const uint32_t hi = ((ok >> (2 * j + 1)) & 1u) ? v[j][1] : 0u;
+      win[a * L2 + e] = lo | (hi << 16);
+    }
+    // word addresses are taken from the start of the LDS block
+    const double off = static_cast<double>((buf - base) / sizeof(uint32_t) + a * L2);
+    double TA, TB;
+    corner(it, ty0, tx0, TA, TB);
+    const double ba = TA - loA + off, bb = TB - loB + off + L;
+    int *tab = reinterpret_cast<int *>(buf + WIN) + a * TAB;
+#pragma unroll
+    for (int j = 0; j < TPT; ++j) {
+      const int e = sub + j * TPA;
+      if (e < RT)
+        tab[row_index(e)] = static_cast<int>(llrint((ba + sl.s * e) * 65536.0));
+      else if (e < 2 * RT)
+        tab[RT + row_index(e - RT)] = static_cast<int>(llrint((bb + sl.s * (e - RT)) * 65536.0));
+      else if (e < TAB)
+        tab[2 * RT + col_index(e - 2 * RT)] = static_cast<int>(llrint(sl.c * (e - 2 * RT) * 65536.0));
+    }
+  }
+};
+
+// Pipeline of the mirrored passes (as run_passes_fx; the stores also form
+// the coordinate tables, which need the tile's corner).
+template <class Stage, class Compute>
+__device__ __forceinline__ void run_passes_mirror(const Item &it, unsigned char *lds, int AC, int ty0, int tx0,
+                                                  bool wskip, Compute compute) {
+  Stage st;
+  st.base = lds;
+  const int N = it.n_slots;
+  const int P = (N + AC - 1) / AC;
+  st.load_angle(it, 0, min(AC, N));
+  st.take_loaded();
+  st.fetch(it, 0, min(AC, N), ty0, tx0);
+  st.load_angle(it, min(1, P - 1) * AC, min(AC, N - min(1, P - 1) * AC));
+  st.store(it, lds, min(AC, N), ty0, tx0);
+  st.put_angle(lds, 1, min(AC, N - min(1, P - 1) * AC));
+  __syncthreads();
+  for (int p = 0; p < P; ++p) {
+    const int a0 = p * AC, na = min(AC, N - a0);
+    const int p1 = min(p + 1, P - 1), p2 = min(p + 2, P - 1);
+    const int na1 = min(AC, N - p1 * AC), na2 = min(AC, N - p2 * AC);
+    st.get_angle(lds, (p + 1) & 1, na1);
+    st.fetch(it, p1 * AC, na1, ty0, tx0);
+    st.load_angle(it, p2 * AC, na2);
+    if (!wskip) compute(lds + (p & 1) * Stage::HALFBUF, na);
+    st.store(it, lds + ((p + 1) & 1) * Stage::HALFBUF, na1, ty0, tx0);
+    st.put_angle(lds, p & 1, na2);
+    __syncthreads();
+  }
+}
+
+template <int KIND, int AC, int ST>
+__device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigned char *lds) {
+  using G = KindGeo<KIND>;
+  using S = StageMirror<KIND, AC, ST>;
+  constexpr int TR = G::TR, TC = G::TC, RT = G::RT, CT = G::CT;
+  const float c = it.center;
+  const int w = it.w, hp = w / 2;
+  const int ty0 = (tile / it.tiles_x) * RT;
+  // tile column 0 ends at column hp; tiles to the left may start below 0
+  const int tx0 = hp + 1 - CT * (tile % it.tiles_x + 1);
+  const int mx0 = 2 * hp - (tx0 + CT - 1);  // first column of the mirrored tile
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = (tid >> 6) * (4 * TR) + ((tid >> 4) & 3);
+  auto rowf = [&](int r) { return ty0 + ty + 4 * r; };
+  auto colf = [&](int q) { return tx0 + tx + 16 * q; };
+  // column hp is stored once, from the first sum
+  auto colm = [&](int q) {
+    const int col = tx0 + tx + 16 * q;
+    return col == hp ? w : 2 * hp - col;
+  };
+  if (tile_outside<KIND>(it, ty0, tx0) && tile_outside<KIND>(it, ty0, mx0)) {
+    zero_tile<TR, TC>(it, rowf, colf);
+    zero_tile<TR, TC>(it, rowf, colm);
+    return;
+  }
+  float acc[TR][TC], accm[TR][TC];
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < TC; ++q) acc[r][q] = accm[r][q] = 0.0f;
+  const int wr0 = ty0 + (tid >> 6) * (4 * TR);
+  const bool wskip = wave_outside(it, c, wr0, 4 * TR, tx0, CT) && wave_outside(it, c, wr0, 4 * TR, mx0, CT);
+  // this lane's entries in the tables
+  const int rbase = ((tid >> 6) * 4 + ((tid >> 4) & 3)) * TR, cbase = 2 * RT + tx * TC;
+
+  run_passes_mirror<S>(it, lds, AC, ty0, tx0, wskip, [&](const unsigned char *buf, int na) {
+    const unsigned char *lds0 = lds;  // word addresses are relative to it
+#pragma unroll 2
+    for (int a = 0; a < na; ++a) {
+      const int *tab = reinterpret_cast<const int *>(buf + S::WIN) + a * S::TAB;
+      int A[TR], B[TR], Cq[TC];
+#pragma unroll
+      for (int r = 0; r < TR; r += 4) {
+        const int4 va = *reinterpret_cast<const int4 *>(tab + rbase + r);
+        const int4 vb = *reinterpret_cast<const int4 *>(tab + RT + rbase + r);
+        A[r] = va.x; A[r + 1] = va.y; A[r + 2] = va.z; A[r + 3] = va.w;
+        B[r] = vb.x; B[r + 1] = vb.y; B[r + 2] = vb.z; B[r + 3] = vb.w;
+      }
+#pragma unroll
+      for (int q = 0; q < TC; q += 4) {
+        const int4 vc = *reinterpret_cast<const int4 *>(tab + cbase + q);
+        Cq[q] = vc.x; Cq[q + 1] = vc.y; Cq[q + 2] = vc.z; Cq[q + 3] = vc.w;
+      }
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        uint32_t WA[TC], WB[TC];
+#pragma unroll
+        for (int q = 0; q < TC; ++q) {
+          const unsigned ua = static_cast<unsigned>(A[r] + Cq[q]);
+          const unsigned ub = static_cast<unsigned>(B[r] - Cq[q]);
+          WA[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ua >> 16) << 2));
+          WB[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ub >> 16) << 2));
+        }
+#pragma unroll
+        for (int q = 0; q < TC; ++q) {
+          acc[r][q] = add_sample<ST, 0>(acc[r][q], WA[q]);
+          acc[r][q] = add_sample<ST, 1>(acc[r][q], WB[q]);
+          accm[r][q] = add_sample<ST, 1>(accm[r][q], WA[q]);
+          accm[r][q] = add_sample<ST, 0>(accm[r][q], WB[q]);
+        }
+      }
+    }
+  });
+  store_tile<TR, TC>(it, acc, rowf, colf);
+  store_tile<TR, TC>(it, accm, rowf, colm);
+}
+
 // Grid-mapped launch: one workgroup per tile; blocks are remapped so that the
 // round-robin dispatch over the 8 XCDs gives each XCD a contiguous range of
 // tiles (neighbouring tiles share sinogram windows in that XCD's L2).
@@ -665,6 +957,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grid_waves<
     bp_tile_px2<KIND, A, ST>(it, b - it.tile_base, lds);
   else
     bp_tile<KIND, A, MODE, ST>(it, b - it.tile_base, lds);
+}
+
+// Mirrored launch: AC slots (2 AC angles) per pass.
+template <int KIND, int AC, int ST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KindGeo<KIND>::WAVES, KindGeo<KIND>::WAVES))) void bp_mirror_kernel(const Item *__restrict__ items, int n_items) {
+  __shared__ __align__(16) unsigned char lds[StageMirror<KIND, AC, ST>::BYTES];
+  int b = blockIdx.x;
+  const int q = gridDim.x >> 3;
+  if (b < (q << 3)) b = (b & 7) * q + (b >> 3);
+  const Item &it = items[find_item_by_tile(items, n_items, b)];
+  bp_tile_mirror<KIND, AC, ST>(it, b - it.tile_base, lds);
 }
 
 template <int A, int B>
@@ -720,8 +1023,20 @@ hipError_t launch_km(const Item *d_items, int n_items, int total_tiles, int ac, 
   return launch_kas<KIND, 32, MODE>(d_items, n_items, total_tiles, store, s);
 }
 
+template <int KIND, int AC>
+hipError_t launch_mirror(const Item *d_items, int n_items, int total_tiles, int store, hipStream_t s) {
+  if (store == ST_F16)
+    hipLaunchKernelGGL((bp_mirror_kernel<KIND, AC, ST_F16>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items);
+  else
+    hipLaunchKernelGGL((bp_mirror_kernel<KIND, AC, ST_BF16>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items);
+  return hipGetLastError();
+}
+
 template <int KIND>
-hipError_t launch_k(const Item *d_items, int n_items, int total_tiles, int ac, int mode, int store, hipStream_t s) {
+hipError_t launch_k(const Item *d_items, int n_items, int total_tiles, int ac, int mode, int store, bool mirror,
+                    hipStream_t s) {
+  // 8 slots (16 angles) per pass: the staging registers of 16 slots spill
+  if (mirror) return launch_mirror<KIND, 8>(d_items, n_items, total_tiles, store, s);
   if (mode == MODE_LINEAR) return launch_km<KIND, MODE_LINEAR>(d_items, n_items, total_tiles, ac, store, s);
   return launch_km<KIND, MODE_CTAS>(d_items, n_items, total_tiles, ac, store, s);
 }
@@ -729,12 +1044,13 @@ hipError_t launch_k(const Item *d_items, int n_items, int total_tiles, int ac, i
 }  // namespace
 
 hipError_t launch_backproject(const Item *d_items, int n_items, int total_tiles, int kind,
-                              int angles_per_pass, int mode, int store, hipStream_t s) {
+                              int angles_per_pass, int mode, int store, bool mirror, hipStream_t s) {
   if (total_tiles <= 0) return hipSuccess;
+  if (mirror && (mode != MODE_CTAS || store == ST_F32)) return hipErrorInvalidValue;
   switch (kind) {
-    case TK_64x64: return launch_k<TK_64x64>(d_items, n_items, total_tiles, angles_per_pass, mode, store, s);
-    case TK_128x64: return launch_k<TK_128x64>(d_items, n_items, total_tiles, angles_per_pass, mode, store, s);
-    default: return launch_k<TK_128x128>(d_items, n_items, total_tiles, angles_per_pass, mode, store, s);
+    case TK_64x64: return launch_k<TK_64x64>(d_items, n_items, total_tiles, angles_per_pass, mode, store, mirror, s);
+    case TK_128x64: return launch_k<TK_128x64>(d_items, n_items, total_tiles, angles_per_pass, mode, store, mirror, s);
+    default: return launch_k<TK_128x128>(d_items, n_items, total_tiles, angles_per_pass, mode, store, mirror, s);
   }
 }
 

@@ -125,7 +125,24 @@ void drop_extra(const ctas_fbp *h) {
   g_extra.erase(h);
 }
 
-Item base_item(const ctas_fbp *h, int kind) {
+// Mirror slots of the handle's FP64 table (one slot per angle at most).
+int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
+  std::vector<MirrorSlot> slots;
+  h->n_paired = mirror_slots(sc64, slots);
+  h->n_slots = static_cast<int>(slots.size());
+  if (!h->d_slots && hipMalloc(&h->d_slots, sizeof(MirrorSlot) * std::max(h->N, 1)) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation of the mirror slots failed");
+  CTAS_HIP_TRY(hipMemcpy(h->d_slots, slots.data(), sizeof(MirrorSlot) * slots.size(), hipMemcpyHostToDevice));
+  return CTAS_OK;
+}
+
+// FP16/BF16 storage with CTAS sampling takes the mirrored path when at
+// least half of the angles pair up as theta and pi - theta.
+bool use_mirror(const ctas_fbp *h) {
+  return mode_of(h->flags) == MODE_CTAS && store_of(h->flags) != ST_F32 && 4 * h->n_paired >= h->N;
+}
+
+Item base_item(const ctas_fbp *h, int kind, bool mirror) {
   Item it{};
   it.sincos = h->d_sincos;
   it.sincos64 = use_fp64_angles(h->flags) ? h->d_sincos64 : nullptr;
@@ -137,9 +154,15 @@ Item base_item(const ctas_fbp *h, int kind) {
   it.N = h->N;
   it.Z = h->Z;
   it.logZ = h->logZ;
-  it.tiles_x = (h->w + tile_cols(kind) - 1) / tile_cols(kind);
+  // mirrored: tiles cover the columns 0 .. w/2, each with its mirror image
+  const int cols = mirror ? h->w / 2 + 1 : h->w;
+  it.tiles_x = (cols + tile_cols(kind) - 1) / tile_cols(kind);
   it.tiles = it.tiles_x * ((h->w + tile_rows(kind) - 1) / tile_rows(kind));
   it.kind = kind;
+  if (mirror) {
+    it.slots = h->d_slots;
+    it.n_slots = h->n_slots;
+  }
   return it;
 }
 
@@ -174,10 +197,11 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   if (int rc = grow(h->d_filt, h->filt_cap, slice_elems * S * store_bytes(st))) return rc;
   if (int rc = ensure_slices(h, S)) return rc;
   const int kind = choose_kind(h);
+  const bool mirror = use_mirror(h);
   const int pairs = (h->N + 1) / 2;
   std::vector<Item> items(S);
   for (int i = 0; i < S; ++i) {
-    Item it = base_item(h, kind);
+    Item it = base_item(h, kind, mirror);
     it.sino = d_sino + slice_elems * i;
     it.filt = static_cast<unsigned char *>(h->d_filt) + slice_elems * i * store_bytes(st);
     it.filt_f32 = d_f32 ? d_f32 + slice_elems * i : nullptr;
@@ -195,7 +219,7 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   CTAS_HIP_TRY(launch_filter(h->d_items, S, pairs * S, h->logZ, h->contrast, st, s));
   if (do_bp)
     CTAS_HIP_TRY(launch_backproject(h->d_items, S, items[0].tiles * S, kind, choose_ac(h),
-                                    mode_of(h->flags), st, s));
+                                    mode_of(h->flags), st, mirror, s));
   h->last_S = S;
   return CTAS_OK;
 }
@@ -212,10 +236,11 @@ int enqueue_repeat(ctas_fbp *h, int slice, float *d_out, const float *centers, i
     if (int rc = check_center(h->w, centers[i])) return rc;
   const int st = store_of(h->flags);
   const int kind = choose_kind(h);
+  const bool mirror = use_mirror(h);
   const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
   std::vector<Item> items(n);
   for (int i = 0; i < n; ++i) {
-    Item it = base_item(h, kind);
+    Item it = base_item(h, kind, mirror);
     it.filt = static_cast<unsigned char *>(h->d_filt) + slice_elems * (per_slice ? i : slice) * store_bytes(st);
     it.out = d_out + static_cast<size_t>(h->w) * h->w * i;
     it.center = centers[i];
@@ -226,7 +251,7 @@ int enqueue_repeat(ctas_fbp *h, int slice, float *d_out, const float *centers, i
   bool changed = false;
   if (int rc = upload_items(h, items, s, &changed, extra(h).resident)) return rc;
   CTAS_HIP_TRY(launch_backproject(h->d_items, n, items[0].tiles * n, kind, choose_ac(h),
-                                  mode_of(h->flags), st, s));
+                                  mode_of(h->flags), st, mirror, s));
   return CTAS_OK;
 }
 
@@ -479,6 +504,7 @@ int ctas_fbp_create(ctas_fbp **ph, int w, int n_angles, int contrast, int filter
     angle_table64(n_angles, anarc, sc64.data());
     h->neg_sin = any_negative_sine(sc64);
     CTAS_HIP_TRY(hipMemcpy(h->d_sincos64, sc64.data(), sizeof(double) * sc64.size(), hipMemcpyHostToDevice));
+    if (int rc = upload_slots(h.get(), sc64)) return rc;
   }
   CTAS_HIP_TRY(hipMemcpy(h->d_W, h->W.data(), sizeof(float) * h->Z, hipMemcpyHostToDevice));
   CTAS_HIP_TRY(hipMemcpy(h->d_sincos, h->sincos.data(), sizeof(float) * h->sincos.size(), hipMemcpyHostToDevice));
@@ -505,7 +531,7 @@ int ctas_fbp_destroy(ctas_fbp *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
-  void *ptrs[] = {h->d_W, h->d_sincos, h->d_sincos64, h->d_tw, h->d_filt, h->d_io, h->d_out, h->d_minmax,
+  void *ptrs[] = {h->d_W, h->d_sincos, h->d_sincos64, h->d_slots, h->d_tw, h->d_filt, h->d_io, h->d_out, h->d_minmax,
                   h->d_status, h->d_items, h->d_centres, h->d_partial, h->d_tmp, h->d_scores};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -549,7 +575,7 @@ int ctas_fbp_set_angles(ctas_fbp *h, const float *sincos) {
   std::vector<double> sc64(h->sincos.begin(), h->sincos.end());
   h->neg_sin = any_negative_sine(sc64);
   CTAS_HIP_TRY(hipMemcpy(h->d_sincos64, sc64.data(), sizeof(double) * sc64.size(), hipMemcpyHostToDevice));
-  return CTAS_OK;
+  return upload_slots(h, sc64);
 }
 
 int ctas_fbp_get_angles(const ctas_fbp *h, float *sincos) {

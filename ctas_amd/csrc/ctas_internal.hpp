@@ -43,6 +43,20 @@ float improve_me(float eps, const std::vector<float> &vax, const std::vector<flo
                  const std::function<float(float)> &f, int *rc);
 
 // ---- device descriptors -----------------------------------------------------
+// One step of the mirrored backprojection (FP16/BF16 storage): angle row_a
+// and angle row_b with theta_b = pi - theta_a (equal sines, opposite
+// cosines), or row_b = -1 when row_a has no such partner. A pixel and its
+// mirror about the axis column w/2 then have swapped ray coordinates:
+// t_a(p) = t_b(p') and t_a(p') = t_b(p).
+struct MirrorSlot {
+  double s, c;  // FP64 sin and cos of angle row_a
+  int row_a, row_b;
+  int pad[2];
+};
+// Pairs first (ordered by row_a), then the angles without a partner.
+// Returns the number of pairs.
+int mirror_slots(const std::vector<double> &sc64, std::vector<MirrorSlot> &slots);
+
 // One sinogram -> one slice. The backprojector walks tiles of every item;
 // the filter walks row pairs of every item that needs filtering.
 struct Item {
@@ -66,6 +80,8 @@ struct Item {
   int pair_base;          // prefix sum of row pairs over the items
   int kind;               // tile kind used for this item (persistent kernel)
   int neg_sin;             // some angle has sin < 0 (arcs past 180 degrees)
+  int n_slots;             // mirrored launch: number of slots
+  const MirrorSlot *slots; // mirrored launch: [n_slots]
 };
 
 constexpr int kMinMaxBlocks = 64;
@@ -92,8 +108,10 @@ hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s);
 hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int max_logZ,
                          int contrast, int store, hipStream_t s);
 // Backprojection of all tiles of all items (grid-mapped).
+// mirror: FP16/BF16 storage with CTAS sampling, items carry mirror slots and
+// their tiles cover the columns up to w/2 together with the mirrored ones.
 hipError_t launch_backproject(const Item *d_items, int n_items, int total_tiles, int kind,
-                              int angles_per_pass, int mode, int store, hipStream_t s);
+                              int angles_per_pass, int mode, int store, bool mirror, hipStream_t s);
 // Same, one persistent launch with a work counter; items may mix tile kinds.
 hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int total_tiles,
                                          int mode, int store, unsigned *d_counter,
@@ -148,6 +166,8 @@ struct ctas_fbp {
   float2 *d_sincos = nullptr;
   double2 *d_sincos64 = nullptr;
   int neg_sin = 0;             // some entry of the FP64 table has sin < 0
+  ctas::MirrorSlot *d_slots = nullptr;  // [N] mirror slots of the FP64 table
+  int n_slots = 0, n_paired = 0;
   float2 *d_tw = nullptr;
   hipStream_t stream = nullptr;
 

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,39 @@ void angle_table64(int n, float anarc, double *sincos) {
     sincos[2 * i] = std::sin(th);
     sincos[2 * i + 1] = std::cos(th);
   }
+}
+
+// Angles whose FP64 entries have equal sines and opposite cosines (to
+// 1e-12) pair up; an angle with cosine 0 is its own mirror and stays
+// single. Lookup by the entries rounded to 1e-9, each candidate checked
+// exactly; a pair missed at a rounding boundary only costs speed.
+int mirror_slots(const std::vector<double> &sc64, std::vector<MirrorSlot> &slots) {
+  const int n = static_cast<int>(sc64.size() / 2);
+  auto key = [](double s, double c) { return std::make_pair(std::llround(s * 1e9), std::llround(c * 1e9)); };
+  std::map<std::pair<long long, long long>, std::vector<int>> by_key;
+  for (int i = 0; i < n; ++i) by_key[key(sc64[2 * i], sc64[2 * i + 1])].push_back(i);
+  std::vector<int> partner(n, -1);
+  for (int i = 0; i < n; ++i) {
+    if (partner[i] >= 0) continue;
+    const double s = sc64[2 * i], c = sc64[2 * i + 1];
+    const auto f = by_key.find(key(s, -c));
+    if (f == by_key.end()) continue;
+    for (int j : f->second) {
+      if (j == i || partner[j] >= 0) continue;
+      if (std::fabs(sc64[2 * j] - s) <= 1e-12 && std::fabs(sc64[2 * j + 1] + c) <= 1e-12) {
+        partner[i] = j;
+        partner[j] = i;
+        break;
+      }
+    }
+  }
+  slots.clear();
+  for (int i = 0; i < n; ++i)
+    if (partner[i] > i) slots.push_back(MirrorSlot{sc64[2 * i], sc64[2 * i + 1], i, partner[i], {0, 0}});
+  const int paired = static_cast<int>(slots.size());
+  for (int i = 0; i < n; ++i)
+    if (partner[i] < 0) slots.push_back(MirrorSlot{sc64[2 * i], sc64[2 * i + 1], i, -1, {0, 0}});
+  return paired;
 }
 
 namespace {

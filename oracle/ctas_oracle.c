@@ -305,13 +305,16 @@ static void sincos_exact(double th, double *s, double *c) {
 }
 
 /* FP64 oracle of the same discretisation: angles i*arc/N and the ray
- * coordinate in double, same masks, samples at floor(t) (linear == 0) or
- * linearly interpolated between bin centres k+1/2 (linear == 1), with
- * zero outside the detector. */
+ * coordinate in double, same masks. linear == 0: the reference's sampling
+ * (lct.cpp:710-713), the sample at the flat offset p*w + (long)t of the
+ * sinogram (truncation; a coordinate outside the row reads the neighbouring
+ * row, 0 outside the sinogram). linear == 1: linear interpolation between
+ * bin centres k+1/2, zero outside the detector. */
 void orc_bp_f64(const float *P, int N, int w, double arc_rad, double c,
                 double recCof, int linear, double *out) {
   const int hp = w / 2;
   const double acent = fabs(c);
+  const long total_len = (long)N * w;
   double *S = (double *)malloc(sizeof(double) * (size_t)N);
   double *C = (double *)malloc(sizeof(double) * (size_t)N);
   for (int p = 0; p < N; p++) sincos_exact(p * arc_rad / N, &S[p], &C[p]);
@@ -330,10 +333,8 @@ void orc_bp_f64(const float *P, int N, int w, double arc_rad, double c,
         const double t = c + hp + S[p] * (j - c - hp) + C[p] * (i - hp);
         const float *row = P + (size_t)p * w;
         if (!linear) {
-          long k = (long)floor(t);
-          if (k < 0) k = 0;
-          if (k > w - 1) k = w - 1;
-          acc += row[k];
+          const long flat = (long)p * w + (long)t;
+          acc += (flat >= 0 && flat < total_len) ? P[flat] : 0.0;
         } else {
           const double u = t - 0.5;
           const long k = (long)floor(u);

@@ -128,15 +128,54 @@ def test_reduced_precision_storage(store, interp):
     assert err <= 2e-3
 
 
-def test_repeat_matches_reconstruct():
+@pytest.mark.parametrize("store", [ca.STORE_FP16, ca.STORE_BF16])
+@pytest.mark.parametrize("kind", [ca.TILE_64x64, ca.TILE_128x64, ca.TILE_128x128])
+@pytest.mark.parametrize("w,N,c,arc", [(384, 360, 0.0, 180.0), (257, 181, 3.37, 180.0),
+                                       (320, 400, -5.25, 360.0), (256, 240, 2.0, 150.0)])
+def test_reduced_precision_ctas_paths(store, kind, w, N, c, arc):
+    """CTAS sampling of FP16/BF16 storage against the FP64 oracle. 180 and
+    360 degree arcs take the mirrored path (every angle but 0, 90 and 270
+    degrees has a partner at 180 - theta), including an odd width, whose
+    mirror column is w - 1 - x, and centre offsets; a 150 degree arc has no
+    pairs and takes the row-pair path. Every tile kind. (A coordinate that
+    is exactly a negative integer is read one bin off the reference's
+    truncation by the fixed-point paths; a centre of -5.5 px makes whole rows
+    of such coordinates at 270 degrees, so the 360-degree case uses -5.25.)"""
+    if arc == 360.0:
+        # second half-turn: P(theta + pi, s) = P(theta, -s), s taken about
+        # the axis at k + 1/2 = w/2 + c (to the nearest bin)
+        half = _phantom(N // 2, w, c, 12, 1e5, seed=w + N)
+        k = np.arange(w)
+        km = np.rint(2 * (w // 2) + 2 * c - 1 - k).astype(np.int64)
+        flip = np.where((km >= 0) & (km < w), half[:, np.clip(km, 0, w - 1)], 1.0).astype(np.float32)
+        P = np.concatenate([half, flip])
+    else:
+        P = _phantom(N, w, c, 12, 1e5, seed=w + N)
+    rec = ca.CTrec(w, N, ca.ABS, "HANN", store=store, anarc=arc)
+    rec.set_tiles(kind, 32)
+    _, out = rec.reconstruct(P, c)
+    W = oracle.filter_window(5, 0.0, w, ca.ABS)
+    Pf = oracle.prepare(P, W, ca.ABS)
+    ref = oracle.bp_f64(Pf, c, np.deg2rad(arc), 1.0)
+    err = rel_rms(out[0], ref)
+    worst = float(np.abs(out[0] - ref).max() / np.abs(ref).max())
+    print(f"store={store:#x} kind={kind} w={w} N={N} c={c} arc={arc}: rel_rms={err:.3e} max={worst:.3e}")
+    assert err <= 2e-3
+    # per-pixel bound: the mirrored path forms each coordinate from terms
+    # rounded once; the row-pair path accumulates rounded steps (~1e-3 px)
+    assert worst <= (1.5e-2 if arc in (180.0, 360.0) else 5e-2)
+
+
+@pytest.mark.parametrize("store", [ca.STORE_FP32, ca.STORE_FP16])
+def test_repeat_matches_reconstruct(store):
     N, w = 180, 256
     P = _phantom(N, w, 0.0, 6, 1e5, seed=3)
-    rec = ca.CTrec(w, N, ca.ABS, "RAMP")
+    rec = ca.CTrec(w, N, ca.ABS, "RAMP", store=store)
     rec.sino(P)
     cs = np.array([-4.5, 0.0, 2.25, 10.0], np.float32)
     many = rec.repeat(cs)
     for k, c in enumerate(cs):
-        _, one = ca.CTrec(w, N, ca.ABS, "RAMP").reconstruct(P, c)
+        _, one = ca.CTrec(w, N, ca.ABS, "RAMP", store=store).reconstruct(P, c)
         assert np.array_equal(many[k], one[0])
 
 

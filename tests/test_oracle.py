@@ -233,6 +233,33 @@ def test_bp_float_vs_f64_floor():
     assert close.mean() > 0.99
 
 
+def test_bp_f64_reads_outside_the_row_like_the_reference():
+    """A 360-degree arc with a centre offset of 9 px puts coordinates of a
+    quarter of the in-mask pixels outside the row. The FP64 oracle samples
+    them as the reference does (lct.cpp:710-713: flat offset p*w + (long)t,
+    0 outside the sinogram), so it agrees with the float restatement
+    (orc_bp_ctas, pinned by its own known-answer tests) wherever rounding
+    does not flip a bin; clamping the index to the row instead would leave
+    only ~71% of the pixels in agreement."""
+    N, w, c, arc = 90, 96, 9.0, 2 * np.pi
+    rng = np.random.default_rng(5)
+    P = (rng.standard_normal((N, w)) + 3.0).astype(np.float32)
+    th = np.arange(N) * arc / N
+    hp = w // 2
+    rows, cols = np.arange(w)[:, None], np.arange(w)[None, :]
+    outside = np.zeros((w, w), bool)
+    for p in range(N):
+        t = (c + hp) + np.sin(th[p]) * (rows - hp - c) + np.cos(th[p]) * (cols - hp)
+        outside |= (t < 0) | (t >= w)
+    b = oracle.bp_f64(P, c, arc, 1.0, linear=False)
+    mask = b != 0
+    assert (outside & mask).sum() > 0.2 * mask.sum()
+    a = oracle.bp_ctas(P, np.stack([np.sin(th), np.cos(th)], 1).astype(np.float32), c)
+    close = np.abs(a - b) <= 1e-5 * np.abs(b).max()
+    assert close[mask].mean() > 0.99
+    assert close[mask & outside].mean() > 0.99
+
+
 def test_reference_table_flips_at_90_degrees():
     """The reference's float table has cos(float(pi/2)) = -4.37e-8, so at
     theta = 90 deg the exact-integer ray coordinate t = j lands just below j
