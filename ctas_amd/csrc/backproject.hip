@@ -172,6 +172,15 @@ template <class T>
 __device__ __forceinline__ T ldg(const T *p, long long i) {
   return ((const __attribute__((address_space(1))) T *)(p))[i];
 }
+// 32-bit element offset from a uniform base (a slice holds fewer than 2^31
+// samples): the load takes the base in scalar registers and the offset in
+// one vector register.
+template <class T>
+__device__ __forceinline__ T ldg32(const T *p, unsigned i) {
+  const unsigned byte = i * static_cast<unsigned>(sizeof(T));
+  const __attribute__((address_space(1))) char *b = (const __attribute__((address_space(1))) char *)(p);
+  return *(const __attribute__((address_space(1))) T *)(b + byte);
+}
 __device__ __forceinline__ float2 ldg(const float2 *p, long long i) {
   const float *q = reinterpret_cast<const float *>(p) + 2 * i;
   return make_float2(ldg(q, 0), ldg(q, 1));
@@ -456,9 +465,7 @@ struct StagePair {
   static constexpr size_t HALFBUF = WIN + 2 * sizeof(int4) * AC;
   static constexpr size_t ANGS = 2 * HALFBUF;
   static constexpr size_t BYTES = ANGS + 2 * sizeof(double2) * AC;
-  static_assert(2 * EPT <= 64, "one validity bit per sample");
   uint16_t v[EPT][2];
-  uint64_t ok;  // bit (2 j + i): sample i of pair j lies inside the slice
   int4 prm0, prm1;
   double2 sd, sd_next;  // table entries of the pass being fetched and of the one after
 
@@ -487,18 +494,14 @@ struct StagePair {
     const int thr = neg ? -S : 65536 - S;
     prm0 = make_int4(bt, S, C, thr | (neg ? (1 << 30) : 0));
     prm1 = make_int4(8 * S, 16 * C, 0, 0);
-    const long long total = static_cast<long long>(it.N) * w;
+    // reads outside the sinogram land in its zero pads
     const long long rb = static_cast<long long>(a0 + a) * w;
     const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
-    ok = 0;
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int k = lo + sub + j * TPA;
-      const long long f0 = rb + trunc_pos(k), f1 = rb + trunc_pos(k + 1);
-      const bool in0 = f0 >= 0 && f0 < total, in1 = f1 >= 0 && f1 < total;
-      v[j][0] = ldg(filt, in0 ? f0 : 0);
-      v[j][1] = ldg(filt, in1 ? f1 : 0);
-      ok |= (static_cast<uint64_t>(in0) | (static_cast<uint64_t>(in1) << 1)) << (2 * j);
+      v[j][0] = ldg(filt, rb + trunc_pos(k));
+      v[j][1] = ldg(filt, rb + trunc_pos(k + 1));
     }
   }
 
@@ -510,9 +513,7 @@ struct StagePair {
     for (int j = 0; j < EPT; ++j) {
       const int o = sub + j * TPA;
       if (o >= L) continue;
-      const uint32_t v0 = ((ok >> (2 * j)) & 1u) ? v[j][0] : 0u;
-      const uint32_t v1 = ((ok >> (2 * j + 1)) & 1u) ? v[j][1] : 0u;
-      win[a * L + o] = make_float2(half_to_float<ST>(v0), half_to_float<ST>(v1));
+      win[a * L + o] = make_float2(half_to_float<ST>(v[j][0]), half_to_float<ST>(v[j][1]));
     }
     if (sub == 0) {
       // the buffer's offset from the start of LDS goes into the base
@@ -706,11 +707,10 @@ struct StageMirror {
   static constexpr size_t HALFBUF = WIN + sizeof(int) * AC * TAB;
   static constexpr size_t ANGS = 2 * HALFBUF;
   static constexpr size_t BYTES = ANGS + 2 * sizeof(MirrorSlot) * AC;
-  static_assert(2 * EPT <= 64, "one validity bit per sample");
   static_assert(TR % 4 == 0 && TC % 4 == 0, "16-byte table reads");
   uint16_t v[EPT][2];
-  uint64_t ok;  // bit (2 j + i): sample i of word j lies inside the slice
   int loA, loB;
+  double TA, TB;  // ray coordinates of the tile corner and of its mirror, slot being fetched
   MirrorSlot sl, sl_next;  // slots of the pass being fetched and of the one after
   const unsigned char *base = nullptr;  // start of the LDS block
 
@@ -753,30 +753,29 @@ struct StageMirror {
     (void)na;
     const int tid = threadIdx.x, sub = tid - (tid / TPA) * TPA;
     const int w = it.w;
-    double TA, TB;
     corner(it, ty0, tx0, TA, TB);
     const double hy = sl.s * (0.5 * (RT - 1)), hx = sl.c * (0.5 * (CT - 1));
     loA = static_cast<int>(floor(TA + hy + hx)) - G::HALF - 2;
     loB = static_cast<int>(floor(TB + hy - hx)) - G::HALF - 2;
-    const long long total = static_cast<long long>(it.N) * w;
-    const long long ra = static_cast<long long>(sl.row_a) * w;
-    const bool has_b = sl.row_b >= 0;
-    const long long rb = static_cast<long long>(has_b ? sl.row_b : 0) * w;
-    const uint16_t *__restrict__ filt = static_cast<const uint16_t *>(it.filt);
-    ok = 0;
+    // offsets from the start of the zero pad before the sinogram; a slot
+    // without a partner reads its second samples from the pad's first element
+    const uint16_t *__restrict__ base16 = static_cast<const uint16_t *>(it.filt) - it.pad;
+    const int ra = it.pad + sl.row_a * w;
+    const int rb = it.pad + sl.row_b * w;
+    const int mb = sl.row_b >= 0 ? -1 : 0;
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;  // word e of the slot: [0, L) window A, [L, 2L) window B
-      const long long k = trunc_pos(e < L ? loA + e : loB + (e - L));
-      const long long fa = ra + k, fb = rb + k;
-      const bool ina = fa >= 0 && fa < total, inb = has_b && fb >= 0 && fb < total;
-      v[j][0] = ldg(filt, ina ? fa : 0);
-      v[j][1] = ldg(filt, inb ? fb : 0);
-      ok |= (static_cast<uint64_t>(ina) | (static_cast<uint64_t>(inb) << 1)) << (2 * j);
+      const int k = trunc_pos(e < L ? loA + e : loB + (e - L));
+      v[j][0] = ldg32(base16, static_cast<unsigned>(ra + k));
+      v[j][1] = ldg32(base16, static_cast<unsigned>((rb + k) & mb));
     }
   }
 
   __device__ __forceinline__ void store(const Item &it, unsigned char *buf, int na, int ty0, int tx0) const {
+    (void)it;
+    (void)ty0;
+    (void)tx0;
     const int tid = threadIdx.x, a = tid / TPA, sub = tid - a * TPA;
     if (a >= na) return;
     uint32_t *win = reinterpret_cast<uint32_t *>(buf);
@@ -784,25 +783,23 @@ struct StageMirror {
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;
       if (e >= L2) continue;
-      const uint32_t lo = ((ok >> (2 * j)) & 1u) ? v[j][0] : 0u;
-      const uint32_t hi = ((ok >> (2 * j + 1)) & 1u) ? v[j][1] : 0u;
-      win[a * L2 + e] = lo | (hi << 16);
+      win[a * L2 + e] = static_cast<uint32_t>(v[j][0]) | (static_cast<uint32_t>(v[j][1]) << 16);
     }
-    // word addresses are taken from the start of the LDS block
+    // word addresses are taken from the start of the LDS block; every entry
+    // is one FP64 fma of pre-scaled terms, rounded once
     const double off = static_cast<double>((buf - base) / sizeof(uint32_t) + a * L2);
-    double TA, TB;
-    corner(it, ty0, tx0, TA, TB);
-    const double ba = TA - loA + off, bb = TB - loB + off + L;
+    const double ba = (TA - loA + off) * 65536.0, bb = (TB - loB + off + L) * 65536.0;
+    const double s16 = sl.s * 65536.0, c16 = sl.c * 65536.0;
     int *tab = reinterpret_cast<int *>(buf + WIN) + a * TAB;
 #pragma unroll
     for (int j = 0; j < TPT; ++j) {
       const int e = sub + j * TPA;
       if (e < RT)
-        tab[row_index(e)] = static_cast<int>(llrint((ba + sl.s * e) * 65536.0));
+        tab[row_index(e)] = static_cast<int>(rint(fma(s16, static_cast<double>(e), ba)));
       else if (e < 2 * RT)
-        tab[RT + row_index(e - RT)] = static_cast<int>(llrint((bb + sl.s * (e - RT)) * 65536.0));
+        tab[RT + row_index(e - RT)] = static_cast<int>(rint(fma(s16, static_cast<double>(e - RT), bb)));
       else if (e < TAB)
-        tab[2 * RT + col_index(e - 2 * RT)] = static_cast<int>(llrint(sl.c * (e - 2 * RT) * 65536.0));
+        tab[2 * RT + col_index(e - 2 * RT)] = static_cast<int>(rint(c16 * (e - 2 * RT)));
     }
   }
 };

@@ -136,10 +136,16 @@ int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
   return CTAS_OK;
 }
 
+// Filtered sinograms of 16-bit storage carry zero pads (read without
+// bounds checks); float32 ones are contiguous (the host API copies them out).
+int filt_pad_of(const ctas_fbp *h) { return store_of(h->flags) == ST_F32 ? 0 : filt_pad(h->w); }
+
 // FP16/BF16 storage with CTAS sampling takes the mirrored path when at
 // least half of the angles pair up as theta and pi - theta.
 bool use_mirror(const ctas_fbp *h) {
-  return mode_of(h->flags) == MODE_CTAS && store_of(h->flags) != ST_F32 && 4 * h->n_paired >= h->N;
+  // its staging indexes a sinogram with 32-bit offsets
+  return mode_of(h->flags) == MODE_CTAS && store_of(h->flags) != ST_F32 && 4 * h->n_paired >= h->N &&
+         static_cast<long long>(h->N) * h->w + 2LL * filt_pad(h->w) < (1LL << 31);
 }
 
 Item base_item(const ctas_fbp *h, int kind, bool mirror) {
@@ -194,7 +200,9 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
       if (int rc = check_center(h->w, centers[i])) return rc;
   const int st = store_of(h->flags);
   const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
-  if (int rc = grow(h->d_filt, h->filt_cap, slice_elems * S * store_bytes(st))) return rc;
+  const int pad = filt_pad_of(h);
+  const size_t stride = slice_elems + 2 * static_cast<size_t>(pad);
+  if (int rc = grow(h->d_filt, h->filt_cap, stride * S * store_bytes(st))) return rc;
   if (int rc = ensure_slices(h, S)) return rc;
   const int kind = choose_kind(h);
   const bool mirror = use_mirror(h);
@@ -203,7 +211,8 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   for (int i = 0; i < S; ++i) {
     Item it = base_item(h, kind, mirror);
     it.sino = d_sino + slice_elems * i;
-    it.filt = static_cast<unsigned char *>(h->d_filt) + slice_elems * i * store_bytes(st);
+    it.filt = static_cast<unsigned char *>(h->d_filt) + (stride * i + pad) * store_bytes(st);
+    it.pad = pad;
     it.filt_f32 = d_f32 ? d_f32 + slice_elems * i : nullptr;
     it.out = d_out ? d_out + static_cast<size_t>(h->w) * h->w * i : nullptr;
     it.minmax = h->d_minmax + static_cast<size_t>(kMinMaxBlocks) * i;
@@ -215,6 +224,7 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   }
   bool changed = false;
   if (int rc = upload_items(h, items, s, &changed, extra(h).resident)) return rc;
+  if (pad) CTAS_HIP_TRY(launch_pad_zero(h->d_items, S, s));
   if (h->contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(h->d_items, S, s));
   CTAS_HIP_TRY(launch_filter(h->d_items, S, pairs * S, h->logZ, h->contrast, st, s));
   if (do_bp)
@@ -238,10 +248,13 @@ int enqueue_repeat(ctas_fbp *h, int slice, float *d_out, const float *centers, i
   const int kind = choose_kind(h);
   const bool mirror = use_mirror(h);
   const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
+  const int pad = filt_pad_of(h);
+  const size_t stride = slice_elems + 2 * static_cast<size_t>(pad);
   std::vector<Item> items(n);
   for (int i = 0; i < n; ++i) {
     Item it = base_item(h, kind, mirror);
-    it.filt = static_cast<unsigned char *>(h->d_filt) + slice_elems * (per_slice ? i : slice) * store_bytes(st);
+    it.filt = static_cast<unsigned char *>(h->d_filt) + (stride * (per_slice ? i : slice) + pad) * store_bytes(st);
+    it.pad = pad;
     it.out = d_out + static_cast<size_t>(h->w) * h->w * i;
     it.center = centers[i];
     it.tile_base = it.tiles * i;

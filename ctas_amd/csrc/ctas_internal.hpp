@@ -82,7 +82,15 @@ struct Item {
   int neg_sin;             // some angle has sin < 0 (arcs past 180 degrees)
   int n_slots;             // mirrored launch: number of slots
   const MirrorSlot *slots; // mirrored launch: [n_slots]
+  int pad;                 // zero samples before and after filt (16-bit storage), 0 otherwise
 };
+
+// Zero samples kept before and after each filtered sinogram of 16-bit
+// storage: every window the backprojection stages (tile coordinates within
+// sqrt(2) (w/2 + 32 + |c|) of the axis, |c| < w/2, plus the window length)
+// then reads inside the allocation, and its reads outside the sinogram
+// return 0 without a bounds check.
+inline int filt_pad(int w) { return (3 * w) / 2 + 256; }
 
 constexpr int kMinMaxBlocks = 64;
 
@@ -104,6 +112,8 @@ inline int tile_cols(int kind) { return kind == TK_128x128 ? 128 : 64; }
 // ---- launchers (filter.hip, backproject.hip) ------------------------------
 // Per-item (min, max) partials of the raw sinograms (ABS contrast only).
 hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s);
+// Zeros the pads around the filtered sinograms of the items (Item::pad).
+hipError_t launch_pad_zero(const Item *d_items, int n_items, hipStream_t s);
 // deAbs + padded FFT filtering + REF prefix sum for all row pairs.
 hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int max_logZ,
                          int contrast, int store, hipStream_t s);

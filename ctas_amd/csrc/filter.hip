@@ -316,7 +316,23 @@ __global__ __launch_bounds__(256) void passthrough_kernel(const Item *__restrict
   if (pair == 0 && threadIdx.x == 0) *it.status = 0;
 }
 
+// Zeros of the pads around one item's filtered sinogram (16-bit samples).
+__global__ void pad_zero_kernel(const Item *__restrict__ items) {
+  const Item &it = items[blockIdx.x];
+  uint16_t *f = static_cast<uint16_t *>(it.filt);
+  const long long n = static_cast<long long>(it.N) * it.w;
+  for (int j = threadIdx.x; j < it.pad; j += blockDim.x) {
+    f[j - it.pad] = 0;
+    f[n + j] = 0;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_pad_zero(const Item *d_items, int n_items, hipStream_t s) {
+  hipLaunchKernelGGL(pad_zero_kernel, dim3(n_items), dim3(256), 0, s, d_items);
+  return hipGetLastError();
+}
 
 hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s) {
   dim3 grid(kMinMaxBlocks, n_items);
