@@ -647,6 +647,7 @@ __device__ __forceinline__ void bp_tile_px2(const Item &it, int tile, unsigned c
 // with zero high halves. The column hp is its own mirror: the first sum
 // alone is its value.
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
 
 // acc + the low (HI = 0) or high (HI = 1) sample of a word, converted and
@@ -686,13 +687,13 @@ __device__ __forceinline__ float add_sample(float acc, uint32_t w) {
 // then reads the pair at word RA[y] + CX[x] of window A, its mirror the pair
 // at RB[y] - CX[x] of window B. A lane reads its row terms {RA, RB} and its
 // column terms as 16-byte words; the slot header holds thr and the sign.
-// Coordinates: per slot, the staging threads tabulate in FP64, each term
-// rounded once to 16.16 fixed point, the row terms RA[y] = TA - loA + s y
-// (window A, word offsets and buffer included), RB[y] likewise for window
-// B, and the column terms CX[x] = c x, for the RT rows and CT columns of
-// the tile. Pixel (y, x) then reads word RA[y] + CX[x] of window A and its
-// mirror word RB[y] - CX[x] of window B, within 2^-16 px of the FP64
-// coordinate. The tables are laid out in lattice order, so that a lane
+// Coordinates, 16.16 fixed point: per slot the staging threads form the
+// row terms RA[y] = round(TA - loA) + SY[y] (window A, word offsets and
+// buffer included), RB[y] likewise for window B, and the column terms
+// CX[x], from the slot's precomputed SY[y] = round(s y 2^16) and
+// CX[x] = round(c x 2^16). Pixel (y, x) then reads word RA[y] + CX[x] of
+// window A and its mirror word RB[y] - CX[x] of window B, within 2^-15 px
+// of the FP64 coordinate. The tables are laid out in lattice order, so that a lane
 // reads its TR row terms and TC column terms as 16-byte words. Per 4
 // updates (pixel and mirror, two angles): two coordinates, two 4-byte LDS
 // reads, four dot2 accumulations.
@@ -702,13 +703,14 @@ struct StageMirror {
   static constexpr int L = G::L, L2 = 2 * L, TPA = 256 / AC, EPT = (L2 + TPA - 1) / TPA;
   static constexpr int RT = G::RT, CT = G::CT, TR = G::TR, TC = G::TC;
   static constexpr int TAB = 2 * RT + CT;  // ints per slot: RA, RB, CX
-  static constexpr int TPT = (TAB + TPA - 1) / TPA;
+  static constexpr int TPT = (RT + CT + TPA - 1) / TPA;  // SY and CX entries per thread
   static constexpr size_t WIN = sizeof(uint32_t) * AC * L2;
   static constexpr size_t HALFBUF = WIN + sizeof(int) * AC * TAB;
   static constexpr size_t ANGS = 2 * HALFBUF;
   static constexpr size_t BYTES = ANGS + 2 * sizeof(MirrorSlot) * AC;
   static_assert(TR % 4 == 0 && TC % 4 == 0, "16-byte table reads");
-  uint16_t v[EPT][2];
+  u16x2 v[EPT];  // the two samples of word j
+  int t[TPT];    // SY (rows) and CX (columns) entries of this thread
   int loA, loB;
   double TA, TB;  // ray coordinates of the tile corner and of its mirror, slot being fetched
   MirrorSlot sl, sl_next;  // slots of the pass being fetched and of the one after
@@ -749,9 +751,13 @@ struct StageMirror {
 
   // tx0: first column of the tile; the mirrored tile's columns are 2hp - x.
   __device__ __forceinline__ void fetch(const Item &it, int a0, int na, int ty0, int tx0) {
-    (void)a0;
-    (void)na;
     const int tid = threadIdx.x, sub = tid - (tid / TPA) * TPA;
+    const int *st = it.slot_tab + (a0 + min(tid / TPA, na - 1)) * kSlotTab;
+#pragma unroll
+    for (int j = 0; j < TPT; ++j) {
+      const int e = min(sub + j * TPA, RT + CT - 1);
+      t[j] = ldg32(st, static_cast<unsigned>(e < RT ? e : kSlotTab / 2 + (e - RT)));
+    }
     const int w = it.w;
     corner(it, ty0, tx0, TA, TB);
     const double hy = sl.s * (0.5 * (RT - 1)), hx = sl.c * (0.5 * (CT - 1));
@@ -767,8 +773,8 @@ struct StageMirror {
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;  // word e of the slot: [0, L) window A, [L, 2L) window B
       const int k = trunc_pos(e < L ? loA + e : loB + (e - L));
-      v[j][0] = ldg32(base16, static_cast<unsigned>(ra + k));
-      v[j][1] = ldg32(base16, static_cast<unsigned>((rb + k) & mb));
+      v[j].x = ldg32(base16, static_cast<unsigned>(ra + k));
+      v[j].y = ldg32(base16, static_cast<unsigned>((rb + k) & mb));
     }
   }
 
@@ -783,23 +789,22 @@ struct StageMirror {
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;
       if (e >= L2) continue;
-      win[a * L2 + e] = static_cast<uint32_t>(v[j][0]) | (static_cast<uint32_t>(v[j][1]) << 16);
+      win[a * L2 + e] = __builtin_bit_cast(uint32_t, v[j]);
     }
-    // word addresses are taken from the start of the LDS block; every entry
-    // is one FP64 fma of pre-scaled terms, rounded once
-    const double off = static_cast<double>((buf - base) / sizeof(uint32_t) + a * L2);
-    const double ba = (TA - loA + off) * 65536.0, bb = (TB - loB + off + L) * 65536.0;
-    const double s16 = sl.s * 65536.0, c16 = sl.c * 65536.0;
+    // word addresses are taken from the start of the LDS block
+    const int off = static_cast<int>((buf - base) / sizeof(uint32_t)) + a * L2;
+    const int ba = static_cast<int>(rint((TA - loA) * 65536.0)) + (off << 16);
+    const int bb = static_cast<int>(rint((TB - loB) * 65536.0)) + ((off + L) << 16);
     int *tab = reinterpret_cast<int *>(buf + WIN) + a * TAB;
 #pragma unroll
     for (int j = 0; j < TPT; ++j) {
       const int e = sub + j * TPA;
-      if (e < RT)
-        tab[row_index(e)] = static_cast<int>(rint(fma(s16, static_cast<double>(e), ba)));
-      else if (e < 2 * RT)
-        tab[RT + row_index(e - RT)] = static_cast<int>(rint(fma(s16, static_cast<double>(e - RT), bb)));
-      else if (e < TAB)
-        tab[2 * RT + col_index(e - 2 * RT)] = static_cast<int>(rint(c16 * (e - 2 * RT)));
+      if (e < RT) {
+        tab[row_index(e)] = ba + t[j];
+        tab[RT + row_index(e)] = bb + t[j];
+      } else if (e < RT + CT) {
+        tab[2 * RT + col_index(e - RT)] = t[j];
+      }
     }
   }
 };

@@ -132,7 +132,16 @@ int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
   h->n_slots = static_cast<int>(slots.size());
   if (!h->d_slots && hipMalloc(&h->d_slots, sizeof(MirrorSlot) * std::max(h->N, 1)) != hipSuccess)
     return fail(CTAS_E_NOMEM, "device allocation of the mirror slots failed");
+  if (!h->d_slot_tab && hipMalloc(&h->d_slot_tab, sizeof(int) * kSlotTab * std::max(h->N, 1)) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation of the mirror slot tables failed");
+  std::vector<int> tab(static_cast<size_t>(kSlotTab) * slots.size());
+  for (size_t i = 0; i < slots.size(); ++i)
+    for (int k = 0; k < kSlotTab / 2; ++k) {
+      tab[i * kSlotTab + k] = static_cast<int>(std::lrint(slots[i].s * k * 65536.0));
+      tab[i * kSlotTab + kSlotTab / 2 + k] = static_cast<int>(std::lrint(slots[i].c * k * 65536.0));
+    }
   CTAS_HIP_TRY(hipMemcpy(h->d_slots, slots.data(), sizeof(MirrorSlot) * slots.size(), hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(h->d_slot_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
   return CTAS_OK;
 }
 
@@ -167,6 +176,7 @@ Item base_item(const ctas_fbp *h, int kind, bool mirror) {
   it.kind = kind;
   if (mirror) {
     it.slots = h->d_slots;
+    it.slot_tab = h->d_slot_tab;
     it.n_slots = h->n_slots;
   }
   return it;
@@ -544,7 +554,7 @@ int ctas_fbp_destroy(ctas_fbp *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
-  void *ptrs[] = {h->d_W, h->d_sincos, h->d_sincos64, h->d_slots, h->d_tw, h->d_filt, h->d_io, h->d_out, h->d_minmax,
+  void *ptrs[] = {h->d_W, h->d_sincos, h->d_sincos64, h->d_slots, h->d_slot_tab, h->d_tw, h->d_filt, h->d_io, h->d_out, h->d_minmax,
                   h->d_status, h->d_items, h->d_centres, h->d_partial, h->d_tmp, h->d_scores};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);

@@ -53,6 +53,9 @@ struct MirrorSlot {
   int row_a, row_b;
   int pad[2];
 };
+// Per slot, kSlotTab ints: round(s y 2^16) for y < 128, then round(c x 2^16)
+// for x < 128 (the row and column terms of a tile's coordinates).
+constexpr int kSlotTab = 256;
 // Pairs first (ordered by row_a), then the angles without a partner.
 // Returns the number of pairs.
 int mirror_slots(const std::vector<double> &sc64, std::vector<MirrorSlot> &slots);
@@ -82,6 +85,7 @@ struct Item {
   int neg_sin;             // some angle has sin < 0 (arcs past 180 degrees)
   int n_slots;             // mirrored launch: number of slots
   const MirrorSlot *slots; // mirrored launch: [n_slots]
+  const int *slot_tab;     // mirrored launch: [n_slots][kSlotTab]
   int pad;                 // zero samples before and after filt (16-bit storage), 0 otherwise
 };
 
@@ -177,6 +181,7 @@ struct ctas_fbp {
   double2 *d_sincos64 = nullptr;
   int neg_sin = 0;             // some entry of the FP64 table has sin < 0
   ctas::MirrorSlot *d_slots = nullptr;  // [N] mirror slots of the FP64 table
+  int *d_slot_tab = nullptr;             // [N][kSlotTab] their row and column terms
   int n_slots = 0, n_paired = 0;
   float2 *d_tw = nullptr;
   hipStream_t stream = nullptr;
