@@ -650,53 +650,37 @@ typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
 
-// acc + the low (HI = 0) or high (HI = 1) sample of a word, converted and
-// added by one v_dot2 with weights (1, 0) or (0, 1): the products are exact
-// and the zero-weighted sample adds nothing, so the result is the rounded
-// float sum acc + sample.
-template <int ST, int HI>
-__device__ __forceinline__ float add_sample(float acc, uint32_t w) {
+// acc + lo(x) + hi(x): both samples of a word, one v_dot2 with weights (1, 1)
+template <int ST>
+__device__ __forceinline__ float add_pair(float acc, uint32_t x) {
   if constexpr (ST == ST_F16) {
-    const half2v wt = HI ? half2v{static_cast<_Float16>(0.0f), static_cast<_Float16>(1.0f)}
-                         : half2v{static_cast<_Float16>(1.0f), static_cast<_Float16>(0.0f)};
-    return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, w), wt, acc, false);
+    const half2v one = {static_cast<_Float16>(1.0f), static_cast<_Float16>(1.0f)};
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, x), one, acc, false);
   } else {
-    // bf16 (0, 1) / (1, 0), held in a scalar register: as an inline constant
-    // the compiler encodes (1, 0) as "1.0", which the bf16 form does not read
-    // as bf16 (1, 0)
-    uint32_t wt = HI ? 0x3F800000u : 0x00003F80u;
+    uint32_t wt = 0x3F803F80u;  // bf16 (1, 1), held in a scalar register
     asm("" : "+s"(wt));
-    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2v, w), __builtin_bit_cast(bf16x2v, wt), acc,
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2v, x), __builtin_bit_cast(bf16x2v, wt), acc,
                                            false);
   }
 }
 
-// Lanes own vertically adjacent pixel pairs (rows j, j+1, as in the row-pair
-// path above): with floor sampling row j+1 reads word k or k+1 (k = floor
-// t_j) when s >= 0, k-1 or k when s < 0, so one 8-byte read of two words
-// serves both rows of a pixel, the second row selected by the fraction of
-// t_j against thr = 1 - s (s >= 0) or -s (s < 0). Per 8 updates (pixel and
-// mirror, two rows, two angles): two 8-byte LDS reads, two compares and
-// selects, eight dot2 accumulations.
-//
-// Coordinates: per slot, the staging threads tabulate in FP64, each term
-// rounded once to 16.16 fixed point, the row terms RA[y] = TA - loA + s y
-// (window A, word offsets and buffer included; one word lower when s < 0,
-// where the pair is taken at k-1) and RB[y] likewise for window B, for the
-// first row y of every pair, and the column terms CX[x] = c x. Pixel (y, x)
-// then reads the pair at word RA[y] + CX[x] of window A, its mirror the pair
-// at RB[y] - CX[x] of window B. A lane reads its row terms {RA, RB} and its
-// column terms as 16-byte words; the slot header holds thr and the sign.
 // Coordinates, 16.16 fixed point: per slot the staging threads form the
 // row terms RA[y] = round(TA - loA) + SY[y] (window A, word offsets and
 // buffer included), RB[y] likewise for window B, and the column terms
 // CX[x], from the slot's precomputed SY[y] = round(s y 2^16) and
 // CX[x] = round(c x 2^16). Pixel (y, x) then reads word RA[y] + CX[x] of
 // window A and its mirror word RB[y] - CX[x] of window B, within 2^-15 px
-// of the FP64 coordinate. The tables are laid out in lattice order, so that a lane
-// reads its TR row terms and TC column terms as 16-byte words. Per 4
+// of the FP64 coordinate. The tables are laid out in lattice order, so that
+// a lane reads its TR row terms and TC column terms as 16-byte words. Per 4
 // updates (pixel and mirror, two angles): two coordinates, two 4-byte LDS
-// reads, four dot2 accumulations.
+// reads, two bit-field merges and two dot2 accumulations.
+// Integer constants of one slot for one tile: window starts, the 16.16
+// bases of the row terms (word offsets within the slot included) and the
+// sinogram offsets of the two rows (rb < 0: no partner).
+struct SlotConsts {
+  int loA, loB, ba, bb, ra, rb, unused0, unused1;
+};
+
 template <int KIND, int AC, int ST>
 struct StageMirror {
   using G = KindGeo<KIND>;
@@ -707,13 +691,13 @@ struct StageMirror {
   static constexpr size_t WIN = sizeof(uint32_t) * AC * L2;
   static constexpr size_t HALFBUF = WIN + sizeof(int) * AC * TAB;
   static constexpr size_t ANGS = 2 * HALFBUF;
-  static constexpr size_t BYTES = ANGS + 2 * sizeof(MirrorSlot) * AC;
+  static constexpr size_t BYTES = ANGS + 2 * sizeof(SlotConsts) * AC;
   static_assert(TR % 4 == 0 && TC % 4 == 0, "16-byte table reads");
+  static_assert(AC <= 64, "one lane of the first wave per slot");
   u16x2 v[EPT];  // the two samples of word j
   int t[TPT];    // SY (rows) and CX (columns) entries of this thread
-  int loA, loB;
-  double TA, TB;  // ray coordinates of the tile corner and of its mirror, slot being fetched
-  MirrorSlot sl, sl_next;  // slots of the pass being fetched and of the one after
+  SlotConsts k;  // constants of this thread's slot, pass being fetched
+  MirrorSlot raw;  // first wave: the slot of its lane, pass after next
   const unsigned char *base = nullptr;  // start of the LDS block
 
   // lattice position of row y / column x in the tables
@@ -722,35 +706,42 @@ struct StageMirror {
   }
   static __device__ __forceinline__ int col_index(int x) { return (x & 15) * TC + x / 16; }
 
-  __device__ __forceinline__ void load_angle(const Item &it, int a0, int na) {
-    const MirrorSlot *p = it.slots + a0 + min(static_cast<int>(threadIdx.x) / TPA, na - 1);
+  // First wave, lane l: slot a0 + min(l, na - 1).
+  __device__ __forceinline__ void load_raw(const Item &it, int a0, int na) {
+    const MirrorSlot *p = it.slots + a0 + min(static_cast<int>(threadIdx.x), na - 1);
     const double *d = reinterpret_cast<const double *>(p);
     const int *r = reinterpret_cast<const int *>(p) + 4;
-    sl_next.s = ldg(d, 0);
-    sl_next.c = ldg(d, 1);
-    sl_next.row_a = ldg(r, 0);
-    sl_next.row_b = ldg(r, 1);
+    raw.s = ldg(d, 0);
+    raw.c = ldg(d, 1);
+    raw.row_a = ldg(r, 0);
+    raw.row_b = ldg(r, 1);
   }
-  __device__ __forceinline__ void put_angle(unsigned char *lds, int slot, int na) const {
-    const int tid = threadIdx.x, a = tid / TPA;
-    if (a < na && tid - a * TPA == 0) reinterpret_cast<MirrorSlot *>(lds + ANGS)[slot * AC + a] = sl_next;
-  }
-  __device__ __forceinline__ void get_angle(const unsigned char *lds, int slot, int na) {
-    sl = reinterpret_cast<const MirrorSlot *>(lds + ANGS)[slot * AC + min(static_cast<int>(threadIdx.x) / TPA, na - 1)];
-  }
-  __device__ __forceinline__ void take_loaded() { sl = sl_next; }
-
-  // ray coordinates of pixel (ty0, tx0) and of its mirror at angle row_a
-  __device__ __forceinline__ void corner(const Item &it, int ty0, int tx0, double &TA, double &TB) const {
-    const int hp = it.w / 2;
-    const double cd = static_cast<double>(it.center), s = sl.s, co = sl.c;
+  // First wave: the constants of its lanes' slots for the tile at (ty0, tx0)
+  // (the mirrored tile's columns are 2hp - x), formed in FP64, into LDS
+  // slot area `par`. The other waves do no FP64 work.
+  __device__ __forceinline__ void derive(const Item &it, unsigned char *lds, int par, int ty0, int tx0) const {
+    const int l = threadIdx.x;
+    const int w = it.w, hp = w / 2;
+    const double cd = static_cast<double>(it.center), s = raw.s, co = raw.c;
     const double rowterm = cd + hp * ((1.0 - s) - co) + s * (ty0 - cd);
-    TA = rowterm + co * tx0;
-    TB = rowterm + co * (2 * hp - tx0);
+    const double TA = rowterm + co * tx0;             // pixel (ty0, tx0)
+    const double TB = rowterm + co * (2 * hp - tx0);  // its mirror
+    const double hy = s * (0.5 * (RT - 1)), hx = co * (0.5 * (CT - 1));
+    SlotConsts c;
+    c.loA = static_cast<int>(floor(TA + hy + hx)) - G::HALF - 2;
+    c.loB = static_cast<int>(floor(TB + hy - hx)) - G::HALF - 2;
+    c.ba = static_cast<int>(rint((TA - c.loA) * 65536.0));
+    c.bb = static_cast<int>(rint((TB - c.loB) * 65536.0));
+    c.ra = it.pad + raw.row_a * w;
+    c.rb = raw.row_b >= 0 ? it.pad + raw.row_b * w : -1;
+    c.unused0 = c.unused1 = 0;
+    if (l < AC) reinterpret_cast<SlotConsts *>(lds + ANGS)[par * AC + l] = c;
+  }
+  __device__ __forceinline__ void get(const unsigned char *lds, int par, int na) {
+    k = reinterpret_cast<const SlotConsts *>(lds + ANGS)[par * AC + min(static_cast<int>(threadIdx.x) / TPA, na - 1)];
   }
 
-  // tx0: first column of the tile; the mirrored tile's columns are 2hp - x.
-  __device__ __forceinline__ void fetch(const Item &it, int a0, int na, int ty0, int tx0) {
+  __device__ __forceinline__ void fetch(const Item &it, int a0, int na) {
     const int tid = threadIdx.x, sub = tid - (tid / TPA) * TPA;
     const int *st = it.slot_tab + (a0 + min(tid / TPA, na - 1)) * kSlotTab;
 #pragma unroll
@@ -758,30 +749,20 @@ struct StageMirror {
       const int e = min(sub + j * TPA, RT + CT - 1);
       t[j] = ldg32(st, static_cast<unsigned>(e < RT ? e : kSlotTab / 2 + (e - RT)));
     }
-    const int w = it.w;
-    corner(it, ty0, tx0, TA, TB);
-    const double hy = sl.s * (0.5 * (RT - 1)), hx = sl.c * (0.5 * (CT - 1));
-    loA = static_cast<int>(floor(TA + hy + hx)) - G::HALF - 2;
-    loB = static_cast<int>(floor(TB + hy - hx)) - G::HALF - 2;
     // offsets from the start of the zero pad before the sinogram; a slot
     // without a partner reads its second samples from the pad's first element
     const uint16_t *__restrict__ base16 = static_cast<const uint16_t *>(it.filt) - it.pad;
-    const int ra = it.pad + sl.row_a * w;
-    const int rb = it.pad + sl.row_b * w;
-    const int mb = sl.row_b >= 0 ? -1 : 0;
+    const int mb = ~(k.rb >> 31);
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;  // word e of the slot: [0, L) window A, [L, 2L) window B
-      const int k = trunc_pos(e < L ? loA + e : loB + (e - L));
-      v[j].x = ldg32(base16, static_cast<unsigned>(ra + k));
-      v[j].y = ldg32(base16, static_cast<unsigned>((rb + k) & mb));
+      const int kk = trunc_pos(e < L ? k.loA + e : k.loB + (e - L));
+      v[j].x = ldg32(base16, static_cast<unsigned>(k.ra + kk));
+      v[j].y = ldg32(base16, static_cast<unsigned>((k.rb + kk) & mb));
     }
   }
 
-  __device__ __forceinline__ void store(const Item &it, unsigned char *buf, int na, int ty0, int tx0) const {
-    (void)it;
-    (void)ty0;
-    (void)tx0;
+  __device__ __forceinline__ void store(unsigned char *buf, int na) const {
     const int tid = threadIdx.x, a = tid / TPA, sub = tid - a * TPA;
     if (a >= na) return;
     uint32_t *win = reinterpret_cast<uint32_t *>(buf);
@@ -793,8 +774,7 @@ struct StageMirror {
     }
     // word addresses are taken from the start of the LDS block
     const int off = static_cast<int>((buf - base) / sizeof(uint32_t)) + a * L2;
-    const int ba = static_cast<int>(rint((TA - loA) * 65536.0)) + (off << 16);
-    const int bb = static_cast<int>(rint((TB - loB) * 65536.0)) + ((off + L) << 16);
+    const int ba = k.ba + (off << 16), bb = k.bb + ((off + L) << 16);
     int *tab = reinterpret_cast<int *>(buf + WIN) + a * TAB;
 #pragma unroll
     for (int j = 0; j < TPT; ++j) {
@@ -809,32 +789,40 @@ struct StageMirror {
   }
 };
 
-// Pipeline of the mirrored passes (as run_passes_fx; the stores also form
-// the coordinate tables, which need the tile's corner).
+// Pipeline of the mirrored passes: fetch(p+1) / compute(p) / store(p+1) /
+// barrier, as run_passes_fx; the first wave also loads the slots of pass
+// p+2 and derives their constants, which reach every thread through LDS.
 template <class Stage, class Compute>
 __device__ __forceinline__ void run_passes_mirror(const Item &it, unsigned char *lds, int AC, int ty0, int tx0,
                                                   bool wskip, Compute compute) {
   Stage st;
   st.base = lds;
+  const bool first = threadIdx.x < 64;
   const int N = it.n_slots;
   const int P = (N + AC - 1) / AC;
-  st.load_angle(it, 0, min(AC, N));
-  st.take_loaded();
-  st.fetch(it, 0, min(AC, N), ty0, tx0);
-  st.load_angle(it, min(1, P - 1) * AC, min(AC, N - min(1, P - 1) * AC));
-  st.store(it, lds, min(AC, N), ty0, tx0);
-  st.put_angle(lds, 1, min(AC, N - min(1, P - 1) * AC));
+  auto count = [&](int p) { return min(AC, N - p * AC); };
+  if (first) {
+    st.load_raw(it, 0, count(0));
+    st.derive(it, lds, 0, ty0, tx0);
+    const int p1 = min(1, P - 1);
+    st.load_raw(it, p1 * AC, count(p1));
+    st.derive(it, lds, 1, ty0, tx0);
+  }
+  __syncthreads();
+  st.get(lds, 0, count(0));
+  st.fetch(it, 0, count(0));
+  st.store(lds, count(0));
   __syncthreads();
   for (int p = 0; p < P; ++p) {
-    const int a0 = p * AC, na = min(AC, N - a0);
+    const int na = count(p);
     const int p1 = min(p + 1, P - 1), p2 = min(p + 2, P - 1);
-    const int na1 = min(AC, N - p1 * AC), na2 = min(AC, N - p2 * AC);
-    st.get_angle(lds, (p + 1) & 1, na1);
-    st.fetch(it, p1 * AC, na1, ty0, tx0);
-    st.load_angle(it, p2 * AC, na2);
+    const int na1 = count(p1);
+    st.get(lds, (p + 1) & 1, na1);
+    st.fetch(it, p1 * AC, na1);
+    if (first) st.load_raw(it, p2 * AC, count(p2));
     if (!wskip) compute(lds + (p & 1) * Stage::HALFBUF, na);
-    st.store(it, lds + ((p + 1) & 1) * Stage::HALFBUF, na1, ty0, tx0);
-    st.put_angle(lds, p & 1, na2);
+    st.store(lds + ((p + 1) & 1) * Stage::HALFBUF, na1);
+    if (first) st.derive(it, lds, p & 1, ty0, tx0);
     __syncthreads();
   }
 }
@@ -904,10 +892,14 @@ __device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigne
         }
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
-          acc[r][q] = add_sample<ST, 0>(acc[r][q], WA[q]);
-          acc[r][q] = add_sample<ST, 1>(acc[r][q], WB[q]);
-          accm[r][q] = add_sample<ST, 1>(accm[r][q], WA[q]);
-          accm[r][q] = add_sample<ST, 0>(accm[r][q], WB[q]);
+          // the pixel's two samples (lo WA, hi WB) and the mirror pixel's
+          // (lo WB, hi WA), merged by bit-field inserts (the asm keeps them
+          // 32-bit merges) and each added by one dot2
+          uint32_t X = (WA[q] & 0xFFFFu) | (WB[q] & 0xFFFF0000u);
+          uint32_t Y = (WB[q] & 0xFFFFu) | (WA[q] & 0xFFFF0000u);
+          asm("" : "+v"(X), "+v"(Y));
+          acc[r][q] = add_pair<ST>(acc[r][q], X);
+          accm[r][q] = add_pair<ST>(accm[r][q], Y);
         }
       }
     }
