@@ -236,7 +236,7 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   if (int rc = upload_items(h, items, s, &changed, extra(h).resident)) return rc;
   if (pad) CTAS_HIP_TRY(launch_pad_zero(h->d_items, S, s));
   if (h->contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(h->d_items, S, s));
-  CTAS_HIP_TRY(launch_filter(h->d_items, S, pairs * S, h->logZ, h->contrast, st, s));
+  CTAS_HIP_TRY(launch_filter(h->d_items, S, pairs * S, h->logZ, h->logZ, h->contrast, st, s));
   if (do_bp)
     CTAS_HIP_TRY(launch_backproject(h->d_items, S, items[0].tiles * S, kind, choose_ac(h),
                                     mode_of(h->flags), st, mirror, s));
@@ -828,7 +828,7 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   const int st = store_of(flags), mode = mode_of(flags);
   std::vector<Item> its(n);
   size_t filt_bytes = 0;
-  int max_logZ = 0, pairs = 0, tiles = 0;
+  int max_logZ = 0, min_logZ = 99, pairs = 0, tiles = 0;
   // Work order: costliest sinograms first, so that the persistent launch
   // ends with the small tiles.
   std::vector<int> order(n);
@@ -860,6 +860,7 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
     it.Z = zidth(bi.w);
     while ((1 << it.logZ) < it.Z) ++it.logZ;
     max_logZ = std::max(max_logZ, it.logZ);
+    min_logZ = std::min(min_logZ, it.logZ);
     int kind;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -892,7 +893,7 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   }
   CTAS_HIP_TRY(hipMemcpy(bw.d_items, its.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
   if (contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(bw.d_items, n, s));
-  CTAS_HIP_TRY(launch_filter(bw.d_items, n, pairs, max_logZ, contrast, st, s));
+  CTAS_HIP_TRY(launch_filter(bw.d_items, n, pairs, min_logZ, max_logZ, contrast, st, s));
   CTAS_HIP_TRY(launch_backproject_persistent(bw.d_items, n, tiles, mode, st, bw.d_counter,
                                              persistent_workgroups(), s));
   CTAS_HIP_TRY(hipStreamSynchronize(s));

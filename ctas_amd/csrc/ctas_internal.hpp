@@ -119,7 +119,8 @@ hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s);
 // Zeros the pads around the filtered sinograms of the items (Item::pad).
 hipError_t launch_pad_zero(const Item *d_items, int n_items, hipStream_t s);
 // deAbs + padded FFT filtering + REF prefix sum for all row pairs.
-hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int max_logZ,
+// min_logZ == max_logZ == 12 (every line 4096 points) takes the radix-16 kernel.
+hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int min_logZ, int max_logZ,
                          int contrast, int store, hipStream_t s);
 // Backprojection of all tiles of all items (grid-mapped).
 // mirror: FP16/BF16 storage with CTAS sampling, items carry mirror slots and

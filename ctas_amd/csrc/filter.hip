@@ -297,6 +297,248 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
   }
 }
 
+// ---- Z = 4096: radix-16 passes in registers --------------------------------
+// The 4096-point line is transformed as 16 x 16 x 16: thread t holds the 16
+// points t + 256 j, takes a 16-point DFT in registers, multiplies by
+// w^(t k1) (w = exp(-2 pi i / 4096)), and two more such passes follow, each
+// after an exchange of the values through LDS (rows padded to 17 entries).
+// The thread (k1, k2) then holds X[k1 + 16 k2 + 256 k3], k3 = 0..15. After
+// the window multiply the inverse runs the same steps backwards with
+// conjugate factors, so that thread t ends with the filtered points
+// t + 256 j in natural order: loads and stores of the rows are coalesced and
+// no index is bit-reversed.
+constexpr float kCos16[16] = {1.0f, 0.92387953f, 0.70710678f, 0.38268343f, 0.0f, -0.38268343f, -0.70710678f,
+                              -0.92387953f, -1.0f, -0.92387953f, -0.70710678f, -0.38268343f, 0.0f, 0.38268343f,
+                              0.70710678f, 0.92387953f};
+constexpr float kSin16[16] = {0.0f, 0.38268343f, 0.70710678f, 0.92387953f, 1.0f, 0.92387953f, 0.70710678f,
+                              0.38268343f, 0.0f, -0.38268343f, -0.70710678f, -0.92387953f, -1.0f, -0.92387953f,
+                              -0.70710678f, -0.38268343f};
+
+// 4-point DFT, exp(-2 pi i/4) per step (forward) or its conjugate (INV).
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &x0, float2 &x1, float2 &x2, float2 &x3) {
+  const float2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3), a3 = csub(x1, x3);
+  x0 = cadd(a0, a2);
+  x2 = csub(a0, a2);
+  if (INV) {
+    x1 = make_float2(a1.x - a3.y, a1.y + a3.x);
+    x3 = make_float2(a1.x + a3.y, a1.y - a3.x);
+  } else {
+    x1 = make_float2(a1.x + a3.y, a1.y - a3.x);
+    x3 = make_float2(a1.x - a3.y, a1.y + a3.x);
+  }
+}
+
+// 16-point DFT in registers, natural order in and out: j = 4a + b,
+// k = c + 4d; inner DFT4 over a, factor exp(-+2 pi i b c / 16), outer DFT4
+// over b.
+template <bool INV>
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) dft4<INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);  // v[b + 4c] = u_b[c]
+#pragma unroll
+  for (int b = 1; b < 4; ++b)
+#pragma unroll
+    for (int c = 1; c < 4; ++c) {
+      const float co = kCos16[b * c], si = INV ? kSin16[b * c] : -kSin16[b * c];
+      v[b + 4 * c] = cmul(v[b + 4 * c], make_float2(co, si));
+    }
+  float2 r[16];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float2 y0 = v[4 * c], y1 = v[4 * c + 1], y2 = v[4 * c + 2], y3 = v[4 * c + 3];
+    dft4<INV>(y0, y1, y2, y3);  // over b -> d
+    r[c] = y0;
+    r[c + 4] = y1;
+    r[c + 8] = y2;
+    r[c + 12] = y3;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = r[k];
+}
+
+// exp(-+2 pi i m / 4096) from the half-turn table tw[m] (m < 2048).
+template <bool INV>
+__device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
+  m &= 4095;
+  float2 t = tw[m & 2047];
+  if (m & 2048) t = make_float2(-t.x, -t.y);
+  return INV ? make_float2(t.x, -t.y) : t;
+}
+
+// v[k] *= r^k, k = 1..15, with r^2, r^4, r^8 from squarings and the other
+// powers as products of those (at most four factors, a few float ulps).
+__device__ __forceinline__ void twiddle_powers(float2 (&v)[16], float2 r) {
+  float2 p[16];
+  p[1] = r;
+  p[2] = cmul(r, r);
+  p[4] = cmul(p[2], p[2]);
+  p[8] = cmul(p[4], p[4]);
+  p[3] = cmul(p[2], r);
+  p[5] = cmul(p[4], r);
+  p[6] = cmul(p[4], p[2]);
+  p[7] = cmul(p[4], p[3]);
+  p[9] = cmul(p[8], r);
+  p[10] = cmul(p[8], p[2]);
+  p[11] = cmul(p[8], p[3]);
+  p[12] = cmul(p[8], p[4]);
+  p[13] = cmul(p[8], p[5]);
+  p[14] = cmul(p[8], p[6]);
+  p[15] = cmul(p[8], p[7]);
+#pragma unroll
+  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], p[k]);
+}
+
+template <int ST>
+__global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ items, int n_items,
+                                                       int contrast) {
+  constexpr int Z = 4096, LOGZ = 12, PADR = 17;
+  __shared__ float2 xch[256 * PADR];
+  const int b = blockIdx.x;
+  const Item &it = items[find_item_by_pair(items, n_items, b)];
+  const int pair = b - it.pair_base;
+  const int w = it.w, N = it.N;
+  const int s0 = (Z - w) >> 1;
+  const int t0 = 2 * pair, t1 = t0 + 1;
+  const bool has1 = t1 < N;
+  const float *__restrict__ r0 = it.sino + static_cast<size_t>(t0) * w;
+  const float *__restrict__ r1 = r0 + w;
+  const float2 *__restrict__ tw = it.tw;
+  const int tid = threadIdx.x;
+
+  // deAbs (physical.world.cpp:88-100), as in filter_kernel.
+  const bool absorb = (contrast == CTAS_CONTRAST_ABS);
+  bool clamp = false;
+  float mina = 0.0f;
+  if (absorb) {
+    const float2 m = it.minmax[tid & 63];
+    float mn = m.x, mx = m.y;
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, o));
+      mx = fmaxf(mx, __shfl_xor(mx, o));
+    }
+    clamp = (mn <= 0.0f);
+    mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
+    if (pair == 0 && tid == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
+  }
+  auto deabs = [&](float v) {
+    if (clamp && v <= mina) v = mina;
+    return -logf(v);
+  };
+
+  // Padded line z[n]: row samples at s0..s0+w-1 and the next row's first
+  // sample at s0+w (lct.cpp:580-584; 0 after the last row).
+  float2 v[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int x = tid + 256 * j - s0;
+    float a = 0.0f, c = 0.0f;
+    if (x >= 0 && x < w) {
+      a = r0[x];
+      if (absorb) a = deabs(a);
+      if (has1) {
+        c = r1[x];
+        if (absorb) c = deabs(c);
+      }
+    } else if (x == w) {
+      if (t0 + 1 < N) {
+        a = r0[w];
+        if (absorb) a = deabs(a);
+      }
+      if (has1 && t1 + 1 < N) {
+        c = r1[w];
+        if (absorb) c = deabs(c);
+      }
+    }
+    v[j] = make_float2(a, c);
+  }
+
+  // forward: thread t -> (t1, k1) -> (k1, k2)
+  const int lo4 = tid & 15, hi4 = tid >> 4;
+  dft16<false>(v);
+  twiddle_powers(v, w4096<false>(tw, tid));
+#pragma unroll
+  for (int k = 0; k < 16; ++k) xch[tid * PADR + k] = v[k];
+  __syncthreads();
+  // thread (t1 = lo4, k1 = hi4): values over t2
+#pragma unroll
+  for (int j = 0; j < 16; ++j) v[j] = xch[(lo4 + 16 * j) * PADR + hi4];
+  __syncthreads();
+  dft16<false>(v);
+  twiddle_powers(v, w4096<false>(tw, 16 * lo4));
+#pragma unroll
+  for (int k = 0; k < 16; ++k) xch[(hi4 * 16 + k) * PADR + lo4] = v[k];
+  __syncthreads();
+  // thread (k1 = hi4, k2 = lo4): values over t1
+#pragma unroll
+  for (int j = 0; j < 16; ++j) v[j] = xch[tid * PADR + j];
+  dft16<false>(v);
+
+  // X[k1 + 16 k2 + 256 k3] times the window
+  const int kbase = hi4 + 16 * lo4;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const float wk = it.W[kbase + 256 * k];
+    v[k].x *= wk;
+    v[k].y *= wk;
+  }
+
+  // inverse: the same steps backwards, conjugate factors
+  dft16<true>(v);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) xch[tid * PADR + j] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = xch[(hi4 * 16 + k) * PADR + lo4];
+  __syncthreads();
+  twiddle_powers(v, w4096<true>(tw, 16 * lo4));
+  dft16<true>(v);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) xch[(lo4 + 16 * j) * PADR + hi4] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = xch[tid * PADR + k];
+  twiddle_powers(v, w4096<true>(tw, tid));
+  dft16<true>(v);
+
+  const float invZ = 1.0f / static_cast<float>(Z);
+  const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
+  if (contrast != CTAS_CONTRAST_REF) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int x = tid + 256 * j - s0;
+      if (x < 0 || x >= w) continue;
+      const float a = v[j].x * invZ, c = v[j].y * invZ;
+      store_val<ST>(it.filt, row0 + x, a);
+      if (it.filt_f32) it.filt_f32[row0 + x] = a;
+      if (has1) {
+        store_val<ST>(it.filt, row1 + x, c);
+        if (it.filt_f32) it.filt_f32[row1 + x] = c;
+      }
+    }
+    return;
+  }
+  // REF (lct.cpp:590-595): the line in natural order through LDS, then the
+  // prefix sum along each row, one lane per row, in sequential order.
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) xch[tid + 256 * j] = v[j];
+  __syncthreads();
+  if (tid == 0 || (tid == 64 && has1)) {
+    const bool im = tid == 64;
+    const size_t rowb = im ? row1 : row0;
+    float run = 0.0f;
+    for (int x = 0; x < w; ++x) {
+      const float2 q = xch[s0 + x];
+      const float f = (im ? q.y : q.x) * invZ;
+      run = (x == 0) ? f : run + f;
+      store_val<ST>(it.filt, rowb + x, run);
+      if (it.filt_f32) it.filt_f32[rowb + x] = run;
+    }
+  }
+  (void)LOGZ;
+}
+
 // FLT contrast: no filtering, only the storage conversion (and float copy).
 template <int ST>
 __global__ __launch_bounds__(256) void passthrough_kernel(const Item *__restrict__ items,
@@ -340,7 +582,7 @@ hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int max_logZ,
+hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int min_logZ, int max_logZ,
                          int contrast, int store, hipStream_t s) {
   if (total_pairs <= 0) return hipSuccess;
   if (contrast == CTAS_CONTRAST_FLT) {
@@ -348,6 +590,14 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
       case ST_F32: hipLaunchKernelGGL(passthrough_kernel<ST_F32>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items); break;
       case ST_F16: hipLaunchKernelGGL(passthrough_kernel<ST_F16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items); break;
       default: hipLaunchKernelGGL(passthrough_kernel<ST_BF16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items); break;
+    }
+    return hipGetLastError();
+  }
+  if (min_logZ == 12 && max_logZ == 12) {
+    switch (store) {
+      case ST_F32: hipLaunchKernelGGL(filter16_kernel<ST_F32>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
+      case ST_F16: hipLaunchKernelGGL(filter16_kernel<ST_F16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
+      default: hipLaunchKernelGGL(filter16_kernel<ST_BF16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
     }
     return hipGetLastError();
   }

@@ -68,6 +68,24 @@ def test_prepare_matches_oracle(contrast, filt):
     assert np.abs(got - ref).max() <= 2e-5 * scale
 
 
+@pytest.mark.parametrize("w", [1100, 2048])
+@pytest.mark.parametrize("contrast,filt", [(ca.ABS, "HANN"), (ca.PHS, "RAMP"), (ca.REF, "BLACKMAN")])
+@pytest.mark.parametrize("store", [ca.STORE_FP32, ca.STORE_BF16])
+def test_prepare_matches_oracle_4096_point_lines(w, contrast, filt, store):
+    """Widths 1025-2048 pad to 4096 points, the radix-16 filter kernel; an
+    odd number of rows leaves the last line with one row."""
+    N = 9
+    P = _phantom(N, w, n_ell=6, seed=w)
+    if contrast != ca.ABS:
+        P = -np.log(np.maximum(P, 1e-3)).astype(np.float32)
+    rec = ca.CTrec(w, N, contrast, filt, store=store)
+    got = rec.prepare(P)
+    f, p = ca.parse_filter(filt)
+    ref = oracle.prepare(P, oracle.filter_window(f, p, w, contrast), contrast)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 2e-5 * scale
+
+
 def test_config1_end_to_end():
     """512 px x 720 angles, 10 ellipses, Poisson 1e5, RAMP, c = 0 (config 1)."""
     N, w = 720, 512
