@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
   }
   auto deabs = [&](float v) {
     if (clamp && v <= mina) v = mina;
-    return -logf(v);
+    return -__logf(v);
   };
 
   // Padded line z[n], n in [0, Z): row samples at s0..s0+w-1 and, from the
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   }
   auto deabs = [&](float v) {
     if (clamp && v <= mina) v = mina;
-    return -logf(v);
+    return -__logf(v);
   };
 
   // Padded line z[n]: row samples at s0..s0+w-1 and the next row's first
