@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # for a 1 GiB copy (profiles/fbp_fp32_ctas_2048x1800x256.md). PMC counters
 # cannot be read from inside this process, so the figure is quoted for that
 # configuration only.
-PMC_TRAFFIC = {(256, 2048, 1800, 1, 32): 79.2e9}
+PMC_TRAFFIC = {(256, 2048, 1800, 1, 32): 79.4e9}
 
 
 def measured_traffic(S, w, N, kind, ac):
