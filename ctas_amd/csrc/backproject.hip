@@ -693,22 +693,16 @@ struct StageMirror {
   static constexpr size_t ANGS = 2 * HALFBUF;
   static constexpr size_t BYTES = ANGS + 2 * sizeof(SlotConsts) * AC;
   static_assert(TR % 4 == 0 && TC % 4 == 0, "16-byte table reads");
-  static_assert(AC <= 64, "one lane of the first wave per slot");
+  static_assert(AC <= 64, "one lane of the deriving wave per slot");
   u16x2 v[EPT];  // the two samples of word j
   int t[TPT];    // SY (rows) and CX (columns) entries of this thread
   SlotConsts k;  // constants of this thread's slot, pass being fetched
-  MirrorSlot raw;  // first wave: the slot of its lane, pass after next
+  MirrorSlot raw;  // deriving wave: the slot of its lane, pass after next
   const unsigned char *base = nullptr;  // start of the LDS block
 
-  // lattice position of row y / column x in the tables
-  static __device__ __forceinline__ int row_index(int y) {
-    return ((y / (4 * TR)) * 4 + (y & 3)) * TR + (y % (4 * TR)) / 4;
-  }
-  static __device__ __forceinline__ int col_index(int x) { return (x & 15) * TC + x / 16; }
-
-  // First wave, lane l: slot a0 + min(l, na - 1).
+  // Deriving wave, lane l: slot a0 + min(l, na - 1).
   __device__ __forceinline__ void load_raw(const Item &it, int a0, int na) {
-    const MirrorSlot *p = it.slots + a0 + min(static_cast<int>(threadIdx.x), na - 1);
+    const MirrorSlot *p = it.slots + a0 + min(static_cast<int>(threadIdx.x & 63), na - 1);
     const double *d = reinterpret_cast<const double *>(p);
     const int *r = reinterpret_cast<const int *>(p) + 4;
     raw.s = ldg(d, 0);
@@ -716,11 +710,11 @@ struct StageMirror {
     raw.row_a = ldg(r, 0);
     raw.row_b = ldg(r, 1);
   }
-  // First wave: the constants of its lanes' slots for the tile at (ty0, tx0)
-  // (the mirrored tile's columns are 2hp - x), formed in FP64, into LDS
-  // slot area `par`. The other waves do no FP64 work.
+  // Deriving wave: the constants of its lanes' slots for the tile at
+  // (ty0, tx0) (the mirrored tile's columns are 2hp - x), formed in FP64,
+  // into LDS slot area `par`. The other waves do no FP64 work.
   __device__ __forceinline__ void derive(const Item &it, unsigned char *lds, int par, int ty0, int tx0) const {
-    const int l = threadIdx.x;
+    const int l = threadIdx.x & 63;
     const int w = it.w, hp = w / 2;
     const double cd = static_cast<double>(it.center), s = raw.s, co = raw.c;
     const double rowterm = cd + hp * ((1.0 - s) - co) + s * (ty0 - cd);
@@ -743,12 +737,11 @@ struct StageMirror {
 
   __device__ __forceinline__ void fetch(const Item &it, int a0, int na) {
     const int tid = threadIdx.x, sub = tid - (tid / TPA) * TPA;
-    const int *st = it.slot_tab + (a0 + min(tid / TPA, na - 1)) * kSlotTab;
+    // the slot's row terms (lattice order of this tile kind), then its
+    // column terms (lattice order)
+    const int *st = it.slot_tab + (a0 + min(tid / TPA, na - 1)) * kSlotTab + lattice_offset(KIND);
 #pragma unroll
-    for (int j = 0; j < TPT; ++j) {
-      const int e = min(sub + j * TPA, RT + CT - 1);
-      t[j] = ldg32(st, static_cast<unsigned>(e < RT ? e : kSlotTab / 2 + (e - RT)));
-    }
+    for (int j = 0; j < TPT; ++j) t[j] = ldg32(st, static_cast<unsigned>(min(sub + j * TPA, RT + CT - 1)));
     // offsets from the start of the zero pad before the sinogram; a slot
     // without a partner reads its second samples from the pad's first element
     const uint16_t *__restrict__ base16 = static_cast<const uint16_t *>(it.filt) - it.pad;
@@ -780,18 +773,18 @@ struct StageMirror {
     for (int j = 0; j < TPT; ++j) {
       const int e = sub + j * TPA;
       if (e < RT) {
-        tab[row_index(e)] = ba + t[j];
-        tab[RT + row_index(e)] = bb + t[j];
+        tab[e] = ba + t[j];
+        tab[RT + e] = bb + t[j];
       } else if (e < RT + CT) {
-        tab[2 * RT + col_index(e - RT)] = t[j];
+        tab[RT + e] = t[j];
       }
     }
   }
 };
 
 // Pipeline of the mirrored passes: fetch(p+1) / compute(p) / store(p+1) /
-// barrier, as run_passes_fx; the first wave also loads the slots of pass
-// p+2 and derives their constants, which reach every thread through LDS.
+// barrier, as run_passes_fx; one wave (in turn) also loads the slots of
+// pass p+2 and derives their constants, which reach every thread through LDS.
 template <class Stage, class Compute>
 __device__ __forceinline__ void run_passes_mirror(const Item &it, unsigned char *lds, int AC, int ty0, int tx0,
                                                   bool wskip, Compute compute) {
@@ -817,12 +810,14 @@ __device__ __forceinline__ void run_passes_mirror(const Item &it, unsigned char 
     const int na = count(p);
     const int p1 = min(p + 1, P - 1), p2 = min(p + 2, P - 1);
     const int na1 = count(p1);
+    // the four waves take turns at deriving, so that no wave waits longest
+    const bool deriver = (static_cast<int>(threadIdx.x) >> 6) == (p & 3);
     st.get(lds, (p + 1) & 1, na1);
     st.fetch(it, p1 * AC, na1);
-    if (first) st.load_raw(it, p2 * AC, count(p2));
+    if (deriver) st.load_raw(it, p2 * AC, count(p2));
     if (!wskip) compute(lds + (p & 1) * Stage::HALFBUF, na);
     st.store(lds + ((p + 1) & 1) * Stage::HALFBUF, na1);
-    if (first) st.derive(it, lds, p & 1, ty0, tx0);
+    if (deriver) st.derive(it, lds, p & 1, ty0, tx0);
     __syncthreads();
   }
 }

@@ -136,9 +136,11 @@ int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
     return fail(CTAS_E_NOMEM, "device allocation of the mirror slot tables failed");
   std::vector<int> tab(static_cast<size_t>(kSlotTab) * slots.size());
   for (size_t i = 0; i < slots.size(); ++i)
-    for (int k = 0; k < kSlotTab / 2; ++k) {
-      tab[i * kSlotTab + k] = static_cast<int>(std::lrint(slots[i].s * k * 65536.0));
-      tab[i * kSlotTab + kSlotTab / 2 + k] = static_cast<int>(std::lrint(slots[i].c * k * 65536.0));
+    for (int kind = TK_64x64; kind <= TK_128x128; ++kind) {
+      const int RT = tile_rows(kind), CT = tile_cols(kind), TR = RT / 16, TC = CT / 16;
+      int *t = tab.data() + i * kSlotTab + lattice_offset(kind);
+      for (int y = 0; y < RT; ++y) t[lattice_row(y, TR)] = static_cast<int>(std::lrint(slots[i].s * y * 65536.0));
+      for (int x = 0; x < CT; ++x) t[RT + lattice_col(x, TC)] = static_cast<int>(std::lrint(slots[i].c * x * 65536.0));
     }
   CTAS_HIP_TRY(hipMemcpy(h->d_slots, slots.data(), sizeof(MirrorSlot) * slots.size(), hipMemcpyHostToDevice));
   CTAS_HIP_TRY(hipMemcpy(h->d_slot_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));

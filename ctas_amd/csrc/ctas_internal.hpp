@@ -53,9 +53,20 @@ struct MirrorSlot {
   int row_a, row_b;
   int pad[2];
 };
-// Per slot, kSlotTab ints: round(s y 2^16) for y < 128, then round(c x 2^16)
-// for x < 128 (the row and column terms of a tile's coordinates).
-constexpr int kSlotTab = 256;
+// Per slot and tile kind, the row terms round(s y 2^16) (y < rows of the
+// tile) and then the column terms round(c x 2^16) (x < columns), each in
+// the lattice order of the kind's lanes (lattice_row / lattice_col): the
+// backprojection copies them into LDS without index arithmetic. The three
+// kinds' sections lie back to back, kSlotTab ints per slot.
+constexpr int kSlotTab = (64 + 64) + (128 + 64) + (128 + 128);
+// Offset of a tile kind's section (TK_64x64 = 1, TK_128x64 = 2, TK_128x128 = 3).
+__host__ __device__ constexpr int lattice_offset(int kind) { return kind == 1 ? 0 : kind == 2 ? 128 : 320; }
+// Lane lattice of a tile with TR x TC pixels per lane (4x16-pixel blocks per
+// wave instruction): position of row y and of column x in the tables.
+__host__ __device__ constexpr int lattice_row(int y, int TR) {
+  return ((y / (4 * TR)) * 4 + (y & 3)) * TR + (y % (4 * TR)) / 4;
+}
+__host__ __device__ constexpr int lattice_col(int x, int TC) { return (x & 15) * TC + x / 16; }
 // Pairs first (ordered by row_a), then the angles without a partner.
 // Returns the number of pairs.
 int mirror_slots(const std::vector<double> &sc64, std::vector<MirrorSlot> &slots);
