@@ -684,7 +684,11 @@ struct SlotConsts {
 template <int KIND, int AC, int ST>
 struct StageMirror {
   using G = KindGeo<KIND>;
-  static constexpr int L = G::L, L2 = 2 * L, TPA = 256 / AC, EPT = (L2 + TPA - 1) / TPA;
+  // window of D + 6 positions from floor(centre) - ceil(D/2) - 1: the tile's
+  // coordinates lie within D/2 of the centre's, so one position of margin
+  // below and three above (for 64x64 tiles 2L = 192 words, six per thread)
+  static constexpr int L = G::D + 6, LO = (G::D + 1) / 2 + 1;
+  static constexpr int L2 = 2 * L, TPA = 256 / AC, EPT = (L2 + TPA - 1) / TPA;
   static constexpr int RT = G::RT, CT = G::CT, TR = G::TR, TC = G::TC;
   static constexpr int TAB = 2 * RT + CT;  // ints per slot: RA, RB, CX
   static constexpr int TPT = (RT + CT + TPA - 1) / TPA;  // SY and CX entries per thread
@@ -722,8 +726,8 @@ struct StageMirror {
     const double TB = rowterm + co * (2 * hp - tx0);  // its mirror
     const double hy = s * (0.5 * (RT - 1)), hx = co * (0.5 * (CT - 1));
     SlotConsts c;
-    c.loA = static_cast<int>(floor(TA + hy + hx)) - G::HALF - 2;
-    c.loB = static_cast<int>(floor(TB + hy - hx)) - G::HALF - 2;
+    c.loA = static_cast<int>(floor(TA + hy + hx)) - LO;
+    c.loB = static_cast<int>(floor(TB + hy - hx)) - LO;
     c.ba = static_cast<int>(rint((TA - c.loA) * 65536.0));
     c.bb = static_cast<int>(rint((TB - c.loB) * 65536.0));
     c.ra = it.pad + raw.row_a * w;
