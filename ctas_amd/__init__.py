@@ -101,6 +101,13 @@ def lib() -> C.CDLL:
         "ctas_fbp_get_tiles": (i, [vp, ip, ip]),
         "ctas_fbp_autotune": (i, [vp, i]),
         "ctas_fbp_batch_uneven": (i, [vp, i, i, i, f, f, i, vp]),
+        "ctas_filter_fill": (i, [i, f, i, fp]),
+        "ctas_ts_window": (i, [i, f, i, ip, fp]),
+        "ctas_ts_add": (i, [fp, i, i, fp, i, i, i, f, f, i, f, i]),
+        "ctas_ts_create": (i, [C.POINTER(vp), i, i, i, i, f]),
+        "ctas_ts_destroy": (i, [vp]),
+        "ctas_ts_info": (i, [vp, ip, ip, ip]),
+        "ctas_ts_add_dev": (i, [vp, vp, i, fp, f, i, vp, vp, vp]),
         "ctas_ring_filter": (i, [fp, i, i, i]),
         "ctas_ring_filter_dev": (i, [vp, i, i, i, i, vp]),
         "ctas_axis_scores": (i, [vp, fp, i, fp]),
@@ -364,6 +371,84 @@ def ring_filter(sino: np.ndarray, box: int) -> np.ndarray:
 def ring_filter_dev(d_sino, box: int, stream=None) -> None:
     S, N, w = (d_sino.shape if d_sino.dim() == 3 else (1,) + tuple(d_sino.shape))
     _check(lib().ctas_ring_filter_dev(_ptr(d_sino), S, N, w, box, _stream_ptr(stream)))
+
+
+def filter_fill(filt: int | str, param: float, n: int) -> np.ndarray:
+    """Filter::fill of an n-point line (lct.cpp:122-209), no gain."""
+    if isinstance(filt, str):
+        filt, param = parse_filter(filt)
+    W = np.zeros(n, np.float32)
+    _check(lib().ctas_filter_fill(filt, param, n, _f32p(W)))
+    return W
+
+
+def ts_window(filt: int, param: float, n: int) -> np.ndarray:
+    """ts_add's row-filter window as a transform of zidth(n) (or n) points."""
+    Z = C.c_int(0)
+    _check(lib().ctas_ts_window(filt, param, n, C.byref(Z), None))
+    W = np.zeros(Z.value, np.float32)
+    _check(lib().ctas_ts_window(filt, param, n, C.byref(Z), _f32p(W)))
+    return W
+
+
+def ts_add(projection: np.ndarray, result: np.ndarray, filt: int | str = 1, param: float = 0.0,
+           center: float = 0.0, contrast: int = ABS, angle: float = 0.0, plane: int = 0) -> None:
+    """ts_add (ct.h:317, lct.cpp:739-787) in place: ``projection`` becomes
+    its filtered rows and ``result`` accumulates the plane's contribution.
+    Both must be C-contiguous float32 arrays."""
+    if isinstance(filt, str):
+        filt, param = parse_filter(filt)
+    for a in (projection, result):
+        if a.dtype != np.float32 or not a.flags.c_contiguous or a.ndim != 2:
+            raise CtasError(CTAS_E_ARG, "ts_add needs C-contiguous 2-D float32 arrays")
+    _check(lib().ctas_ts_add(_f32p(projection), projection.shape[0], projection.shape[1], _f32p(result),
+                             result.shape[0], result.shape[1], filt, param, center, contrast, angle, plane))
+
+
+class TsPlan:
+    """Device plan of ts_add for projections of rows x pixels."""
+
+    def __init__(self, rows: int, pixels: int, contrast: int = ABS, filt: int | str = 1, param: float = 0.0):
+        if isinstance(filt, str):
+            filt, param = parse_filter(filt)
+        h = C.c_void_p()
+        _check(lib().ctas_ts_create(C.byref(h), rows, pixels, contrast, filt, param))
+        self._h = h
+        self.rows, self.pixels = rows, pixels
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().ctas_ts_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def zidth(self) -> int:
+        z = C.c_int(0)
+        _check(lib().ctas_ts_info(self._h, None, None, C.byref(z)))
+        return z.value
+
+    def add_dev(self, d_proj, angles, d_result, center: float = 0.0, plane: int = 0, d_filtered=None,
+                stream=None) -> None:
+        """Successive ts_add of the projections d_proj ([P][rows][pixels] or
+        [rows][pixels]) at angles (radians) into d_result."""
+        a = np.ascontiguousarray(np.atleast_1d(angles), np.float32)
+        for t in (d_proj, d_result, d_filtered):
+            if t is not None and str(t.dtype) != "torch.float32":
+                raise CtasError(CTAS_E_ARG, "ts_add needs float32 device arrays")
+        P = 1 if d_proj.dim() == 2 else d_proj.shape[0]
+        if a.size != P or tuple(d_proj.shape[-2:]) != (self.rows, self.pixels):
+            raise CtasError(CTAS_E_ARG, "projection batch does not match the plan")
+        if tuple(d_result.shape) != (self.rows, self.pixels):
+            raise CtasError(CTAS_E_ARG, "The size of the result array does not match the requested reconstruction.")
+        _check(lib().ctas_ts_add_dev(self._h, _ptr(d_proj), P, _f32p(a), center, plane,
+                                     _ptr(d_filtered) if d_filtered is not None else None, _ptr(d_result),
+                                     _current_stream(stream)))
 
 
 def raxis(sino: np.ndarray, anarc: float, max_dev: float = 0.25, algo: int = 0) -> float:

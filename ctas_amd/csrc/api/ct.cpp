@@ -87,11 +87,8 @@ std::string Filter::name() const {
 Line &Filter::fill(Line &filt, int pixels) const {
   if (!pixels) pixels = static_cast<int>(filt.size());
   if (!pixels) throw_error("filter", "Zero-size filter requested.");
-  // The engine's window carries the CTrec gain zidth/w for the width whose
-  // zidth is `pixels`; fill() reports it for a power-of-two `pixels` line.
-  const int w = pixels / 2;
-  filt.assign(static_cast<size_t>(ctas_zidth(w)), 0.0f);
-  check(ctas_filter_window(tp_, as_, w, CTAS_CONTRAST_ABS, filt.data()), "filter");
+  filt.assign(static_cast<size_t>(pixels), 0.0f);
+  check(ctas_filter_fill(tp_, as_, pixels, filt.data()), "filter");
   return filt;
 }
 
@@ -174,6 +171,16 @@ void RingFilter::apply(Map &sinogram) {
   check(ctas_ring_filter(sinogram.data(), static_cast<int>(sinogram.rows()), static_cast<int>(sinogram.columns()),
                          static_cast<int>(box_)),
         "RingFilter");
+}
+
+void ts_add(Map &projection, Map &result, const Filter &filter, const float center, const Contrast contrast,
+            const float angle, const int plane) {
+  if (result.columns() != projection.columns() || result.rows() != projection.rows())
+    throw_error("tomosynthesis", "The size of the result array does not match the requested reconstruction.");
+  check(ctas_ts_add(projection.data(), static_cast<int>(projection.rows()), static_cast<int>(projection.columns()),
+                    result.data(), static_cast<int>(result.rows()), static_cast<int>(result.columns()),
+                    filter.filter(), filter.param(), center, contrast.contrast(), angle, plane),
+        "tomosynthesis");
 }
 
 float raxis(Map &sino, float anarc, float maxDev, int algo) {

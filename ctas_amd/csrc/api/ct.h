@@ -89,7 +89,8 @@ class Filter {
   Ftype filter() const { return tp_; }
   float param() const { return as_; }
   std::string name() const;
-  // The frequency window of zidth points with the CTrec gain (ABS rules).
+  // Filter::fill (lct.cpp:122-209): the window of a `pixels`-point line
+  // (the line's own size when 0), without the CTrec gain.
   Line &fill(Line &filt, int pixels = 0) const;
 
  private:
@@ -155,6 +156,11 @@ class RingFilter {
  private:
   size_t width_, box_;
 };
+
+// ts_add (ct.h:317, lct.cpp:739-787): projection is replaced by its
+// filtered rows; result accumulates the projection resampled for the plane.
+void ts_add(Map &projection, Map &result, const Filter &filter, const float center, const Contrast contrast,
+            const float angle, const int plane);
 
 float raxis(Map &sino, float anarc, float maxDev, int algo = 0);
 float raxis(Map &proj0, Map &proj180, float maxDev);

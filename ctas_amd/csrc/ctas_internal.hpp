@@ -35,6 +35,11 @@ inline bool use_fp64_angles(int flags) {
   return (flags & CTAS_INTERP_MASK) != CTAS_INTERP_CTAS || (flags & CTAS_STORE_MASK) != CTAS_STORE_FP32;
 }
 int filter_window(int filter, float param, int w, int contrast, std::vector<float> &W);
+// Filter::fill (lct.cpp:122-209): the window of an n-point line, no gain.
+int filter_fill(int filter, float param, int n, std::vector<float> &W);
+// The window of ts_add's row filter as a transform of Z points (Z = n for a
+// power of two n, else zidth(n)), inverse scaled by 1/Z.
+int ts_window(int filter, float param, int n, int &Z, std::vector<float> &W);
 
 // improveMe (lct.cpp:1108-1129): bisection from the bracket around vax[idx]
 // down to eps; f evaluates the score of one trial offset and sets *rc on
@@ -144,6 +149,18 @@ hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int t
                                          int n_workgroups, hipStream_t s);
 // Ring filter (RingFilter::apply) on S sinograms in place.
 hipError_t launch_ring(float *d_sino, float *d_avg, int S, int N, int w, int box, hipStream_t s);
+
+// Tomosynthesis (ts.hip): per projection of a launch {cos a, P, R, 0} of
+// ts_add's column map, passed by value; projections beyond kTsBatch are
+// split over launches.
+constexpr int kTsBatch = 64;
+constexpr int kTsMaxWidth = 8192;
+struct TsConsts {
+  float4 k[kTsBatch];
+};
+hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
+                     float *filtered, float *result, const float *W, const float2 *tw, int rows, int n, int Z,
+                     int logZ, int contrast, hipStream_t s);
 // Axis-score kernels (axis.hip).
 // AverageGradient::apply scores of n slices [n][w][w] reconstructed at the
 // given centres; d_tmp holds n*w*w floats, d_partial n*kScoreBlocks*2.

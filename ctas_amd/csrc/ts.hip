@@ -1,0 +1,328 @@
+// Tomosynthesis: ts_add (lct.cpp:739-787) on the device.
+//
+// For one projection (rows x n) at angle a, the reference takes -log of the
+// whole array (ABS contrast), filters every row in place with the window
+// Filter::fill(n) by n-point FFTW transforms (and takes the running sum of
+// the row for REF contrast), and then adds to result(y, x) the filtered
+// sample projection(y, (long)d(x)), where
+//   d(x) = (x - R) * cos(a) - P + R,  R = n/2 + center,
+//   P = (plane - center) * sin(a),
+// is evaluated in float and clamped to [0, n - 1]. The column map d(x) is
+// the same for every row.
+//
+// Layout: one workgroup of 256 threads owns a pair of rows and walks through
+// a batch of projections in order. The two rows travel through the FFT as
+// the real and imaginary parts of one complex line in LDS (the window is
+// real and even); the filtered line stays in LDS, so the resampling reads it
+// there and the only HBM traffic per projection and pixel is the raw sample
+// in and, when asked for, the filtered sample out. Thread t keeps result
+// columns t + 256 j of both rows in registers across the whole batch, so
+// the result is read and written once per batch; the per-pixel sums happen
+// in projection order, as successive ts_add calls make them.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ctas_internal.hpp"
+#include "line_fft.hpp"
+
+namespace ctas {
+namespace {
+
+// Column index of ts_add's resampling (lct.cpp:779-782), float arithmetic
+// in the reference's order, each product and sum rounded on its own.
+__device__ __forceinline__ int ts_column(int x, float cs, float pc, float rc, int n) {
+#pragma clang fp contract(off)
+  float di = (static_cast<float>(x) - rc) * cs - pc + rc;
+  di = (di < 0.0f) ? 0.0f : (di >= static_cast<float>(n)) ? static_cast<float>(n - 1) : di;
+  int i = static_cast<int>(di);
+  // A NaN d(x) (non-finite angle or centre) converts to an unspecified
+  // integer; keep the LDS read inside the line.
+  i = i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+  return i;
+}
+
+// J = result columns per thread and row: n <= 256 J.
+template <int J>
+__global__ __launch_bounds__(256) void ts_kernel(const float *proj, long long proj_stride, int n_proj,
+                                                 TsConsts consts, float *filtered,
+                                                 float *__restrict__ result, const float *__restrict__ W,
+                                                 const float2 *__restrict__ tw, int rows, int n, int Z,
+                                                 int logZ, int contrast) {
+  extern __shared__ float2 buf[];
+  const int tid = threadIdx.x;
+  const int t0 = 2 * blockIdx.x, t1 = t0 + 1;
+  const bool has1 = t1 < rows;
+  const int s0 = (Z - n) >> 1;
+  const unsigned rsh = 32 - logZ;
+  const size_t row0 = static_cast<size_t>(t0) * n, row1 = row0 + n;
+  const bool absorb = contrast == CTAS_CONTRAST_ABS;
+
+  // Accumulators are loaded, updated and kept without branches (columns
+  // past the row and a missing second row alias valid elements and are
+  // never stored): conditional updates of register arrays this long make
+  // the compiler copy the whole array per update.
+  const size_t row1c = has1 ? row1 : row0;
+  float acc0[J], acc1[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = min(tid + 256 * j, n - 1);
+    acc0[j] = result[row0 + x];
+    acc1[j] = result[row1c + x];
+  }
+
+  for (int p = 0; p < n_proj; ++p) {
+    const float *r0 = proj + static_cast<size_t>(p) * proj_stride + row0;
+    const float *r1 = r0 + n;
+    // The row pair at positions s0..s0+n-1 of the zeroed line, bit-reversed.
+    for (int m = tid; m < Z; m += 256) {
+      const int x = m - s0;
+      float a = 0.0f, c = 0.0f;
+      if (x >= 0 && x < n) {
+        a = r0[x];
+        if (absorb) a = -__logf(a);
+        if (has1) {
+          c = r1[x];
+          if (absorb) c = -__logf(c);
+        }
+      }
+      buf[__brev(static_cast<unsigned>(m)) >> rsh] = make_float2(a, c);
+    }
+    __syncthreads();
+    line_filter(buf, tw, W, Z, logZ, tid);
+
+    float scale = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
+    if (contrast == CTAS_CONTRAST_REF) {
+      // std::partial_sum of the filtered row (lct.cpp:774-775), one lane per
+      // row in sequential order, written back into the line.
+      if (tid == 0 || (tid == 64 && has1)) {
+        const bool im = tid == 64;
+        float run = 0.0f;
+        float *line = reinterpret_cast<float *>(buf) + (im ? 1 : 0);  // one float of each pair
+        for (int x = 0; x < n; ++x) {
+          const unsigned at = 2 * (__brev(static_cast<unsigned>(s0 + x)) >> rsh);
+          const float f = line[at] * scale;
+          run = (x == 0) ? f : run + f;
+          line[at] = run;
+        }
+      }
+      __syncthreads();
+      scale = 1.0f;
+    }
+
+    const float4 k = consts.k[p];  // {cos a, P, R, -}
+    if (filtered) {
+      float *f0 = filtered + static_cast<size_t>(p) * proj_stride + row0;
+      for (int x = tid; x < n; x += 256) {
+        const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+        f0[x] = v.x * scale;
+        if (has1) f0[n + x] = v.y * scale;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = min(tid + 256 * j, n - 1);
+      const int i = ts_column(x, k.x, k.y, k.z, n);
+      const float2 v = buf[__brev(static_cast<unsigned>(s0 + i)) >> rsh];
+      acc0[j] = acc0[j] + v.x * scale;
+      acc1[j] = acc1[j] + v.y * scale;
+    }
+    __syncthreads();  // the next projection overwrites the line
+  }
+
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = tid + 256 * j;
+    if (x < n) {
+      result[row0 + x] = acc0[j];
+      if (has1) result[row1 + x] = acc1[j];
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
+                     float *filtered, float *result, const float *W, const float2 *tw, int rows, int n, int Z,
+                     int logZ, int contrast, hipStream_t s) {
+  if (rows <= 0 || n_proj <= 0) return hipSuccess;
+  if (n_proj > kTsBatch || n > kTsMaxWidth) return hipErrorInvalidValue;
+  const dim3 grid((rows + 1) / 2), block(256);
+  const size_t lds = sizeof(float2) * static_cast<size_t>(Z);
+  const int J = (n + 255) / 256;
+#define CTAS_TS_CASE(JJ)                                                                                  \
+  hipLaunchKernelGGL(ts_kernel<JJ>, grid, block, lds, s, proj, proj_stride, n_proj, consts, filtered, result, \
+                     W, tw, rows, n, Z, logZ, contrast)
+  if (J <= 1) CTAS_TS_CASE(1);
+  else if (J <= 2) CTAS_TS_CASE(2);
+  else if (J <= 4) CTAS_TS_CASE(4);
+  else if (J <= 8) CTAS_TS_CASE(8);
+  else if (J <= 16) CTAS_TS_CASE(16);
+  else CTAS_TS_CASE(32);
+#undef CTAS_TS_CASE
+  return hipGetLastError();
+}
+
+namespace {
+
+// ts_add's per-projection constants (lct.cpp:762-765), with the reference's
+// types: sin and cos of the float angle in float, P = (plane - center) *
+// sin(a) in float, R = pixels * 0.5 + center in double, narrowed.
+float4 ts_consts(int n, float center, float angle, int plane) {
+  const float cur_sin = std::sin(angle), cur_cos = std::cos(angle);
+  const float plane_cos = (static_cast<float>(plane) - center) * cur_sin;
+  const float rcenter = static_cast<float>(n * 0.5 + static_cast<double>(center));
+  return make_float4(cur_cos, plane_cos, rcenter, 0.0f);
+}
+
+}  // namespace
+}  // namespace ctas
+
+using namespace ctas;
+
+struct ctas_ts {
+  int device = 0;
+  int rows = 0, n = 0, Z = 0, logZ = 0, contrast = 0;
+  float *d_W = nullptr;
+  float2 *d_tw = nullptr;
+};
+
+extern "C" {
+
+int ctas_filter_fill(int filter, float param, int n, float *W) {
+  if (!W) return fail(CTAS_E_ARG, "null argument");
+  std::vector<float> v;
+  if (int rc = filter_fill(filter, param, n, v)) return rc;
+  std::copy(v.begin(), v.end(), W);
+  return CTAS_OK;
+}
+
+int ctas_ts_window(int filter, float param, int n, int *Z, float *W) {
+  if (!Z) return fail(CTAS_E_ARG, "null argument");
+  std::vector<float> v;
+  int z = 0;
+  if (int rc = ts_window(filter, param, n, z, v)) return rc;
+  *Z = z;
+  if (W) std::copy(v.begin(), v.end(), W);
+  return CTAS_OK;
+}
+
+int ctas_ts_create(ctas_ts **ph, int rows, int pixels, int contrast, int filter, float filter_param) {
+  if (!ph) return fail(CTAS_E_ARG, "null handle pointer");
+  *ph = nullptr;
+  if (rows < 1 || pixels < 1) return fail(CTAS_E_ARG, "tomosynthesis: empty projection");
+  if (contrast < CTAS_CONTRAST_ABS || contrast > CTAS_CONTRAST_FLT)
+    return fail(CTAS_E_ARG, "unknown contrast code " + std::to_string(contrast));
+  if (pixels > kTsMaxWidth)
+    return fail(CTAS_E_ARG, "tomosynthesis: projection width " + std::to_string(pixels) + " above " +
+                                std::to_string(kTsMaxWidth));
+  std::unique_ptr<ctas_ts> h(new ctas_ts);
+  h->rows = rows;
+  h->n = pixels;
+  h->contrast = contrast;
+  std::vector<float> W;
+  if (int rc = ts_window(filter, filter_param, pixels, h->Z, W)) return rc;
+  while ((1 << h->logZ) < h->Z) ++h->logZ;
+  if (h->Z > kTsMaxWidth)
+    return fail(CTAS_E_ARG, "tomosynthesis: width " + std::to_string(pixels) + " needs a line of " +
+                                std::to_string(h->Z) + " points, more than the 64 KiB of LDS of one workgroup holds");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
+  CTAS_HIP_TRY(hipGetDevice(&h->device));
+  std::vector<float2> tw(std::max(h->Z / 2, 1));
+  for (int m = 0; m < h->Z / 2; ++m) {
+    const double a = -2.0 * M_PI * m / h->Z;
+    tw[m] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  if (hipMalloc(&h->d_W, sizeof(float) * W.size()) != hipSuccess ||
+      hipMalloc(&h->d_tw, sizeof(float2) * tw.size()) != hipSuccess) {
+    (void)hipFree(h->d_W);
+    return fail(CTAS_E_NOMEM, "device allocation failed");
+  }
+  CTAS_HIP_TRY(hipMemcpy(h->d_W, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(h->d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+  *ph = h.release();
+  return CTAS_OK;
+}
+
+int ctas_ts_destroy(ctas_ts *h) {
+  if (!h) return CTAS_OK;
+  (void)hipFree(h->d_W);
+  (void)hipFree(h->d_tw);
+  delete h;
+  return CTAS_OK;
+}
+
+int ctas_ts_info(const ctas_ts *h, int *rows, int *pixels, int *zidth) {
+  if (!h) return fail(CTAS_E_ARG, "null handle");
+  if (rows) *rows = h->rows;
+  if (pixels) *pixels = h->n;
+  if (zidth) *zidth = h->Z;
+  return CTAS_OK;
+}
+
+int ctas_ts_add_dev(ctas_ts *h, const float *d_proj, int n_proj, const float *angles, float center, int plane,
+                    float *d_filtered, float *d_result, void *stream) {
+  if (!h || !d_proj || !angles || !d_result || n_proj < 0) return fail(CTAS_E_ARG, "bad arguments");
+  const long long stride = static_cast<long long>(h->rows) * h->n;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (int p0 = 0; p0 < n_proj; p0 += kTsBatch) {
+    const int np = std::min(kTsBatch, n_proj - p0);
+    TsConsts k{};
+    for (int i = 0; i < np; ++i) k.k[i] = ts_consts(h->n, center, angles[p0 + i], plane);
+    const hipError_t e = launch_ts(d_proj + p0 * stride, stride, np, k, d_filtered ? d_filtered + p0 * stride : nullptr,
+                                   d_result, h->d_W, h->d_tw, h->rows, h->n, h->Z, h->logZ, h->contrast, s);
+    if (e != hipSuccess) return fail(CTAS_E_DEVICE, std::string("tomosynthesis: ") + hipGetErrorString(e));
+  }
+  return CTAS_OK;
+}
+
+int ctas_ts_add(float *projection, int rows, int pixels, float *result, int result_rows, int result_pixels,
+                int filter, float filter_param, float center, int contrast, float angle, int plane) {
+  if (!projection || !result) return fail(CTAS_E_ARG, "null argument");
+  if (result_pixels != pixels || result_rows != rows)
+    return fail(CTAS_E_ARG, "The size of the result array does not match the requested reconstruction.");
+  // The last geometry's plan and staging buffers are kept (until the
+  // process ends) for the next call: the tool calls this once per
+  // projection of a series. Calls are serialised.
+  struct Cached {
+    std::tuple<int, int, int, int, int, float> key{-1, 0, 0, 0, 0, 0.0f};
+    ctas_ts *plan = nullptr;
+    float *d_io = nullptr;
+  };
+  static Cached c;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(CTAS_E_DEVICE, "no HIP device is available");
+  const auto key = std::make_tuple(dev, rows, pixels, contrast, filter, filter_param);
+  if (!c.plan || c.key != key) {
+    ctas_ts_destroy(c.plan);
+    c.plan = nullptr;
+    if (c.d_io) (void)hipFree(c.d_io);
+    c.d_io = nullptr;
+    if (int rc = ctas_ts_create(&c.plan, rows, pixels, contrast, filter, filter_param)) return rc;
+    if (hipMalloc(&c.d_io, 2 * sizeof(float) * rows * static_cast<size_t>(pixels)) != hipSuccess)
+      return fail(CTAS_E_NOMEM, "device allocation failed");
+    c.key = key;
+  }
+  const size_t elems = static_cast<size_t>(rows) * pixels;
+  float *d_proj = c.d_io, *d_res = c.d_io + elems;
+  CTAS_HIP_TRY(hipMemcpy(d_proj, projection, sizeof(float) * elems, hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(d_res, result, sizeof(float) * elems, hipMemcpyHostToDevice));
+  // The reference filters the projection in place (the row view of lct.cpp:768).
+  if (int rc = ctas_ts_add_dev(c.plan, d_proj, 1, &angle, center, plane, d_proj, d_res, nullptr)) return rc;
+  CTAS_HIP_TRY(hipDeviceSynchronize());
+  CTAS_HIP_TRY(hipMemcpy(projection, d_proj, sizeof(float) * elems, hipMemcpyDeviceToHost));
+  CTAS_HIP_TRY(hipMemcpy(result, d_res, sizeof(float) * elems, hipMemcpyDeviceToHost));
+  return CTAS_OK;
+}
+
+}  // extern "C"

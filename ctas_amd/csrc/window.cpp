@@ -1,5 +1,6 @@
 // Host-side geometry of the reconstruction: padding width, arc and angle
 // table, and the frequency window of each filter.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -139,8 +140,13 @@ double bessel_i0(double x) {
 // and rounded once. The window factor of each filter follows the float /
 // double mix of lct.cpp:146-205, including its PARZEN, LANCKZOS and
 // NONE == RAMP behaviour.
-int filter_window(int filter, float param, int w, int contrast, std::vector<float> &W) {
-  if (w < 1) return fail(CTAS_E_ARG, "filter window: width must be positive");
+// Filter::fill (lct.cpp:122-209) for a line of n points: the Ram-Lak kernel
+// (h[0] = 1/4, h[odd k < n/2] = -1/(k pi)^2) taken to the frequency domain
+// by a half-complex inverse transform (H[k] = h0 + 2 sum h[m] cos(2 pi m k/n);
+// FP64 FFT for powers of two, a direct FP64 sum otherwise), times the
+// window of f = 2 min(k, n - k)/n, halved.
+int filter_fill(int filter, float param, int n, std::vector<float> &W) {
+  if (n < 1) return fail(CTAS_E_ARG, "Zero-size filter requested.");
   if (filter < CTAS_FILTER_NONE || filter > CTAS_FILTER_GAUSS)
     return fail(CTAS_E_ARG, "filter window: unknown filter code " + std::to_string(filter));
   if (filter == CTAS_FILTER_KAISER && param < 0.0f)
@@ -149,15 +155,26 @@ int filter_window(int filter, float param, int w, int contrast, std::vector<floa
   if (filter == CTAS_FILTER_GAUSS && param <= 0.0f)
     return fail(CTAS_E_ARG, "Sigma parameter of Guass filter must be strictly positive (set " +
                                 std::to_string(param) + ").");
-  const int Z = zidth(w);
+  const int Z = n;
   std::vector<std::complex<double>> a(Z, 0.0);
   a[0] = static_cast<double>(0.25f);
-  for (int n = 1; n < Z / 2; n += 2) {
-    const float h = static_cast<float>(-1.0 / (static_cast<double>(n) * n * M_PI * M_PI));
-    a[n] = h;
-    a[Z - n] = h;
+  for (int m = 1; m < Z / 2; m += 2) {
+    const float h = static_cast<float>(-1.0 / (static_cast<double>(m) * m * M_PI * M_PI));
+    a[m] = h;
+    a[Z - m] = h;
   }
-  fft_inplace(a);
+  if ((Z & (Z - 1)) == 0) {
+    fft_inplace(a);
+  } else {
+    std::vector<std::complex<double>> b(Z);
+    for (int k = 0; k < Z; ++k) {
+      double acc = a[0].real();
+      for (int m = 1; m < Z / 2; m += 2)
+        acc += 2.0 * a[m].real() * std::cos(2.0 * M_PI * static_cast<double>((static_cast<long long>(m) * k) % Z) / Z);
+      b[k] = acc;
+    }
+    a.swap(b);
+  }
   W.assign(Z, 0.0f);
   for (int k = 0; k < Z; ++k) {
     const float H = static_cast<float>(a[k].real());
@@ -188,9 +205,54 @@ int filter_window(int filter, float param, int w, int contrast, std::vector<floa
     }
     W[k] = static_cast<float>(static_cast<double>(H) * (static_cast<double>(fp) / 2.0));
   }
+  return CTAS_OK;
+}
+
+// CTrec's window (lct.cpp:487-508): Filter::fill at the padded width Z,
+// times the gain Z/w; REF: no DC.
+int filter_window(int filter, float param, int w, int contrast, std::vector<float> &W) {
+  if (w < 1) return fail(CTAS_E_ARG, "filter window: width must be positive");
+  const int Z = zidth(w);
+  if (int rc = filter_fill(filter, param, Z, W)) return rc;
   const float gain = static_cast<float>(Z) / static_cast<float>(w);
   for (float &v : W) v = v * gain;
   if (contrast == CTAS_CONTRAST_REF) W[0] = 0.0f;
+  return CTAS_OK;
+}
+
+// ts_add's filter (lct.cpp:758-777) convolves each row of n samples
+// circularly with g = IDFT_n(W), W = Filter::fill(n), by FFTW transforms of
+// exactly n points. The device transforms powers of two only, so for any
+// other n the same circular convolution runs as a zero-padded one of
+// Z = zidth(n) >= 2n - 1 points: the 2n - 1 lags -(n-1)..n-1 of the
+// periodic g then fall on distinct residues mod Z, and the window of that
+// longer line is DFT_Z of g laid out at lags 0..n-1 and Z-(n-1)..Z-1 (real,
+// as g is even). For a power of two n >= 2, Z = n and the window is W itself.
+// Both windows are for a transform whose inverse is scaled by 1/Z.
+int ts_window(int filter, float param, int n, int &Z, std::vector<float> &out) {
+  std::vector<float> W;
+  if (int rc = filter_fill(filter, param, n, W)) return rc;
+  if (n >= 2 && (n & (n - 1)) == 0) {
+    Z = n;
+    out = W;
+    return CTAS_OK;
+  }
+  Z = std::max(2, zidth(n));
+  // g[m] = (1/n) (sum_k hc[k] cos(2 pi k m / n) terms of FFTW's HC2R with a
+  // zero imaginary half): W is even, so g[m] = (1/n) sum_k W[k] cos(2 pi k m/n).
+  std::vector<double> cs(n);
+  for (int j = 0; j < n; ++j) cs[j] = std::cos(2.0 * M_PI * j / n);
+  std::vector<std::complex<double>> a(Z, 0.0);
+  for (int m = 0; m < n; ++m) {
+    double acc = 0.0;
+    for (int k = 0; k < n; ++k) acc += static_cast<double>(W[k]) * cs[(static_cast<long long>(k) * m) % n];
+    acc /= n;
+    a[m] += acc;
+    if (m) a[Z - m] += acc;
+  }
+  fft_inplace(a);
+  out.resize(Z);
+  for (int k = 0; k < Z; ++k) out[k] = static_cast<float>(a[k].real());
   return CTAS_OK;
 }
 

@@ -163,6 +163,36 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
 int ctas_ring_filter(float *sino, int N, int w, int box);
 int ctas_ring_filter_dev(float *d_sino, int S, int N, int w, int box, void *stream);
 
+/* ---- tomosynthesis (ts_add, ct.h:317 / lct.cpp:739-787) --------------- */
+/* Filter::fill(filt, n) (lct.cpp:122-209): the window of an n-point line,
+ * without the CTrec gain. */
+int ctas_filter_fill(int filter, float param, int n, float *W);
+/* The window of ts_add's row filter as a transform of *Z points (Z = n for
+ * a power of two n >= 2, else zidth(n)); W (optional) receives Z values. */
+int ctas_ts_window(int filter, float param, int n, int *Z, float *W);
+/* ts_add(projection, result, Filter(filter, param), center, contrast,
+ * angle, plane): projection (rows x pixels, row-major) is replaced by its
+ * filtered rows (after -log for ABS, running sums for REF), as in the
+ * reference; result (result_rows x result_pixels) accumulates the projection
+ * resampled for the plane. Angle in radians. Fails, like the reference's
+ * size check, unless result has the projection's shape. */
+int ctas_ts_add(float *projection, int rows, int pixels, float *result, int result_rows,
+                int result_pixels, int filter, float filter_param, float center, int contrast,
+                float angle, int plane);
+/* Device plan of ts_add for projections of rows x pixels (pixels <= 8192;
+ * 4096 unless a power of two): the window and twiddles of the row filter. */
+typedef struct ctas_ts ctas_ts;
+int ctas_ts_create(ctas_ts **h, int rows, int pixels, int contrast, int filter, float filter_param);
+int ctas_ts_destroy(ctas_ts *h);
+int ctas_ts_info(const ctas_ts *h, int *rows, int *pixels, int *zidth);
+/* n_proj successive ts_add calls on device arrays, enqueued on `stream`:
+ * d_proj holds the projections back to back ([n_proj][rows][pixels]),
+ * angles[n_proj] (host, radians) their angles; d_result accumulates them in
+ * order; d_filtered (optional, may equal d_proj) receives the filtered
+ * projections. */
+int ctas_ts_add_dev(ctas_ts *h, const float *d_proj, int n_proj, const float *angles, float center,
+                    int plane, float *d_filtered, float *d_result, void *stream);
+
 /* ---- rotation axis (lct.cpp:804-1326, axis.cpp:116-208) -------------- */
 /* AverageGradient::apply for n trial axis positions axisR (pixels from the
  * left edge) on the sinogram kept by ctas_fbp_sino of an ABS/RAMP handle. */

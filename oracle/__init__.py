@@ -39,6 +39,7 @@ def lib(fast: bool = False) -> C.CDLL:
         "orc_arcO": (f, [f, i]),
         "orc_angO": (f, [f, i]),
         "orc_angles": (None, [i, f, fp]),
+        "orc_filter_fill": (i, [i, f, i, fp]),
         "orc_filter_window": (i, [i, f, i, i, i, fp]),
         "orc_deabs": (i, [fp, C.c_long]),
         "orc_filter_rows": (None, [fp, i, i, fp, i]),
@@ -50,6 +51,7 @@ def lib(fast: bool = False) -> C.CDLL:
         "orc_ag_score": (d, [fp, i, i, fp, f, f, fp]),
         "orc_raxis": (i, [fp, i, i, f, f, i, fp]),
         "orc_raxis_pair": (i, [fp, fp, i, i, f, fp]),
+        "orc_ts_add": (i, [fp, fp, i, i, fp, f, i, f, i]),
         "orc_phantom_sino": (None, [i, i, d, i, dp, d, C.c_ulonglong, fp]),
     }
     for n, (r, a) in sig.items():
@@ -196,3 +198,25 @@ def random_ellipses(n_ell: int, w: int, axis_scale: float = 1.0, seed: int = 0) 
     e[:, 4] = np.pi * rng.random(n_ell)
     e[:, 5] = (-0.5 + 1.5 * rng.random(n_ell)) * (2.0 / w)
     return e
+
+
+def filter_fill(ftype: int, param: float, n: int) -> np.ndarray:
+    """Filter::fill of an n-point line (no gain)."""
+    W = np.zeros(n, np.float32)
+    rc = lib().orc_filter_fill(ftype, param, n, _f(W))
+    if rc:
+        raise ValueError(f"orc_filter_fill failed ({rc})")
+    return W
+
+
+def ts_add(proj: np.ndarray, result: np.ndarray, W: np.ndarray, center: float, contrast: int,
+           angle: float, plane: int) -> tuple[np.ndarray, np.ndarray]:
+    """ts_add on copies: (filtered projection, result + resampled projection)."""
+    P = np.array(proj, np.float32, copy=True, order="C")
+    R = np.array(result, np.float32, copy=True, order="C")
+    Wc = np.ascontiguousarray(W, np.float32)
+    assert P.shape == R.shape and Wc.shape == (P.shape[1],)
+    rc = lib().orc_ts_add(_f(P), _f(R), P.shape[0], P.shape[1], _f(Wc), center, contrast, angle, plane)
+    if rc:
+        raise ValueError(f"orc_ts_add failed ({rc})")
+    return P, R

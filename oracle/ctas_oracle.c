@@ -110,30 +110,43 @@ static double bessel_i0(double x) {
   return sum;
 }
 
-/* Filter::fill (lct.cpp:122-209) followed by the CTrec gain and REF rule
- * (lct.cpp:505-508). The spatial Ram-Lak taps are built in double and stored
- * as float (lct.cpp:132-135); FFTW's HC2R of them is evaluated here in FP64
- * and rounded once to float. */
-int orc_filter_window(int ftype, float param, int Z, int w, int contrast, float *W) {
-  if (Z <= 0 || w <= 0) return -1;
+/* Filter::fill (lct.cpp:122-209) for a line of n points. The spatial
+ * Ram-Lak taps h[0] = 1/4, h[m] = -1/(m pi)^2 (odd m < n/2, integer
+ * division) are built in double and stored as float (lct.cpp:132-135);
+ * FFTW's HC2R of them, H[k] = h[0] + 2 sum h[m] cos(2 pi m k / n) (a zero
+ * imaginary half), is evaluated here in FP64 (an FFT for a power of two n,
+ * the direct sum otherwise) and rounded once to float. Each value is then
+ * multiplied by the window of f = 2 min(k, n - k) / n, halved. */
+int orc_filter_fill(int ftype, float param, int n, float *W) {
+  if (n <= 0) return -1;
   if (ftype == ORC_KAISER && param < 0.0f) return -2;
   if (ftype == ORC_GAUSS && param <= 0.0f) return -2;
-  double *re = (double *)calloc((size_t)Z, sizeof(double));
-  double *im = (double *)calloc((size_t)Z, sizeof(double));
+  double *re = (double *)calloc((size_t)n, sizeof(double));
+  double *im = (double *)calloc((size_t)n, sizeof(double));
   if (!re || !im) { free(re); free(im); return -3; }
-  /* halfcomplex input r_0 = 1/4, r_n = h[n] (odd n < Z/2), i_n = 0:
-   * H[k] = r_0 + 2 sum r_n cos(2 pi n k / Z) = DFT of the even extension. */
   re[0] = (double)0.25f;
-  for (int n = 1; n < Z / 2; n += 2) {
-    const float h = (float)(-1.0 / ((double)(n * n) * M_PI * M_PI));
-    re[n] = (double)h;
-    re[Z - n] = (double)h;
+  for (int m = 1; m < n / 2; m += 2) {
+    const float h = (float)(-1.0 / ((double)(m * m) * M_PI * M_PI));
+    re[m] = (double)h;
+    re[n - m] = (double)h;
   }
-  fft64(re, im, Z, 0);
-  for (int k = 0; k < Z; k++) {
+  if ((n & (n - 1)) == 0) {
+    fft64(re, im, n, 0);
+  } else {
+    /* the even extension above makes the sum over all n entries equal the
+     * halfcomplex HC2R sum */
+    for (int k = 0; k < n; k++) {
+      double acc = 0.0;
+      for (int m = 0; m < n; m++)
+        if (re[m] != 0.0) acc += re[m] * cos(2.0 * M_PI * (double)(((long long)m * k) % n) / n);
+      im[k] = acc;
+    }
+    memcpy(re, im, sizeof(double) * (size_t)n);
+  }
+  for (int k = 0; k < n; k++) {
     float filt = (float)re[k];
-    float fp = (float)((k < Z / 2) ? k : Z - k);
-    fp = (float)((double)fp * (2.0 / Z));
+    float fp = (float)((k < n / 2) ? k : n - k);
+    fp = (float)((double)fp * (2.0 / n));
     switch (ftype) {
     case ORC_NONE:
     case ORC_RAMP: fp = 1.0f; break;
@@ -161,6 +174,15 @@ int orc_filter_window(int ftype, float param, int Z, int w, int contrast, float 
   }
   free(re);
   free(im);
+  return 0;
+}
+
+/* CTrec's window (lct.cpp:487-508): Filter::fill at the padded width Z,
+ * times the gain Z/w; REF: no DC. */
+int orc_filter_window(int ftype, float param, int Z, int w, int contrast, float *W) {
+  if (Z <= 0 || w <= 0) return -1;
+  const int rc = orc_filter_fill(ftype, param, Z, W);
+  if (rc) return rc;
   const float gain = (float)Z / (float)w;
   for (int k = 0; k < Z; k++) W[k] = W[k] * gain;
   if (contrast == ORC_REF) W[0] = 0.0f;
@@ -871,4 +893,82 @@ void orc_phantom_sino(int N, int w, double c, int n_ell, const double *ell, doub
       out[(size_t)p * w + k] = (float)v;
     }
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* tomosynthesis                                                       */
+/* ------------------------------------------------------------------ */
+
+/* filter_line (lct.cpp:280-291) on one row of n samples, by the defining
+ * sums of FFTW's transforms in FP64: R2HC gives hc[k] = sum x[m] cos(2 pi
+ * k m / n) for k <= n/2 and hc[n-k] = -sum x[m] sin(2 pi k m / n) for
+ * 0 < k < n - k; hc is multiplied by W position by position; HC2R gives
+ * y[m] = hc[0] + 2 sum_{0<k<n-k} (hc[k] cos - hc[n-k] sin)(2 pi k m / n)
+ * (+ hc[n/2] (-1)^m for even n); the row becomes y/n, rounded to float.
+ * cs and sn hold cos and sin of 2 pi j / n, j < n. */
+static void ts_filter_row(float *row, int n, const float *W, const double *cs, const double *sn,
+                          double *hc, double *y) {
+  for (int k = 0; k <= n / 2; k++) {
+    double r = 0.0, i = 0.0;
+    for (int m = 0; m < n; m++) {
+      const long long j = ((long long)k * m) % n;
+      r += (double)row[m] * cs[j];
+      i -= (double)row[m] * sn[j];
+    }
+    hc[k] = r;
+    if (k > 0 && k < n - k) hc[n - k] = i;
+  }
+  for (int k = 0; k < n; k++) hc[k] *= (double)W[k];
+  for (int m = 0; m < n; m++) {
+    double acc = hc[0];
+    for (int k = 1; k < n - k; k++) {
+      const long long j = ((long long)k * m) % n;
+      acc += 2.0 * (hc[k] * cs[j] - hc[n - k] * sn[j]);
+    }
+    if (n % 2 == 0) acc += (m % 2 ? -1.0 : 1.0) * hc[n / 2];
+    y[m] = acc;
+  }
+  for (int m = 0; m < n; m++) row[m] = (float)(y[m] / n);
+}
+
+/* ts_add (lct.cpp:739-787). proj (rows x n) is replaced by its filtered
+ * rows, as the reference's in-place row view leaves it: -log of every
+ * sample for ABS (lct.cpp:744-745), filter_line with W = Filter::fill(n)
+ * (FP64 sums, float result), and std::partial_sum in float for REF. result
+ * gets result(y, x) += proj(y, (long)d), with d evaluated in float in the
+ * reference's order and clamped to [0, n - 1] (lct.cpp:762-783). */
+int orc_ts_add(float *proj, float *result, int rows, int n, const float *W, float center, int contrast,
+               float angle, int plane) {
+  if (rows <= 0 || n <= 0) return -1;
+  if (contrast == ORC_ABS)
+    for (long i = 0; i < (long)rows * n; i++) proj[i] = -logf(proj[i]);
+  double *cs = (double *)malloc(sizeof(double) * (size_t)n);
+  double *sn = (double *)malloc(sizeof(double) * (size_t)n);
+  double *hc = (double *)malloc(sizeof(double) * (size_t)n);
+  double *y = (double *)malloc(sizeof(double) * (size_t)n);
+  if (!cs || !sn || !hc || !y) { free(cs); free(sn); free(hc); free(y); return -3; }
+  for (int j = 0; j < n; j++) {
+    cs[j] = cos(2.0 * M_PI * j / n);
+    sn[j] = sin(2.0 * M_PI * j / n);
+  }
+  const float cur_sin = sinf(angle);
+  const float cur_cos = cosf(angle);
+  const float plane_cos = ((float)plane - center) * cur_sin;
+  const float Rcenter = (float)(n * 0.5 + (double)center);
+  for (int yc = 0; yc < rows; yc++) {
+    float *row = proj + (size_t)yc * n;
+    ts_filter_row(row, n, W, cs, sn, hc, y);
+    if (contrast == ORC_REF)
+      for (int x = 1; x < n; x++) row[x] = row[x - 1] + row[x];
+    for (int x = 0; x < n; x++) {
+      float di = ((float)x - Rcenter) * cur_cos - plane_cos + Rcenter;
+      di = (di < 0) ? 0 : (di >= (float)n) ? (float)(n - 1) : di;
+      result[(size_t)yc * n + x] += row[(long)di];
+    }
+  }
+  free(cs);
+  free(sn);
+  free(hc);
+  free(y);
+  return 0;
 }

@@ -42,6 +42,7 @@ int   orc_zidth(int w);
 float orc_arcO(float anarc, int points);
 float orc_angO(float anarc, int points);
 void  orc_angles(int n, float anarc, float *sincos);
+int   orc_filter_fill(int ftype, float param, int n, float *W);
 int   orc_filter_window(int ftype, float param, int Z, int w, int contrast, float *W);
 int   orc_deabs(float *a, long n);
 void  orc_filter_rows(float *P, int N, int w, const float *W, int Z);
@@ -60,6 +61,8 @@ int   orc_raxis(const float *sino, int N, int w, float anarc, float maxDev,
                 int algo, float *result);
 int   orc_raxis_pair(const float *p0, const float *p180, int rows, int w,
                      float maxDev, float *result);
+int   orc_ts_add(float *proj, float *result, int rows, int n, const float *W,
+                 float center, int contrast, float angle, int plane);
 void  orc_phantom_sino(int N, int w, double c, int n_ell, const double *ell,
                        double N0, unsigned long long seed, float *out);
 

@@ -1,0 +1,125 @@
+"""The tomosynthesis oracle (ts_add, lct.cpp:739-787) pinned by what the
+mathematics fixes, and the host-side checks of the engine's ts entry points.
+
+* Filter::fill of an n-point line: the CTrec window is fill(zidth) times the
+  gain; the Ram-Lak response is the Fourier series of the triangle wave
+  |f| (exactly 1/4 at f = 1/4, within ~1/n of |f| elsewhere); the engine's
+  fill agrees with the oracle's for powers of two and other lengths.
+* filter_line: the oracle's FFTW-definition sums agree with numpy's
+  rfft/irfft route; a window of ones is the identity.
+* Resampling: angle 0 is the identity map; angle pi mirrors the row about
+  n/2; angle pi/2 puts column n/2 + 2 center - plane everywhere (pins the
+  signs of the plane and centre terms); columns past the row clamp.
+* Contrasts: ABS is -log before the filter, REF a running sum after it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ctas_amd as ca
+import oracle
+
+PHS, ABS, REF = oracle.PHS, oracle.ABS, oracle.REF
+
+
+def test_fill_times_gain_is_the_ctrec_window():
+    for w in (5, 64, 300):
+        Z = oracle.zidth(w)
+        for ft, p in ((1, 0.0), (5, 0.0), (9, 2.5), (10, 0.4)):
+            fill = oracle.filter_fill(ft, p, Z)
+            win = oracle.filter_window(ft, p, w, ABS)
+            np.testing.assert_array_equal(fill * np.float32(Z / w), win)
+
+
+@pytest.mark.parametrize("n", [16, 64, 1000, 1024, 4096])
+def test_ramp_fill_is_the_triangle_series(n):
+    W = oracle.filter_fill(1, 0.0, n).astype(np.float64)
+    k = np.arange(n)
+    f = np.minimum(k, n - k) / n
+    # H = 2 W is the Fourier series of |f| on [-1/2, 1/2], truncated at odd
+    # m < n/2: 1/4 - (2/pi^2) sum cos(2 pi m f)/m^2.
+    assert np.abs(2 * W - f).max() < 1.5 / n
+    if n % 4 == 0:
+        assert abs(W[n // 4] - 0.125) < 1e-7  # every cos(pi m/2) vanishes
+    np.testing.assert_array_equal(W[1:], W[1:][::-1])  # even
+
+
+@pytest.mark.parametrize("n", [7, 16, 100, 257, 1000, 1024])
+@pytest.mark.parametrize("ft,p", [(1, 0.0), (5, 0.0), (7, 0.0), (9, 2.5)])
+def test_engine_fill_matches_oracle(n, ft, p):
+    np.testing.assert_allclose(ca.filter_fill(ft, p, n), oracle.filter_fill(ft, p, n), rtol=2e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("n", [64, 100, 257])
+@pytest.mark.parametrize("ft", [1, 5])
+def test_ts_filter_matches_numpy_fft(n, ft):
+    rng = np.random.default_rng(n)
+    P = rng.standard_normal((5, n)).astype(np.float32)
+    W = oracle.filter_fill(ft, 0.0, n)
+    filt, res = oracle.ts_add(P, np.zeros_like(P), W, 0.0, PHS, 0.0, 0)
+    ref = np.fft.irfft(np.fft.rfft(P.astype(np.float64), axis=1) * W[: n // 2 + 1], n, axis=1)
+    np.testing.assert_allclose(filt, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+    np.testing.assert_array_equal(res, filt)  # angle 0: identity map
+
+
+def _identity(n):
+    return np.ones(n, np.float32)
+
+
+@pytest.mark.parametrize("n", [32, 33])
+def test_ts_resample_known_answers(n):
+    rng = np.random.default_rng(2)
+    P = rng.standard_normal((3, n)).astype(np.float32) + 5
+    one = _identity(n)
+    filt, res = oracle.ts_add(P, np.zeros_like(P), one, 0.25, PHS, 0.0, 7)
+    np.testing.assert_allclose(filt, P, rtol=1e-6)
+    np.testing.assert_array_equal(res, filt)
+    # angle pi: d(x) = n - x (+ a rounding-level term), x = 0 clamps to n - 1.
+    filt, res = oracle.ts_add(P, np.zeros_like(P), one, 0.0, PHS, np.float32(math.pi), 3)
+    cols = [n - 1] + [n - x for x in range(1, n)]
+    np.testing.assert_array_equal(res, filt[:, cols])
+    # angle pi/2: d = n/2 + 2 c - plane for every x.
+    for plane, c in ((5, 0.25), (-4, 0.25), (0, 1.25)):
+        filt, res = oracle.ts_add(P, np.zeros_like(P), one, c, PHS, np.float32(math.pi / 2), plane)
+        col = int(math.floor(n * 0.5 + 2 * c - plane))
+        np.testing.assert_array_equal(res, np.repeat(filt[:, col:col + 1], n, axis=1))
+    # clamps on both sides
+    filt, res = oracle.ts_add(P, np.zeros_like(P), one, 0.0, PHS, np.float32(math.pi / 2), -3 * n)
+    np.testing.assert_array_equal(res, np.repeat(filt[:, n - 1:], n, axis=1))
+    filt, res = oracle.ts_add(P, np.zeros_like(P), one, 0.0, PHS, np.float32(math.pi / 2), 3 * n)
+    np.testing.assert_array_equal(res, np.repeat(filt[:, :1], n, axis=1))
+
+
+def test_ts_contrasts():
+    n = 48
+    rng = np.random.default_rng(4)
+    P = rng.uniform(0.2, 1.0, (4, n)).astype(np.float32)
+    W = oracle.filter_fill(1, 0.0, n)
+    fa, _ = oracle.ts_add(P, np.zeros_like(P), _identity(n), 0.0, ABS, 0.0, 0)
+    np.testing.assert_allclose(fa, -np.log(P.astype(np.float64)), rtol=1e-6, atol=1e-7)
+    fp, _ = oracle.ts_add(P, np.zeros_like(P), W, 0.0, PHS, 0.0, 0)
+    fr, rr = oracle.ts_add(P, np.zeros_like(P), W, 0.0, REF, 0.0, 0)
+    np.testing.assert_allclose(fr, np.cumsum(fp.astype(np.float64), axis=1), rtol=0, atol=1e-5)
+    np.testing.assert_array_equal(rr, fr)
+
+
+def test_ts_accumulates_into_result():
+    n = 40
+    rng = np.random.default_rng(8)
+    P = rng.standard_normal((2, n)).astype(np.float32)
+    R0 = rng.standard_normal((2, n)).astype(np.float32)
+    W = oracle.filter_fill(5, 0.0, n)
+    _, r1 = oracle.ts_add(P, R0, W, 1.5, PHS, 0.4, 6)
+    _, r0 = oracle.ts_add(P, np.zeros_like(P), W, 1.5, PHS, 0.4, 6)
+    np.testing.assert_array_equal(r1, R0 + r0)
+
+
+def test_ts_host_argument_errors():
+    P = np.ones((4, 8), np.float32)
+    with pytest.raises(ca.CtasError, match="size of the result array"):
+        ca.ts_add(P, np.zeros((4, 9), np.float32))
+    with pytest.raises(ca.CtasError, match="above 8192"):
+        ca.TsPlan(2, 9000)
+    with pytest.raises(ca.CtasError, match="LDS"):
+        ca.TsPlan(2, 5000)  # padded to 16384 points
