@@ -160,7 +160,7 @@ struct TsConsts {
 };
 hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
                      float *filtered, float *result, const float *W, const float2 *tw, int rows, int n, int Z,
-                     int logZ, int contrast, hipStream_t s);
+                     int logZ, int contrast, int cus, int lds_max, hipStream_t s);
 // Axis-score kernels (axis.hip).
 // AverageGradient::apply scores of n slices [n][w][w] reconstructed at the
 // given centres; d_tmp holds n*w*w floats, d_partial n*kScoreBlocks*2.

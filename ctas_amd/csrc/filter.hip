@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
   const Item &it = items[find_item_by_pair(items, n_items, b)];
   const int pair = b - it.pair_base;
   const int w = it.w, N = it.N, Z = it.Z, logZ = it.logZ;
+  const int psh = line_psh(Z);
   const int s0 = (Z - w) >> 1;
   const int t0 = 2 * pair, t1 = t0 + 1;
   const bool has1 = t1 < N;
@@ -155,18 +156,18 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
         if (absorb) c = deabs(c);
       }
     }
-    buf[__brev(static_cast<unsigned>(n)) >> (32 - logZ)] = make_float2(a, c);
+    buf[lds_pad(__brev(static_cast<unsigned>(n)) >> (32 - logZ), psh)] = make_float2(a, c);
   }
   __syncthreads();
 
-  line_filter(buf, tw, it.W, Z, logZ, tid);
+  line_filter(buf, tw, it.W, Z, logZ, tid, psh);
 
   const float invZ = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
   const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
   const unsigned rsh = 32 - logZ;
   if (contrast != CTAS_CONTRAST_REF) {
     for (int x = tid; x < w; x += 256) {
-      const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+      const float2 v = buf[lds_pad(__brev(static_cast<unsigned>(s0 + x)) >> rsh, psh)];
       const float a = v.x * invZ, c = v.y * invZ;
       store_val<ST>(it.filt, row0 + x, a);
       if (it.filt_f32) it.filt_f32[row0 + x] = a;
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
     const size_t rowb = im ? row1 : row0;
     float run = 0.0f;
     for (int x = 0; x < w; ++x) {
-      const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+      const float2 v = buf[lds_pad(__brev(static_cast<unsigned>(s0 + x)) >> rsh, psh)];
       const float f = (im ? v.y : v.x) * invZ;
       run = (x == 0) ? f : run + f;
       store_val<ST>(it.filt, rowb + x, run);
@@ -497,7 +498,8 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
     }
     return hipGetLastError();
   }
-  const size_t lds = sizeof(float2) << max_logZ;
+  // a padded line of any item fits in that of the largest
+  const size_t lds = sizeof(float2) * padded_len(1 << max_logZ, line_psh(1 << max_logZ));
   switch (store) {
     case ST_F32: hipLaunchKernelGGL(filter_kernel<ST_F32>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
     case ST_F16: hipLaunchKernelGGL(filter_kernel<ST_F16>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;

@@ -49,15 +49,32 @@ __device__ __forceinline__ int ts_column(int x, float cs, float pc, float rc, in
   return i;
 }
 
-// J = result columns per thread and row: n <= 256 J.
-template <int J>
-__global__ __launch_bounds__(256) void ts_kernel(const float *proj, long long proj_stride, int n_proj,
-                                                 TsConsts consts, float *filtered,
-                                                 float *__restrict__ result, const float *__restrict__ W,
-                                                 const float2 *__restrict__ tw, int rows, int n, int Z,
-                                                 int logZ, int contrast) {
-  extern __shared__ float2 buf[];
+// K projections are filtered at once, one per 256-thread quarter of the
+// workgroup, each in its own line of LDS; all 256 K threads then add them
+// to their columns (x = tid + 256 K j, j < J) in projection order.
+template <int J, int K, bool TAB>
+__global__ __launch_bounds__(256 * K) void ts_kernel(const float *proj, long long proj_stride, int n_proj,
+                                                     TsConsts consts, float *filtered,
+                                                     float *__restrict__ result, const float *__restrict__ W,
+                                                     const float2 *__restrict__ tw, int rows, int n, int Z,
+                                                     int logZ, int contrast) {
+  extern __shared__ float2 lines[];
   const int tid = threadIdx.x;
+  const int sub = tid >> 8, lt = tid & 255;  // quarter and thread within it
+  const int psh = line_psh(Z), L = padded_len(Z, psh);  // line stride in LDS
+  float2 *buf = lines + sub * L;
+  // TAB: the twiddles and the window, read at every stage of every
+  // projection, are staged in LDS behind the lines.
+  const float2 *twp = tw;
+  const float *Wp = W;
+  if constexpr (TAB) {
+    float2 *tw_s = lines + K * L;
+    float *W_s = reinterpret_cast<float *>(tw_s + Z / 2);
+    for (int m = tid; m < Z / 2; m += 256 * K) tw_s[m] = tw[m];
+    for (int m = tid; m < Z; m += 256 * K) W_s[m] = W[m];
+    twp = tw_s;
+    Wp = W_s;
+  }
   const int t0 = 2 * blockIdx.x, t1 = t0 + 1;
   const bool has1 = t1 < rows;
   const int s0 = (Z - n) >> 1;
@@ -73,19 +90,21 @@ __global__ __launch_bounds__(256) void ts_kernel(const float *proj, long long pr
   float acc0[J], acc1[J];
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    const int x = min(tid + 256 * j, n - 1);
+    const int x = min(tid + 256 * K * j, n - 1);
     acc0[j] = result[row0 + x];
     acc1[j] = result[row1c + x];
   }
 
-  for (int p = 0; p < n_proj; ++p) {
+  for (int p0 = 0; p0 < n_proj; p0 += K) {
+    const int p = p0 + sub;
+    const bool live = p < n_proj;  // the last group may be short
     const float *r0 = proj + static_cast<size_t>(p) * proj_stride + row0;
     const float *r1 = r0 + n;
     // The row pair at positions s0..s0+n-1 of the zeroed line, bit-reversed.
-    for (int m = tid; m < Z; m += 256) {
+    for (int m = lt; m < Z; m += 256) {
       const int x = m - s0;
       float a = 0.0f, c = 0.0f;
-      if (x >= 0 && x < n) {
+      if (live && x >= 0 && x < n) {
         a = r0[x];
         if (absorb) a = -__logf(a);
         if (has1) {
@@ -93,21 +112,20 @@ __global__ __launch_bounds__(256) void ts_kernel(const float *proj, long long pr
           if (absorb) c = -__logf(c);
         }
       }
-      buf[__brev(static_cast<unsigned>(m)) >> rsh] = make_float2(a, c);
+      buf[lds_pad(__brev(static_cast<unsigned>(m)) >> rsh, psh)] = make_float2(a, c);
     }
     __syncthreads();
-    line_filter(buf, tw, W, Z, logZ, tid);
+    line_filter(buf, twp, Wp, Z, logZ, lt, psh);
 
     float scale = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
     if (contrast == CTAS_CONTRAST_REF) {
       // std::partial_sum of the filtered row (lct.cpp:774-775), one lane per
       // row in sequential order, written back into the line.
-      if (tid == 0 || (tid == 64 && has1)) {
-        const bool im = tid == 64;
+      if (lt == 0 || (lt == 64 && has1)) {
+        float *line = reinterpret_cast<float *>(buf) + (lt == 64 ? 1 : 0);  // one float of each pair
         float run = 0.0f;
-        float *line = reinterpret_cast<float *>(buf) + (im ? 1 : 0);  // one float of each pair
         for (int x = 0; x < n; ++x) {
-          const unsigned at = 2 * (__brev(static_cast<unsigned>(s0 + x)) >> rsh);
+          const unsigned at = 2 * lds_pad(__brev(static_cast<unsigned>(s0 + x)) >> rsh, psh);
           const float f = line[at] * scale;
           run = (x == 0) ? f : run + f;
           line[at] = run;
@@ -117,29 +135,34 @@ __global__ __launch_bounds__(256) void ts_kernel(const float *proj, long long pr
       scale = 1.0f;
     }
 
-    const float4 k = consts.k[p];  // {cos a, P, R, -}
-    if (filtered) {
+    if (filtered && live) {
       float *f0 = filtered + static_cast<size_t>(p) * proj_stride + row0;
-      for (int x = tid; x < n; x += 256) {
-        const float2 v = buf[__brev(static_cast<unsigned>(s0 + x)) >> rsh];
+      for (int x = lt; x < n; x += 256) {
+        const float2 v = buf[lds_pad(__brev(static_cast<unsigned>(s0 + x)) >> rsh, psh)];
         f0[x] = v.x * scale;
         if (has1) f0[n + x] = v.y * scale;
       }
     }
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int x = min(tid + 256 * j, n - 1);
-      const int i = ts_column(x, k.x, k.y, k.z, n);
-      const float2 v = buf[__brev(static_cast<unsigned>(s0 + i)) >> rsh];
-      acc0[j] = acc0[j] + v.x * scale;
-      acc1[j] = acc1[j] + v.y * scale;
+    for (int q = 0; q < K; ++q) {
+      if (p0 + q >= n_proj) break;
+      const float4 k = consts.k[p0 + q];  // {cos a, P, R, -}
+      const float2 *lq = lines + q * L;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int x = min(tid + 256 * K * j, n - 1);
+        const int i = ts_column(x, k.x, k.y, k.z, n);
+        const float2 v = lq[lds_pad(__brev(static_cast<unsigned>(s0 + i)) >> rsh, psh)];
+        acc0[j] = acc0[j] + v.x * scale;
+        acc1[j] = acc1[j] + v.y * scale;
+      }
     }
-    __syncthreads();  // the next projection overwrites the line
+    __syncthreads();  // the next group overwrites the lines
   }
 
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    const int x = tid + 256 * j;
+    const int x = tid + 256 * K * j;
     if (x < n) {
       result[row0 + x] = acc0[j];
       if (has1) result[row1 + x] = acc1[j];
@@ -151,21 +174,55 @@ __global__ __launch_bounds__(256) void ts_kernel(const float *proj, long long pr
 
 hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
                      float *filtered, float *result, const float *W, const float2 *tw, int rows, int n, int Z,
-                     int logZ, int contrast, hipStream_t s) {
+                     int logZ, int contrast, int cus, int lds_max, hipStream_t s) {
   if (rows <= 0 || n_proj <= 0) return hipSuccess;
-  if (n_proj > kTsBatch || n > kTsMaxWidth) return hipErrorInvalidValue;
-  const dim3 grid((rows + 1) / 2), block(256);
-  const size_t lds = sizeof(float2) * static_cast<size_t>(Z);
-  const int J = (n + 255) / 256;
-#define CTAS_TS_CASE(JJ)                                                                                  \
-  hipLaunchKernelGGL(ts_kernel<JJ>, grid, block, lds, s, proj, proj_stride, n_proj, consts, filtered, result, \
-                     W, tw, rows, n, Z, logZ, contrast)
-  if (J <= 1) CTAS_TS_CASE(1);
-  else if (J <= 2) CTAS_TS_CASE(2);
-  else if (J <= 4) CTAS_TS_CASE(4);
-  else if (J <= 8) CTAS_TS_CASE(8);
-  else if (J <= 16) CTAS_TS_CASE(16);
-  else CTAS_TS_CASE(32);
+  if (n_proj > kTsBatch || n > kTsMaxWidth || Z > kTsMaxWidth) return hipErrorInvalidValue;
+  // Projections in flight per workgroup (K): up to 4, at most the batch, as
+  // long as every workgroup of the launch stays resident (160 KiB of LDS per
+  // CU; a second round of workgroups would wait for the first to finish the
+  // whole batch) within the workgroup limit of lds_max bytes. The twiddle
+  // and window tables go to LDS too (TAB) where that keeps one line per
+  // workgroup resident.
+  const int psh = line_psh(Z);
+  const size_t line_bytes = sizeof(float2) * padded_len(Z, psh);
+  const size_t tab_bytes = (sizeof(float2) / 2 + sizeof(float)) * static_cast<size_t>(Z);
+  const int pairs = (rows + 1) / 2;
+  const size_t per_cu = (pairs + cus - 1) / cus;
+  auto fits = [&](size_t b) { return b <= static_cast<size_t>(lds_max) && per_cu * b <= 160 * 1024; };
+  const bool tab = fits(line_bytes + tab_bytes);
+  const size_t extra = tab ? tab_bytes : 0;
+  int K = 1;
+  while (K < 4 && K < n_proj && 2 * K * Z <= 8192 && fits(2 * K * line_bytes + extra)) K *= 2;
+  const size_t lds = line_bytes * K + extra;
+  if (lds > static_cast<size_t>(lds_max)) return hipErrorInvalidValue;
+  const dim3 grid(pairs), block(256 * K);
+  const int J = (n + 256 * K - 1) / (256 * K);
+#define CTAS_TS_CASE(JJ, KK)                                                                                  \
+  do {                                                                                                        \
+    if (tab)                                                                                                  \
+      hipLaunchKernelGGL((ts_kernel<JJ, KK, true>), grid, block, lds, s, proj, proj_stride, n_proj, consts,   \
+                         filtered, result, W, tw, rows, n, Z, logZ, contrast);                                \
+    else                                                                                                      \
+      hipLaunchKernelGGL((ts_kernel<JJ, KK, false>), grid, block, lds, s, proj, proj_stride, n_proj, consts,  \
+                         filtered, result, W, tw, rows, n, Z, logZ, contrast);                                \
+  } while (0)
+  // K = 4 only for Z <= 2048 (so J <= 2), K = 2 for Z <= 4096 (J <= 8).
+  if (K == 4) {
+    if (J <= 1) CTAS_TS_CASE(1, 4);
+    else CTAS_TS_CASE(2, 4);
+  } else if (K == 2) {
+    if (J <= 1) CTAS_TS_CASE(1, 2);
+    else if (J <= 2) CTAS_TS_CASE(2, 2);
+    else if (J <= 4) CTAS_TS_CASE(4, 2);
+    else CTAS_TS_CASE(8, 2);
+  } else {
+    if (J <= 1) CTAS_TS_CASE(1, 1);
+    else if (J <= 2) CTAS_TS_CASE(2, 1);
+    else if (J <= 4) CTAS_TS_CASE(4, 1);
+    else if (J <= 8) CTAS_TS_CASE(8, 1);
+    else if (J <= 16) CTAS_TS_CASE(16, 1);
+    else CTAS_TS_CASE(32, 1);
+  }
 #undef CTAS_TS_CASE
   return hipGetLastError();
 }
@@ -190,6 +247,7 @@ using namespace ctas;
 struct ctas_ts {
   int device = 0;
   int rows = 0, n = 0, Z = 0, logZ = 0, contrast = 0;
+  int cus = 1, lds_max = 65536;  // compute units, LDS bytes per workgroup
   float *d_W = nullptr;
   float2 *d_tw = nullptr;
 };
@@ -236,6 +294,12 @@ int ctas_ts_create(ctas_ts **ph, int rows, int pixels, int contrast, int filter,
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
   CTAS_HIP_TRY(hipGetDevice(&h->device));
+  CTAS_HIP_TRY(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device));
+  CTAS_HIP_TRY(hipDeviceGetAttribute(&h->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+  h->cus = std::max(h->cus, 1);
+  if (sizeof(float2) * static_cast<size_t>(padded_len(h->Z, line_psh(h->Z))) > static_cast<size_t>(h->lds_max))
+    return fail(CTAS_E_ARG, "tomosynthesis: width " + std::to_string(pixels) + " needs a line of " +
+                                std::to_string(h->Z) + " points, more than one workgroup's LDS holds");
   std::vector<float2> tw(std::max(h->Z / 2, 1));
   for (int m = 0; m < h->Z / 2; ++m) {
     const double a = -2.0 * M_PI * m / h->Z;
@@ -278,7 +342,8 @@ int ctas_ts_add_dev(ctas_ts *h, const float *d_proj, int n_proj, const float *an
     TsConsts k{};
     for (int i = 0; i < np; ++i) k.k[i] = ts_consts(h->n, center, angles[p0 + i], plane);
     const hipError_t e = launch_ts(d_proj + p0 * stride, stride, np, k, d_filtered ? d_filtered + p0 * stride : nullptr,
-                                   d_result, h->d_W, h->d_tw, h->rows, h->n, h->Z, h->logZ, h->contrast, s);
+                                   d_result, h->d_W, h->d_tw, h->rows, h->n, h->Z, h->logZ, h->contrast, h->cus,
+                                   h->lds_max, s);
     if (e != hipSuccess) return fail(CTAS_E_DEVICE, std::string("tomosynthesis: ") + hipGetErrorString(e));
   }
   return CTAS_OK;

@@ -6,7 +6,9 @@ arithmetic, bit-exact), so the result differs from the oracle only by the
 filter's float rounding: 1e-5 relative RMS, and every pixel within 1e-4 of
 the largest filtered magnitude, beyond the rounding of its float sum (an
 index off by one column fails that).
-Batches are bit-identical to successive single calls.
+REF contrast, whose running sum carries the rounding of every filtered
+sample before it, is held to 1e-4. Batches are bit-identical to successive
+single calls.
 """
 import math
 
@@ -65,7 +67,7 @@ def test_ts_add_matches_oracle(rows, n, contrast):
         ca.ts_add(Pd, Rd, "HANN", 0.0, c, contrast, float(a), plane)
         # REF: the running sum of n filtered samples carries their rounding
         _check(Pd, Rd, fref, rref, f"rows={rows} n={n} contrast={contrast} angle={angle} c={c} plane={plane}",
-               5e-5 if contrast == ca.REF else 1e-5)
+               1e-4 if contrast == ca.REF else 1e-5)
 
 
 @pytest.mark.parametrize("filt", ["RAMP", "KAISER:2.5", "GAUSS:0.4", "BLACKMAN"])
@@ -92,11 +94,13 @@ def test_ts_add_widest_lines(rows, n):
     _check(Pd, Rd, fref, rref, f"n={n}")
 
 
-@pytest.mark.parametrize("n_proj", [3, 70])
+@pytest.mark.parametrize("rows,n,n_proj", [(9, 200, 3), (9, 200, 70), (3, 2000, 3), (2, 4000, 2)])
 @pytest.mark.parametrize("contrast", [ca.ABS, ca.REF])
-def test_ts_batch_equals_successive_calls(torch_cuda, n_proj, contrast):
+def test_ts_batch_equals_successive_calls(torch_cuda, rows, n, n_proj, contrast):
+    """Batches run 4 (lines of up to 2048 points), 2 (4096) or 1 (8192)
+    projections per workgroup at once, with short last groups and a split
+    over launches past 64 projections."""
     torch = torch_cuda
-    rows, n = 9, 200
     Ps = np.stack([_proj(rows, n, 100 + i, contrast) for i in range(n_proj)])
     angles = np.linspace(-0.6, 0.6, n_proj).astype(np.float32)
     plan = ca.TsPlan(rows, n, contrast, "HANN")
@@ -113,10 +117,11 @@ def test_ts_batch_equals_successive_calls(torch_cuda, n_proj, contrast):
     W = oracle.filter_fill(5, 0.0, n)
     R = np.zeros((rows, n), np.float32)
     F = d_F.cpu().numpy()
+    tol = 1e-4 if contrast == ca.REF else 1e-5  # REF: running sums of up to 4000 samples
     for i in range(n_proj):
         f, R = oracle.ts_add(Ps[i], R, W, 1.25, contrast, angles[i], -6)
-        assert float(np.abs(F[i] - f).max()) <= 5e-5 * float(np.abs(f).max())
-    assert rel_rms(d_R.cpu().numpy(), R) <= 1e-5
+        assert float(np.abs(F[i] - f).max()) <= tol * float(np.abs(f).max())
+    assert rel_rms(d_R.cpu().numpy(), R) <= tol
 
 
 def test_ts_filtered_in_place(torch_cuda):
