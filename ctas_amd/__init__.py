@@ -108,6 +108,7 @@ def lib() -> C.CDLL:
         "ctas_ts_destroy": (i, [vp]),
         "ctas_ts_info": (i, [vp, ip, ip, ip]),
         "ctas_ts_add_dev": (i, [vp, vp, i, fp, f, i, vp, vp, vp]),
+        "ctas_ts_accumulate": (i, [vp, fp, i, fp, f, i, fp]),
         "ctas_ring_filter": (i, [fp, i, i, i]),
         "ctas_ring_filter_dev": (i, [vp, i, i, i, i, vp]),
         "ctas_axis_scores": (i, [vp, fp, i, fp]),
@@ -432,6 +433,20 @@ class TsPlan:
         z = C.c_int(0)
         _check(lib().ctas_ts_info(self._h, None, None, C.byref(z)))
         return z.value
+
+    def accumulate(self, projections: np.ndarray, angles, result: np.ndarray, center: float = 0.0,
+                   plane: int = 0) -> np.ndarray:
+        """Successive ts_add of host projections ([P][rows][pixels]) into a
+        copy of ``result``, which is returned."""
+        P = np.ascontiguousarray(projections, np.float32).reshape(-1, self.rows, self.pixels)
+        a = np.ascontiguousarray(np.atleast_1d(angles), np.float32)
+        if a.size != P.shape[0]:
+            raise CtasError(CTAS_E_ARG, "one angle per projection")
+        R = np.array(result, np.float32, copy=True, order="C")
+        if R.shape != (self.rows, self.pixels):
+            raise CtasError(CTAS_E_ARG, "The size of the result array does not match the requested reconstruction.")
+        _check(lib().ctas_ts_accumulate(self._h, _f32p(P), P.shape[0], _f32p(a), center, plane, _f32p(R)))
+        return R
 
     def add_dev(self, d_proj, angles, d_result, center: float = 0.0, plane: int = 0, d_filtered=None,
                 stream=None) -> None:

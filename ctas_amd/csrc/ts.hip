@@ -349,6 +349,51 @@ int ctas_ts_add_dev(ctas_ts *h, const float *d_proj, int n_proj, const float *an
   return CTAS_OK;
 }
 
+int ctas_ts_accumulate(ctas_ts *h, const float *projections, int n_proj, const float *angles, float center,
+                       int plane, float *result) {
+  if (!h || !projections || !angles || !result || n_proj < 0) return fail(CTAS_E_ARG, "bad arguments");
+  const size_t elems = static_cast<size_t>(h->rows) * h->n;
+  // Projections go up in chunks of kTsBatch through two staging buffers, so
+  // the upload of one chunk overlaps the kernel on the previous one.
+  const int chunk = std::min(std::max(n_proj, 1), kTsBatch);
+  float *d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * elems * (1 + 2 * static_cast<size_t>(chunk))) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation failed");
+  float *d_res = d, *d_stage[2] = {d + elems, d + elems * (1 + chunk)};
+  hipStream_t s = nullptr;
+  hipEvent_t done[2] = {nullptr, nullptr};
+  int rc = CTAS_OK;
+  auto dev = [&](hipError_t e) {
+    if (e != hipSuccess && rc == CTAS_OK) rc = fail(CTAS_E_DEVICE, std::string("tomosynthesis: ") + hipGetErrorString(e));
+    return rc == CTAS_OK;
+  };
+  if (dev(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) && dev(hipEventCreate(&done[0])) &&
+      dev(hipEventCreate(&done[1])) &&
+      dev(hipMemcpyAsync(d_res, result, sizeof(float) * elems, hipMemcpyHostToDevice, s))) {
+    for (int p0 = 0, b = 0; p0 < n_proj && rc == CTAS_OK; p0 += chunk, b ^= 1) {
+      const int np = std::min(chunk, n_proj - p0);
+      // the kernel that last read this staging buffer has finished
+      if (!dev(hipEventSynchronize(done[b]))) break;
+      if (!dev(hipMemcpyAsync(d_stage[b], projections + p0 * elems, sizeof(float) * elems * np,
+                              hipMemcpyHostToDevice, s)))
+        break;
+      if (int r = ctas_ts_add_dev(h, d_stage[b], np, angles + p0, center, plane, nullptr, d_res, s)) {
+        rc = r;
+        break;
+      }
+      dev(hipEventRecord(done[b], s));
+    }
+    if (rc == CTAS_OK && dev(hipMemcpyAsync(result, d_res, sizeof(float) * elems, hipMemcpyDeviceToHost, s)))
+      dev(hipStreamSynchronize(s));
+  }
+  if (s) (void)hipStreamSynchronize(s);
+  for (hipEvent_t e : done)
+    if (e) (void)hipEventDestroy(e);
+  if (s) (void)hipStreamDestroy(s);
+  (void)hipFree(d);
+  return rc;
+}
+
 int ctas_ts_add(float *projection, int rows, int pixels, float *result, int result_rows, int result_pixels,
                 int filter, float filter_param, float center, int contrast, float angle, int plane) {
   if (!projection || !result) return fail(CTAS_E_ARG, "null argument");

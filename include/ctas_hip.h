@@ -192,6 +192,12 @@ int ctas_ts_info(const ctas_ts *h, int *rows, int *pixels, int *zidth);
  * projections. */
 int ctas_ts_add_dev(ctas_ts *h, const float *d_proj, int n_proj, const float *angles, float center,
                     int plane, float *d_filtered, float *d_result, void *stream);
+/* n_proj successive ts_add calls on host arrays: projections
+ * ([n_proj][rows][pixels], left unchanged) are streamed to the device in
+ * chunks and added in order to result (host, [rows][pixels]); the ts tool's
+ * loop over its projection list (ts.cpp:212-217). */
+int ctas_ts_accumulate(ctas_ts *h, const float *projections, int n_proj, const float *angles,
+                       float center, int plane, float *result);
 
 /* ---- rotation axis (lct.cpp:804-1326, axis.cpp:116-208) -------------- */
 /* AverageGradient::apply for n trial axis positions axisR (pixels from the

@@ -141,3 +141,19 @@ def test_ts_plan_limits():
     plan = ca.TsPlan(3, 1000, ca.PHS, "RAMP")
     assert plan.zidth == 2048
     assert ca.TsPlan(3, 1024, ca.PHS, "RAMP").zidth == 1024
+
+
+def test_ts_accumulate_host_batch():
+    """The host batch entry streams 70 projections in two chunks."""
+    rows, n, n_proj = 5, 96, 70
+    Ps = np.stack([_proj(rows, n, 200 + i, ca.PHS) for i in range(n_proj)])
+    angles = np.linspace(-0.4, 0.5, n_proj).astype(np.float32)
+    R0 = np.random.default_rng(1).standard_normal((rows, n)).astype(np.float32)
+    plan = ca.TsPlan(rows, n, ca.PHS, "RAMP")
+    got = plan.accumulate(Ps, angles, R0, center=-2.0, plane=3)
+    W = oracle.filter_fill(1, 0.0, n)
+    R = R0
+    for i in range(n_proj):
+        _, R = oracle.ts_add(Ps[i], R, W, -2.0, ca.PHS, angles[i], 3)
+    assert rel_rms(got, R) <= 1e-5
+

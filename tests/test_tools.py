@@ -1,5 +1,6 @@
-"""The drop-in call sites: ctas-ct and ctas-ax (ct.cpp, axis.cpp of the
-reference) over the C ABI, on small float32 TIFF / .npy files."""
+"""The drop-in call sites: ctas-ct, ctas-ax and ctas-ts (ct.cpp, axis.cpp,
+ts.cpp of the reference) over the C ABI, on small float32 TIFF / .npy
+files."""
 import subprocess
 from pathlib import Path
 
@@ -47,7 +48,7 @@ def read_tiff_f32(path):
 
 
 def test_tools_built_and_print_usage():
-    for tool in ("ctas-ct", "ctas-ax"):
+    for tool in ("ctas-ct", "ctas-ax", "ctas-ts"):
         r = run(tool)
         assert r.returncode == 0 and "Usage" in r.stdout
 
@@ -58,6 +59,44 @@ def test_ct_fails_loudly_without_device(tmp_path):
     write_tiff(tmp_path / "s.tif", np.ones((16, 32), np.float32))
     r = run("ctas-ct", tmp_path / "s.tif", "-o", tmp_path / "o.tif")
     assert r.returncode != 0 and "ERROR!" in r.stderr
+
+
+def test_ts_tool_argument_errors(tmp_path):
+    write_tiff(tmp_path / "p0.tif", np.ones((8, 32), np.float32))
+    (tmp_path / "list.txt").write_text("-10.0 p0.tif\n")
+    r = run("ctas-ts", tmp_path / "list.txt")
+    assert r.returncode != 0 and "The reconstruction plane must be explicitly set" in r.stderr
+    r = run("ctas-ts", tmp_path / "list.txt", "-y", "16")
+    assert r.returncode != 0 and "The reconstruction plane is outside the image" in r.stderr
+    (tmp_path / "empty.txt").write_text("")
+    r = run("ctas-ts", tmp_path / "empty.txt", "-y", "1")
+    assert r.returncode != 0 and "does not provide any input info" in r.stderr
+
+
+@pytest.mark.gpu
+def test_ts_tool_matches_oracle(tmp_path):
+    """ts.cpp's loop: the projections of the list file, in order, each
+    added by ts_add with the default filter (RAMP) to the plane."""
+    if ca.device_count() < 1:
+        pytest.skip("no HIP device")
+    rows, w, c, plane = 24, 200, 1.5, -7
+    rng = np.random.default_rng(5)
+    angles_deg = [-20.0, -7.5, 5.0, 18.25]
+    lines, projs = [], []
+    for k, a in enumerate(angles_deg):
+        P = np.exp(-rng.uniform(0.1, 1.0, (rows, w))).astype(np.float32)
+        write_tiff(tmp_path / f"p{k}.tif", P)
+        lines.append(f"{a} p{k}.tif")
+        projs.append(P)
+    (tmp_path / "list.txt").write_text("\n".join(lines) + "\n")
+    r = run("ctas-ts", tmp_path / "list.txt", tmp_path / "plane.tif", "-y", plane, "-c", c, "-C", "ABS")
+    assert r.returncode == 0, r.stderr
+    got = read_tiff_f32(tmp_path / "plane.tif")
+    W = oracle.filter_fill(1, 0.0, w)
+    R = np.zeros((rows, w), np.float32)
+    for a, P in zip(angles_deg, projs):
+        _, R = oracle.ts_add(P, R, W, c, oracle.ABS, np.float32(np.float32(a) * np.pi / 180), plane)
+    assert rel_rms(got, R) <= 1e-5
 
 
 @pytest.mark.gpu
