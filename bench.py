@@ -102,15 +102,16 @@ def bench_axis(torch, np, ca, d_sino, w, N):
     rec.sino(host[0])
     rec.axis_scores(offs[:4])  # warm
     t0 = time.perf_counter()
-    best = []
     for s in range(host.shape[0]):
         rec.sino(host[s])
-        sc = rec.axis_scores(offs)
-        best.append(float(offs[int(np.argmin(sc))] - cent))
+        rec.axis_scores(offs)
     sweep_ms = (time.perf_counter() - t0) * 1e3
+    # Only raxis's search (sweep, walk to the local minimum nearest the
+    # centre, bisection) gives an axis; the raw scores' global minimum can
+    # sit at the edge of a +-20 px range.
     return {"raxis_sweep_ms": round(raxis_ms, 2), "raxis_offset": round(r - cent, 3),
             "sweep81x8_ms": round(sweep_ms, 1), "sweep81_ms_per_slice": round(sweep_ms / host.shape[0], 1),
-            "sweep81_best_offsets": best, "true_offset": 3.37}
+            "true_offset": 3.37}
 
 
 def bench_axis_cnn(torch, np, ca, n_fixed=24, n_mixed=48):
@@ -163,6 +164,34 @@ def bench_uneven(torch, np, ca, n_items=96, seed=7):
             "value": round(updates / (ms / 1e3) / 1e9, 1), "unit": "Gvoxel-updates/s"}
 
 
+def bench_ts(torch, np, ca, rows=2048, pixels=2048, n_proj=64, reps=5):
+    """Tomosynthesis (ts_add): a batch of 2048 x 2048 projections added to
+    one plane, inputs in HBM; HIP events on the stream the kernel runs on."""
+    print("[bench] tomosynthesis", file=sys.stderr, flush=True)
+    plan = ca.TsPlan(rows, pixels, ca.ABS, "RAMP")
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    d_p = torch.rand((n_proj, rows, pixels), device="cuda", generator=gen) * 0.9 + 0.05
+    d_r = torch.zeros((rows, pixels), device="cuda")
+    ang = np.linspace(-0.5, 0.5, n_proj).astype(np.float32)
+    stream = torch.cuda.Stream()
+    out = {}
+    for name, filt in (("plane_only", None), ("with_filtered_projections", torch.empty_like(d_p))):
+        plan.add_dev(d_p, ang, d_r, center=1.0, plane=10, d_filtered=filt, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            plan.add_dev(d_p, ang, d_r, center=1.0, plane=10, d_filtered=filt, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        per_px = 4 + (4 if filt is not None else 0)  # raw sample in (+ filtered sample out)
+        hbm = (n_proj * per_px + 8) * rows * pixels  # + the plane read and written once
+        out[name] = {"ms_per_batch": round(ms, 3), "us_per_projection": round(ms * 1e3 / n_proj, 2),
+                     "hbm_GBs": round(hbm / (ms / 1e3) / 1e9, 1),
+                     "hbm_frac": round(hbm / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return {"workload": f"{n_proj} projections of {rows} x {pixels}, ABS, RAMP, one plane", **out}
+
+
 def cpu_baseline(d_sino, w, N, c, threads):
     """The oracle's -O3 -ffast-math build (the reference's flags), one slice
     per thread as CTAS's RecInThread does, on 2 x threads slices."""
@@ -193,6 +222,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-axis", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-ts", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); slices are "
@@ -281,6 +311,8 @@ def main() -> int:
         result["axis_cnn"] = bench_axis_cnn(torch, np, ca)
     if not args.no_uneven:
         result["uneven_batch"] = bench_uneven(torch, np, ca)
+    if not args.no_ts:
+        result["tomosynthesis"] = bench_ts(torch, np, ca)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
