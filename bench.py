@@ -164,6 +164,42 @@ def bench_uneven(torch, np, ca, n_items=96, seed=7):
             "value": round(updates / (ms / 1e3) / 1e9, 1), "unit": "Gvoxel-updates/s"}
 
 
+def bench_4096x3000(torch, np, ca, S=64, N=3000, w=4096, c=3.37, steps=2):
+    """The reduced-precision workload: 64 slices of 4096 px x 3000 angles
+    (40 ellipses, Poisson 1e5, 1% per-detector gain errors), FP32 against
+    FP16 and BF16 storage, each as one replayed hipGraph per step."""
+    print("[bench] 4096 x 3000 x 64", file=sys.stderr, flush=True)
+    d_sino = torch.empty((S, N, w), dtype=torch.float32, device="cuda")
+    gain = torch.from_numpy((1.0 + 0.01 * np.random.default_rng(5).standard_normal(w)).astype(np.float32)).cuda()
+    for s in range(S):
+        ca.phantom_sino_dev(d_sino[s], ca.phantom_ellipses(40, w, 8.0, 3000 + s), c=c, N0=1e5, seed=3000 + s,
+                            d_gain=gain)
+    d_out = torch.empty((S, w, w), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.Stream()
+    updates = S * w * w * N
+    out = {"workload": f"{S} slices x {w} px x {N} angles, HANN, ABS, c=+{c}, 1% detector gain errors"}
+    for name, store in (("fp32", ca.STORE_FP32), ("fp16", ca.STORE_FP16), ("bf16", ca.STORE_BF16)):
+        rec = ca.CTrec(w, N, ca.ABS, "HANN", store=store)
+        rec.autotune(8)
+        rec.graph_capture(d_sino, d_out, [c] * S, stream)
+        rec.graph_launch(stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rec.graph_launch(stream)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        assert (rec.status(S) == 0).all(), "deAbs reported bad data"
+        out[name] = {"ms_per_step": round(ms, 2), "value": round(updates / (ms / 1e3) / 1e9, 1),
+                     "unit": "Gvoxel-updates/s"}
+        rec.close()
+    for name in ("fp16", "bf16"):
+        out[name]["speedup_vs_fp32"] = round(out["fp32"]["ms_per_step"] / out[name]["ms_per_step"], 3)
+    del d_sino, d_out
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_ts(torch, np, ca, rows=2048, pixels=2048, n_proj=64, reps=5):
     """Tomosynthesis (ts_add): a batch of 2048 x 2048 projections added to
     one plane, inputs in HBM; HIP events on the stream the kernel runs on."""
@@ -223,6 +259,7 @@ def main() -> int:
     ap.add_argument("--no-axis", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-ts", action="store_true")
+    ap.add_argument("--no-4096", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); slices are "
@@ -311,6 +348,8 @@ def main() -> int:
         result["axis_cnn"] = bench_axis_cnn(torch, np, ca)
     if not args.no_uneven:
         result["uneven_batch"] = bench_uneven(torch, np, ca)
+    if not args.no_4096:
+        result["fbp_4096x3000x64"] = bench_4096x3000(torch, np, ca)
     if not args.no_ts:
         result["tomosynthesis"] = bench_ts(torch, np, ca)
     if not args.no_cpu:

@@ -45,7 +45,9 @@ def lib(fast: bool = False) -> C.CDLL:
         "orc_filter_rows": (None, [fp, i, i, fp, i]),
         "orc_prepare_sino": (i, [fp, i, i, fp, i, i]),
         "orc_bp_ctas": (None, [fp, i, i, fp, f, f, fp]),
+        "orc_bp_ctas_rows": (None, [fp, i, i, fp, f, f, C.POINTER(i), i, fp]),
         "orc_bp_f64": (None, [fp, i, i, d, d, d, i, dp]),
+        "orc_bp_f64_rows": (None, [fp, i, i, d, d, d, i, C.POINTER(i), i, dp]),
         "orc_fbp_batch": (i, [fp, i, i, i, i, i, f, f, fp, f, fp, i]),
         "orc_ring_filter": (None, [fp, i, i, i]),
         "orc_ag_score": (d, [fp, i, i, fp, f, f, fp]),
@@ -119,12 +121,35 @@ def bp_ctas(Pf: np.ndarray, sincos: np.ndarray, c: float, rec_cof: float = 1.0) 
     return out
 
 
+def bp_ctas_rows(Pf: np.ndarray, sincos: np.ndarray, c: float, rows, rec_cof: float = 1.0) -> np.ndarray:
+    """bp_ctas for the given slice rows only: [len(rows)][w]."""
+    P = np.ascontiguousarray(Pf, np.float32)
+    N, w = P.shape
+    r = np.ascontiguousarray(rows, np.int32)
+    out = np.zeros((r.size, w), np.float32)
+    lib().orc_bp_ctas_rows(_f(P), N, w, _f(np.ascontiguousarray(sincos, np.float32)), c, rec_cof,
+                           r.ctypes.data_as(C.POINTER(C.c_int)), r.size, _f(out))
+    return out
+
+
 def bp_f64(Pf: np.ndarray, c: float, arc_rad: float = np.pi, rec_cof: float = 1.0,
            linear: bool = False) -> np.ndarray:
     P = np.ascontiguousarray(Pf, np.float32)
     N, w = P.shape
     out = np.zeros((w, w), np.float64)
     lib().orc_bp_f64(_f(P), N, w, arc_rad, c, rec_cof, int(linear), _d(out))
+    return out
+
+
+def bp_f64_rows(Pf: np.ndarray, c: float, rows, arc_rad: float = np.pi, rec_cof: float = 1.0,
+                linear: bool = False) -> np.ndarray:
+    """bp_f64 for the given slice rows only: [len(rows)][w]."""
+    P = np.ascontiguousarray(Pf, np.float32)
+    N, w = P.shape
+    r = np.ascontiguousarray(rows, np.int32)
+    out = np.zeros((r.size, w), np.float64)
+    lib().orc_bp_f64_rows(_f(P), N, w, arc_rad, c, rec_cof, int(linear),
+                          r.ctypes.data_as(C.POINTER(C.c_int)), r.size, _d(out))
     return out
 
 

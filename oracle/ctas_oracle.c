@@ -276,23 +276,31 @@ int orc_prepare_sino(float *P, int N, int w, const float *W, int Z, int contrast
  * the buffer (never reached inside the mask) read 0. */
 void orc_bp_ctas(const float *P, int N, int w, const float *sincos, float c,
                  float recCof, float *out) {
+  orc_bp_ctas_rows(P, N, w, sincos, c, recCof, NULL, w, out);
+}
+
+/* orc_bp_ctas for the slice rows rows[0..n_rows) only (all rows when rows is
+ * NULL): out[k * w + i] is pixel (rows[k], i), the same float operations. */
+void orc_bp_ctas_rows(const float *P, int N, int w, const float *sincos, float c,
+                      float recCof, const int *rows, int n_rows, float *out) {
   const int hp = w / 2;
   const float acent = fabsf(c);
   const long total_len = (long)N * w;
-  for (long i = 0; i < w; i++) {
-    int ip2 = (int)i - hp;
-    ip2 *= ip2;
-    if ((float)i <= acent || (float)i >= (float)w - acent - 1) {
-      for (long j = 0; j < w; j++) out[j * w + i] = 0.0f;
-      continue;
-    }
-    for (long j = 0; j < w; j++) {
+  for (long k = 0; k < n_rows; k++) {
+    const long j = rows ? rows[k] : k;
+    for (long i = 0; i < w; i++) {
+      int ip2 = (int)i - hp;
+      ip2 *= ip2;
+      if ((float)i <= acent || (float)i >= (float)w - acent - 1) {
+        out[k * w + i] = 0.0f;
+        continue;
+      }
       float ijp2 = (float)(j - hp) - c;
       ijp2 *= ijp2;
       ijp2 += (float)ip2;
       const float up = (float)hp + c, dn = (float)hp - c;
       if (((float)j <= up && ijp2 >= up * up) || ((float)j >= up && ijp2 >= dn * dn)) {
-        out[j * w + i] = 0.0f;
+        out[k * w + i] = 0.0f;
         continue;
       }
       float acc = 0.0f;
@@ -305,7 +313,7 @@ void orc_bp_ctas(const float *P, int N, int w, const float *sincos, float c,
         acc += (flat >= 0 && flat < total_len) ? P[flat] : 0.0f;
       }
       if (recCof != 0.0f) acc *= recCof;
-      out[j * w + i] = acc / (float)N;
+      out[k * w + i] = acc / (float)N;
     }
   }
 }
@@ -334,20 +342,28 @@ static void sincos_exact(double th, double *s, double *c) {
  * bin centres k+1/2, zero outside the detector. */
 void orc_bp_f64(const float *P, int N, int w, double arc_rad, double c,
                 double recCof, int linear, double *out) {
+  orc_bp_f64_rows(P, N, w, arc_rad, c, recCof, linear, NULL, w, out);
+}
+
+/* orc_bp_f64 for the slice rows rows[0..n_rows) only (all rows when rows is
+ * NULL): out[k * w + i] is pixel (rows[k], i). */
+void orc_bp_f64_rows(const float *P, int N, int w, double arc_rad, double c,
+                     double recCof, int linear, const int *rows, int n_rows, double *out) {
   const int hp = w / 2;
   const double acent = fabs(c);
   const long total_len = (long)N * w;
   double *S = (double *)malloc(sizeof(double) * (size_t)N);
   double *C = (double *)malloc(sizeof(double) * (size_t)N);
   for (int p = 0; p < N; p++) sincos_exact(p * arc_rad / N, &S[p], &C[p]);
-  for (long j = 0; j < w; j++) {
+  for (long k = 0; k < n_rows; k++) {
+    const long j = rows ? rows[k] : k;
     for (long i = 0; i < w; i++) {
       const double dy = (double)(j - hp) - c;
       const double d2 = dy * dy + (double)((i - hp) * (i - hp));
       const double up = hp + c, dn = hp - c;
       if (i <= acent || i >= w - acent - 1 || (j <= up && d2 >= up * up) ||
           (j >= up && d2 >= dn * dn)) {
-        out[j * w + i] = 0.0;
+        out[k * w + i] = 0.0;
         continue;
       }
       double acc = 0.0;
@@ -367,7 +383,7 @@ void orc_bp_f64(const float *P, int N, int w, double arc_rad, double c,
         }
       }
       if (recCof != 0.0) acc *= recCof;
-      out[j * w + i] = acc / N;
+      out[k * w + i] = acc / N;
     }
   }
   free(S);

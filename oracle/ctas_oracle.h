@@ -49,8 +49,13 @@ void  orc_filter_rows(float *P, int N, int w, const float *W, int Z);
 int   orc_prepare_sino(float *P, int N, int w, const float *W, int Z, int contrast);
 void  orc_bp_ctas(const float *P, int N, int w, const float *sincos, float c,
                   float recCof, float *out);
+void  orc_bp_ctas_rows(const float *P, int N, int w, const float *sincos, float c,
+                       float recCof, const int *rows, int n_rows, float *out);
 void  orc_bp_f64(const float *P, int N, int w, double arc_rad, double c,
                  double recCof, int linear, double *out);
+void  orc_bp_f64_rows(const float *P, int N, int w, double arc_rad, double c,
+                      double recCof, int linear, const int *rows, int n_rows,
+                      double *out);
 int   orc_fbp_batch(float *sino, int S, int N, int w, int contrast, int ftype,
                     float fparam, float anarc, const float *centers,
                     float recCof, float *out, int nthreads);

@@ -438,3 +438,16 @@ def test_ag_score_matches_fft_restatement(N, w, c0):
         c = np.float32(np.float64(a) - 0.5 * (w - 1))
         ref = _ag_score_fft(oracle.bp_ctas(Pf, sc, float(c)), c, sigma)
         assert oracle.ag_score(Pf, sc, sigma, a) == pytest.approx(ref, rel=1e-3)
+
+
+def test_bp_f64_rows_is_bp_f64_restricted():
+    rng = np.random.default_rng(12)
+    P = rng.standard_normal((120, 80)).astype(np.float32)
+    for c, linear in ((0.0, False), (-2.25, False), (1.5, True)):
+        full = oracle.bp_f64(P, c, linear=linear)
+        rows = [0, 7, 40, 41, 79]
+        np.testing.assert_array_equal(oracle.bp_f64_rows(P, c, rows, linear=linear), full[rows])
+        if not linear:
+            sc = oracle.angles(120)
+            np.testing.assert_array_equal(oracle.bp_ctas_rows(P, sc, c, rows), oracle.bp_ctas(P, sc, c)[rows])
+
