@@ -43,6 +43,10 @@ class CtasError(RuntimeError):
 _lib = None
 
 
+_READ_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_long, C.POINTER(C.c_float))
+_WRITE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_long, C.POINTER(C.c_float))
+
+
 def _f32p(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
@@ -109,6 +113,7 @@ def lib() -> C.CDLL:
         "ctas_ts_info": (i, [vp, ip, ip, ip]),
         "ctas_ts_add_dev": (i, [vp, vp, i, fp, f, i, vp, vp, vp]),
         "ctas_ts_accumulate": (i, [vp, fp, i, fp, f, i, fp]),
+        "ctas_fbp_pipeline": (i, [vp, C.c_long, i, fp, i, _READ_FN, _WRITE_FN, vp, i]),
         "ctas_ring_filter": (i, [fp, i, i, i]),
         "ctas_ring_filter_dev": (i, [vp, i, i, i, i, vp]),
         "ctas_axis_scores": (i, [vp, fp, i, fp]),
@@ -313,6 +318,37 @@ class CTrec:
         return s, out
 
     # device API (torch tensors on the GPU)
+    def pipeline(self, read, write, n_slices: int, centers, batch: int = 16, ring_box: int = 0,
+                 io_threads: int = 1) -> None:
+        """ctas_fbp_pipeline: read(i) -> sinogram i ([N][w] float32) on a
+        reader thread, write(i, slice) with slice i ([w][w], a view valid
+        during the call) on a writer thread (in order for io_threads = 1,
+        concurrently otherwise); batches of `batch` overlap reading, device
+        work and writing."""
+        N, w = self.n, self.w
+        cen = np.ascontiguousarray(np.broadcast_to(np.asarray(centers, np.float32), (n_slices,)))
+
+        def rd(_u, i, ptr):
+            try:
+                a = np.ascontiguousarray(read(int(i)), np.float32)
+                if a.shape != (N, w):
+                    return 1
+                C.memmove(ptr, a.ctypes.data, a.nbytes)
+                return 0
+            except Exception:
+                return 1
+
+        def wr(_u, i, ptr):
+            try:
+                write(int(i), np.ctypeslib.as_array(ptr, (w, w)))
+                return 0
+            except Exception:
+                return 1
+
+        rd_c, wr_c = _READ_FN(rd), _WRITE_FN(wr)  # kept alive for the call
+        _check(lib().ctas_fbp_pipeline(self._h, n_slices, batch, _f32p(cen), ring_box, rd_c, wr_c, None,
+                                       io_threads))
+
     def reconstruct_dev(self, d_sino, d_out, centers, stream=None) -> None:
         S = d_sino.shape[0] if d_sino.dim() == 3 else 1
         c = np.broadcast_to(np.asarray(centers, np.float32), (S,)).copy()

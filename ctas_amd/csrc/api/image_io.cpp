@@ -1,5 +1,8 @@
 #include "image_io.h"
 
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -93,9 +96,19 @@ Map read_tiff(const std::string &path) {
   return img;
 }
 
-std::vector<Map> read_npy(const std::string &path) {
-  const std::vector<unsigned char> b = slurp(path);
-  if (b.size() < 10 || std::memcmp(b.data(), "\x93NUMPY", 6) != 0) throw_error("image", "\"" + path + "\" is not a .npy file.");
+// Shape (planes, rows, columns) and data offset of a float32 .npy file,
+// from its header alone.
+struct NpyInfo {
+  long dims[3];
+  size_t data;
+};
+
+NpyInfo npy_info(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw_error("image", "Could not open file \"" + path + "\".");
+  unsigned char b[12] = {};
+  f.read(reinterpret_cast<char *>(b), 12);
+  if (f.gcount() < 10 || std::memcmp(b, "\x93NUMPY", 6) != 0) throw_error("image", "\"" + path + "\" is not a .npy file.");
   const int major = b[6];
   size_t hlen, hoff;
   if (major == 1) {
@@ -105,7 +118,10 @@ std::vector<Map> read_npy(const std::string &path) {
     hlen = b[8] | (b[9] << 8) | (size_t(b[10]) << 16) | (size_t(b[11]) << 24);
     hoff = 12;
   }
-  const std::string h(reinterpret_cast<const char *>(b.data()) + hoff, hlen);
+  std::string h(hlen, ' ');
+  f.seekg(static_cast<std::streamoff>(hoff));
+  f.read(&h[0], static_cast<std::streamsize>(hlen));
+  if (static_cast<size_t>(f.gcount()) != hlen) throw_error("image", "\"" + path + "\": truncated header.");
   if (h.find("'<f4'") == std::string::npos) throw_error("image", "\"" + path + "\": only little-endian float32 .npy is supported.");
   if (h.find("'fortran_order': True") != std::string::npos) throw_error("image", "\"" + path + "\": Fortran order is not supported.");
   const size_t a = h.find('('), z = h.find(')');
@@ -116,13 +132,20 @@ std::vector<Map> read_npy(const std::string &path) {
     if (tok.find_first_not_of(" ") != std::string::npos) dims.push_back(std::stol(tok));
   if (dims.size() == 2) dims.insert(dims.begin(), 1);
   if (dims.size() != 3) throw_error("image", "\"" + path + "\": expected a 2-D or 3-D array.");
-  const size_t plane = static_cast<size_t>(dims[1]) * dims[2];
-  const size_t data = hoff + hlen;
-  if (data + plane * dims[0] * 4 > b.size()) throw_error("image", "\"" + path + "\": truncated data.");
-  std::vector<Map> out(dims[0]);
-  for (long s = 0; s < dims[0]; ++s) {
-    out[s].resize(dims[1], dims[2]);
-    std::memcpy(out[s].data(), b.data() + data + plane * s * 4, plane * 4);
+  f.seekg(0, std::ios::end);
+  const size_t size = static_cast<size_t>(f.tellg());
+  NpyInfo info{{dims[0], dims[1], dims[2]}, hoff + hlen};
+  if (info.data + static_cast<size_t>(dims[0]) * dims[1] * dims[2] * 4 > size)
+    throw_error("image", "\"" + path + "\": truncated data.");
+  return info;
+}
+
+std::vector<Map> read_npy(const std::string &path) {
+  const NpyInfo info = npy_info(path);
+  std::vector<Map> out(info.dims[0]);
+  for (long s = 0; s < info.dims[0]; ++s) {
+    out[s].resize(info.dims[1], info.dims[2]);
+    read_plane({path, s}, out[s].shape(), out[s].data());
   }
   return out;
 }
@@ -137,13 +160,84 @@ void put32(std::vector<unsigned char> &v, uint32_t x) {
 
 }  // namespace
 
+std::vector<ImagePlane> index_images(const std::vector<std::string> &paths, Shape2 &shape) {
+  std::vector<ImagePlane> planes;
+  shape = {0, 0};
+  for (const std::string &p : paths) {
+    Shape2 sh;
+    long n = 1;
+    if (ends_with(p, ".npy")) {
+      const NpyInfo info = npy_info(p);
+      n = info.dims[0];
+      sh = {info.dims[1], info.dims[2]};
+    } else {
+      sh = read_tiff(p).shape();
+    }
+    if (planes.empty()) shape = sh;
+    else if (sh != shape) throw_error("image", "All images must have the same shape (\"" + p + "\" differs).");
+    for (long k = 0; k < n; ++k) planes.push_back({p, k});
+  }
+  return planes;
+}
+
+void read_plane(const ImagePlane &p, const Shape2 &shape, float *dst) {
+  const size_t elems = static_cast<size_t>(shape[0]) * shape[1];
+  if (ends_with(p.path, ".npy")) {
+    const NpyInfo info = npy_info(p.path);
+    if (info.dims[1] != shape[0] || info.dims[2] != shape[1] || p.plane >= info.dims[0])
+      throw_error("image", "\"" + p.path + "\" changed shape.");
+    std::ifstream f(p.path, std::ios::binary);
+    f.seekg(static_cast<std::streamoff>(info.data + elems * 4 * p.plane));
+    f.read(reinterpret_cast<char *>(dst), static_cast<std::streamsize>(elems * 4));
+    if (static_cast<size_t>(f.gcount()) != elems * 4) throw_error("image", "\"" + p.path + "\": truncated data.");
+    return;
+  }
+  const Map m = read_tiff(p.path);
+  if (m.shape() != shape) throw_error("image", "\"" + p.path + "\" changed shape.");
+  std::memcpy(dst, m.data(), elems * 4);
+}
+
+NpyWriter::NpyWriter(const std::string &path, long count, long rows, long cols)
+    : plane_bytes_(static_cast<size_t>(rows) * cols * 4), path_(path) {
+  fd_ = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  if (fd_ < 0) throw_error("image", "Could not write file \"" + path + "\".");
+  std::string h = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(count) + ", " +
+                  std::to_string(rows) + ", " + std::to_string(cols) + "), }";
+  while ((10 + h.size() + 1) % 64) h += ' ';
+  h += '\n';
+  std::string head("\x93NUMPY\x01\x00", 8);
+  head += static_cast<char>(h.size() & 255);
+  head += static_cast<char>(h.size() >> 8);
+  head += h;
+  data_ = head.size();
+  if (::pwrite(fd_, head.data(), head.size(), 0) != static_cast<ssize_t>(head.size()))
+    throw_error("image", "Could not write file \"" + path + "\".");
+}
+
+NpyWriter::~NpyWriter() {
+  if (fd_ >= 0) ::close(fd_);
+}
+
+void NpyWriter::write(long index, const float *plane) {
+  const char *p = reinterpret_cast<const char *>(plane);
+  size_t done = 0;
+  const off_t at = static_cast<off_t>(data_ + plane_bytes_ * static_cast<size_t>(index));
+  while (done < plane_bytes_) {
+    const ssize_t n = ::pwrite(fd_, p + done, plane_bytes_ - done, at + static_cast<off_t>(done));
+    if (n <= 0) throw_error("image", "writing plane " + std::to_string(index) + " of \"" + path_ + "\" failed");
+    done += static_cast<size_t>(n);
+  }
+}
+
 std::vector<Map> read_images(const std::string &path) {
   if (ends_with(path, ".npy")) return read_npy(path);
   return {read_tiff(path)};
 }
 
-void write_tiff(const std::string &path, const Map &img) {
-  const uint32_t w = static_cast<uint32_t>(img.columns()), h = static_cast<uint32_t>(img.rows());
+void write_tiff(const std::string &path, const Map &img) { write_tiff(path, img.data(), img.rows(), img.columns()); }
+
+void write_tiff(const std::string &path, const float *data, long rows, long cols) {
+  const uint32_t w = static_cast<uint32_t>(cols), h = static_cast<uint32_t>(rows);
   std::vector<unsigned char> v;
   v.insert(v.end(), {'I', 'I'});
   put16(v, 42);
@@ -175,7 +269,7 @@ void write_tiff(const std::string &path, const Map &img) {
   put32(v, 0);
   const size_t at = v.size();
   v.resize(at + static_cast<size_t>(w) * h * 4);
-  std::memcpy(v.data() + at, img.data(), static_cast<size_t>(w) * h * 4);
+  std::memcpy(v.data() + at, data, static_cast<size_t>(w) * h * 4);
   std::ofstream f(path, std::ios::binary);
   if (!f) throw_error("image", "Could not write file \"" + path + "\".");
   f.write(reinterpret_cast<const char *>(v.data()), static_cast<std::streamsize>(v.size()));

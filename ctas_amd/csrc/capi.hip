@@ -9,8 +9,12 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -905,6 +909,201 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
     for (int i = 0; i < n; ++i)
       if (stv[i]) return fail(CTAS_E_DATA, "Bad absorption data (maximum is sub-zero).");
   }
+  return CTAS_OK;
+}
+
+int ctas_fbp_pipeline(ctas_fbp *h, long n_slices, int batch, const float *centers, int ring_box,
+                      ctas_read_fn read, ctas_write_fn write, void *user, int io_threads) {
+  if (!h || !centers || !read || !write || n_slices < 0 || batch < 1 || ring_box < 0 || io_threads < 1)
+    return fail(CTAS_E_ARG, "bad arguments");
+  if (n_slices == 0) return CTAS_OK;
+  for (long i = 0; i < n_slices; ++i)
+    if (int rc = check_center(h->w, centers[i])) return rc;
+  const int B = static_cast<int>(std::min<long>(batch, n_slices));
+  const long n_batches = (n_slices + B - 1) / B;
+  const size_t in_elems = static_cast<size_t>(h->N) * h->w, out_elems = static_cast<size_t>(h->w) * h->w;
+  struct DeviceGuard {  // the handle's device for this call, the caller's afterwards
+    int prev = -1;
+    ~DeviceGuard() {
+      if (prev >= 0) (void)hipSetDevice(prev);
+    }
+  } guard;
+  CTAS_HIP_TRY(hipGetDevice(&guard.prev));
+  CTAS_HIP_TRY(hipSetDevice(h->device));
+  CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
+
+  // Two of everything: pinned host batches, device batches, events.
+  struct Res {
+    float *hin[2] = {}, *hout[2] = {}, *din[2] = {}, *dout[2] = {}, *avg = nullptr;
+    hipStream_t s_in = nullptr, s_comp = nullptr, s_out = nullptr;
+    hipEvent_t ev_in[2] = {}, ev_comp[2] = {}, ev_out[2] = {};
+    ~Res() {
+      for (hipStream_t st : {s_in, s_comp, s_out})
+        if (st) (void)hipStreamSynchronize(st);
+      for (int b = 0; b < 2; ++b) {
+        if (hin[b]) (void)hipHostFree(hin[b]);
+        if (hout[b]) (void)hipHostFree(hout[b]);
+        if (din[b]) (void)hipFree(din[b]);
+        if (dout[b]) (void)hipFree(dout[b]);
+        for (hipEvent_t e : {ev_in[b], ev_comp[b], ev_out[b]})
+          if (e) (void)hipEventDestroy(e);
+      }
+      if (avg) (void)hipFree(avg);
+      for (hipStream_t st : {s_in, s_comp, s_out})
+        if (st) (void)hipStreamDestroy(st);
+    }
+  } r;
+  for (int b = 0; b < 2; ++b) {
+    if (hipHostMalloc(reinterpret_cast<void **>(&r.hin[b]), sizeof(float) * in_elems * B) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&r.hout[b]), sizeof(float) * out_elems * B) != hipSuccess ||
+        hipMalloc(&r.din[b], sizeof(float) * in_elems * B) != hipSuccess ||
+        hipMalloc(&r.dout[b], sizeof(float) * out_elems * B) != hipSuccess)
+      return fail(CTAS_E_NOMEM, "pipeline buffer allocation failed");
+    CTAS_HIP_TRY(hipEventCreateWithFlags(&r.ev_in[b], hipEventDisableTiming));
+    CTAS_HIP_TRY(hipEventCreateWithFlags(&r.ev_comp[b], hipEventDisableTiming));
+    CTAS_HIP_TRY(hipEventCreateWithFlags(&r.ev_out[b], hipEventDisableTiming));
+  }
+  if (ring_box && hipMalloc(&r.avg, sizeof(float) * h->w * B) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "pipeline buffer allocation failed");
+  CTAS_HIP_TRY(hipStreamCreateWithFlags(&r.s_in, hipStreamNonBlocking));
+  CTAS_HIP_TRY(hipStreamCreateWithFlags(&r.s_comp, hipStreamNonBlocking));
+  CTAS_HIP_TRY(hipStreamCreateWithFlags(&r.s_out, hipStreamNonBlocking));
+
+  // Progress shared by the reader, the writer and this thread: batches read
+  // into host buffers, batches whose upload is queued, downloads queued,
+  // batches written. A failure anywhere sets `abort`.
+  std::mutex mu;
+  std::condition_variable cv;
+  long filled = -1, up_queued = -1, down_queued = -1, written = -1;
+  bool abort = false;
+  std::string err;
+  int err_code = CTAS_OK;
+  auto set_fail = [&](int code, const std::string &m) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!abort) {
+      abort = true;
+      err = m;
+      err_code = code;
+    }
+    cv.notify_all();
+  };
+  auto count = [&](long k) { return static_cast<int>(std::min<long>(B, n_slices - k * B)); };
+  // Calls f(i) for i < n: in order on this thread, or spread over
+  // io_threads threads (i = t, t + T, ...); false if any call failed.
+  auto for_each = [&](int n, const std::function<int(int)> &f) {
+    const int T = std::min(io_threads, n);
+    if (T <= 1) {
+      for (int i = 0; i < n; ++i)
+        if (f(i) != 0) return false;
+      return true;
+    }
+    std::atomic<bool> ok{true};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t)
+      pool.emplace_back([&, t] {
+        for (int i = t; i < n && ok.load(); i += T)
+          if (f(i) != 0) ok = false;
+      });
+    for (std::thread &th : pool) th.join();
+    return ok.load();
+  };
+
+  std::thread reader([&] {
+    (void)hipSetDevice(h->device);
+    for (long k = 0; k < n_batches; ++k) {
+      const int b = static_cast<int>(k & 1);
+      {
+        // the upload of batch k - 2 from this host buffer is queued ...
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return abort || up_queued >= k - 2; });
+        if (abort) return;
+      }
+      // ... and finished
+      if (k >= 2 && hipEventSynchronize(r.ev_in[b]) != hipSuccess) return set_fail(CTAS_E_DEVICE, "upload failed");
+      std::atomic<long> bad{-1};
+      if (!for_each(count(k), [&](int i) {
+            const int rc = read(user, k * B + i, r.hin[b] + in_elems * i);
+            if (rc != 0) bad = k * B + i;
+            return rc;
+          }))
+        return set_fail(CTAS_E_IO, "reading sinogram " + std::to_string(bad.load()) + " failed");
+      std::lock_guard<std::mutex> lk(mu);
+      filled = k;
+      cv.notify_all();
+    }
+  });
+  std::thread writer([&] {
+    (void)hipSetDevice(h->device);
+    for (long k = 0; k < n_batches; ++k) {
+      const int b = static_cast<int>(k & 1);
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return abort || down_queued >= k; });
+        if (abort) return;
+      }
+      if (hipEventSynchronize(r.ev_out[b]) != hipSuccess) return set_fail(CTAS_E_DEVICE, "download failed");
+      std::atomic<long> bad{-1};
+      if (!for_each(count(k), [&](int i) {
+            const int rc = write(user, k * B + i, r.hout[b] + out_elems * i);
+            if (rc != 0) bad = k * B + i;
+            return rc;
+          }))
+        return set_fail(CTAS_E_IO, "writing slice " + std::to_string(bad.load()) + " failed");
+      std::lock_guard<std::mutex> lk(mu);
+      written = k;
+      cv.notify_all();
+    }
+  });
+
+  auto run = [&]() -> int {
+    for (long k = 0; k < n_batches; ++k) {
+      const int b = static_cast<int>(k & 1), n = count(k);
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return abort || filled >= k; });
+        if (abort) return CTAS_OK;
+      }
+      // upload once the compute that last read this device buffer is done
+      if (k >= 2) CTAS_HIP_TRY(hipStreamWaitEvent(r.s_in, r.ev_comp[b], 0));
+      CTAS_HIP_TRY(hipMemcpyAsync(r.din[b], r.hin[b], sizeof(float) * in_elems * n, hipMemcpyHostToDevice, r.s_in));
+      CTAS_HIP_TRY(hipEventRecord(r.ev_in[b], r.s_in));
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        up_queued = k;
+        cv.notify_all();
+      }
+      // the previous batch's compute is done: its deAbs status can be read
+      // before this batch overwrites it
+      CTAS_HIP_TRY(hipStreamSynchronize(r.s_comp));
+      if (k >= 1)
+        if (int rc = read_status(h, count(k - 1))) return rc;
+      CTAS_HIP_TRY(hipStreamWaitEvent(r.s_comp, r.ev_in[b], 0));
+      if (k >= 2) CTAS_HIP_TRY(hipStreamWaitEvent(r.s_comp, r.ev_out[b], 0));  // d_out[b] downloaded
+      if (ring_box) CTAS_HIP_TRY(launch_ring(r.din[b], r.avg, n, h->N, h->w, ring_box, r.s_comp));
+      if (int rc = enqueue_slices(h, r.din[b], r.dout[b], nullptr, centers + k * B, n, true, r.s_comp)) return rc;
+      CTAS_HIP_TRY(hipEventRecord(r.ev_comp[b], r.s_comp));
+      {
+        // the writer is done with the host buffer of batch k - 2
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return abort || written >= k - 2; });
+        if (abort) return CTAS_OK;
+      }
+      CTAS_HIP_TRY(hipStreamWaitEvent(r.s_out, r.ev_comp[b], 0));
+      CTAS_HIP_TRY(hipMemcpyAsync(r.hout[b], r.dout[b], sizeof(float) * out_elems * n, hipMemcpyDeviceToHost, r.s_out));
+      CTAS_HIP_TRY(hipEventRecord(r.ev_out[b], r.s_out));
+      std::lock_guard<std::mutex> lk(mu);
+      down_queued = k;
+      cv.notify_all();
+    }
+    CTAS_HIP_TRY(hipStreamSynchronize(r.s_comp));
+    return read_status(h, count(n_batches - 1));
+  };
+  const int rc = run();
+  if (rc != CTAS_OK) set_fail(rc, ctas_last_error());
+  reader.join();
+  writer.join();
+  if (abort) return fail(err_code, err);
+  h->kept_slices = 0;  // the filtered sinograms of the last batch are not kept for repeat()
   return CTAS_OK;
 }
 

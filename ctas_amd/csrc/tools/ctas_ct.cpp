@@ -1,10 +1,14 @@
 // ctas-ct: CT reconstruction of sinogram(s), the drop-in for the reference's
 // `ctas ct` (src/executables/ct.cpp:43-244): same options and defaults; the
-// slices go to the GPU in batches instead of one per CPU thread.
+// slices go to the GPU in batches instead of one per CPU thread, with the
+// reading of the next batch and the writing of the previous one overlapping
+// the device work (ctas_fbp_pipeline).
 #include <getopt.h>
 
+#include <atomic>
 #include <cstdlib>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -30,6 +34,7 @@ void usage(const char *cmd) {
             << "      --interp ctas|linear  detector sampling (default ctas, the reference's)\n"
             << "      --store fp32|fp16|bf16  filtered-sinogram storage (default fp32)\n"
             << "      --batch INT         slices per GPU launch (default 16)\n"
+            << "      --io-threads INT    threads reading / writing the slices of a batch (default 8)\n"
             << "  -v, --verbose\n";
 }
 
@@ -44,7 +49,7 @@ std::string upgrade(const std::string &path, const std::string &prefix) {
 int main(int argc, char **argv) {
   std::string out, contrast = "ABS", filter = "RAMP", interp = "ctas", store = "fp32";
   float center = 0, arc = 180, dd = 0, lambda = 0;
-  int ring = 0, batch = 16;
+  int ring = 0, batch = 16, io_threads = 8;
   bool verbose = false, have_out = false;
   static option opts[] = {{"output", required_argument, nullptr, 'o'},   {"contrast", required_argument, nullptr, 'k'},
                           {"center", required_argument, nullptr, 'c'},   {"arcan", required_argument, nullptr, 'a'},
@@ -53,7 +58,8 @@ int main(int argc, char **argv) {
                           {"min", required_argument, nullptr, 'm'},      {"max", required_argument, nullptr, 'M'},
                           {"int", required_argument, nullptr, 'i'},      {"cpu", no_argument, nullptr, 1},
                           {"interp", required_argument, nullptr, 2},     {"store", required_argument, nullptr, 3},
-                          {"batch", required_argument, nullptr, 4},      {"verbose", no_argument, nullptr, 'v'},
+                          {"batch", required_argument, nullptr, 4},      {"io-threads", required_argument, nullptr, 5},
+                          {"verbose", no_argument, nullptr, 'v'},
                           {"help", no_argument, nullptr, 'h'},           {nullptr, 0, nullptr, 0}};
   int ch;
   while ((ch = getopt_long(argc, argv, "o:k:c:a:f:r:w:R:m:M:i:vh", opts, nullptr)) != -1) {
@@ -81,6 +87,7 @@ int main(int argc, char **argv) {
       case 2: interp = optarg; break;
       case 3: store = optarg; break;
       case 4: batch = std::max(1, std::atoi(optarg)); break;
+      case 5: io_threads = std::max(1, std::atoi(optarg)); break;
       case 'v': verbose = true; break;
       default: usage(argv[0]); return ch == 'h' ? 0 : 1;
     }
@@ -91,13 +98,10 @@ int main(int argc, char **argv) {
   }
   if (arc <= 0.0f) { std::cerr << "CT arc (given by -a) must be strictly positive.\n"; return 1; }
   try {
-    std::vector<Map> sinos;
-    for (int k = optind; k < argc; ++k)
-      for (Map &m : read_images(argv[k])) sinos.push_back(std::move(m));
+    const std::vector<std::string> paths(argv + optind, argv + argc);
+    Shape2 ish;
+    const std::vector<ImagePlane> planes = index_images(paths, ish);
     if (!have_out) out = upgrade(argv[optind], "reconstructed-");
-    const Shape2 ish = sinos.at(0).shape();
-    for (const Map &m : sinos)
-      if (m.shape() != ish) throw_error("ct", "All sinograms must have the same shape.");
     int flags = 0;
     if (interp == "linear") flags |= INTERP_LINEAR;
     else if (interp != "ctas") throw_error("ct", "Unknown interpolation \"" + interp + "\".");
@@ -108,22 +112,46 @@ int main(int argc, char **argv) {
     if (!_conversion(&ft, filter)) throw_error("ct", "Bad filter \"" + filter + "\".");
     CTrec rec(ish, Contrast(contrast), arc, ft, flags);
     rec.setPhysics(dd, lambda);
-    RingFilter rf(static_cast<size_t>(ish[1]), static_cast<size_t>(ring));
-    const long S = static_cast<long>(sinos.size());
+    const long S = static_cast<long>(planes.size());
     const bool npy = out.size() > 4 && out.substr(out.size() - 4) == ".npy";
-    std::vector<Map> all;
-    for (long s0 = 0; s0 < S; s0 += batch) {
-      const long n = std::min<long>(batch, S - s0);
-      std::vector<Map> part(sinos.begin() + s0, sinos.begin() + s0 + n), slices;
-      for (Map &m : part) rf.apply(m);
-      rec.reconstruct(part, slices, std::vector<float>(n, center));
-      for (long k = 0; k < n; ++k) {
-        if (npy) all.push_back(std::move(slices[k]));
-        else write_tiff(mask_name(out, s0 + k, S), slices[k]);
+    const Shape2 osh{ish[1], ish[1]};  // full slices, as the reference's tool writes them
+    // Sinograms are read, reconstructed and written in overlapping batches
+    // (reader and writer threads around the device work).
+    struct Ctx {
+      const std::vector<ImagePlane> &planes;
+      Shape2 ish, osh;
+      std::string out;
+      long S;
+      std::unique_ptr<NpyWriter> npy;
+      bool verbose;
+      long batch;
+      std::atomic<long> done{0};
+    } ctx{planes, ish, osh, out, S, npy ? std::make_unique<NpyWriter>(out, S, osh[0], osh[1]) : nullptr, verbose,
+          batch};
+    auto read = [](void *u, long i, float *sino) -> int {
+      Ctx &c = *static_cast<Ctx *>(u);
+      try {
+        read_plane(c.planes[i], c.ish, sino);
+        return 0;
+      } catch (const CtasErr &) {
+        return 1;
       }
-      if (verbose) std::cerr << "CT'ing: " << (s0 + n) << "/" << S << "\n";
-    }
-    if (npy) write_npy(out, all);
+    };
+    auto write = [](void *u, long i, const float *slice) -> int {
+      Ctx &c = *static_cast<Ctx *>(u);
+      try {
+        if (c.npy) c.npy->write(i, slice);
+        else write_tiff(mask_name(c.out, i, c.S), slice, c.osh[0], c.osh[1]);
+        const long done = ++c.done;
+        if (c.verbose && (done % c.batch == 0 || done == c.S)) std::cerr << "CT'ing: " << done << "/" << c.S << "\n";
+        return 0;
+      } catch (const CtasErr &) {
+        return 1;
+      }
+    };
+    const std::vector<float> centers(S, center);
+    if (ctas_fbp_pipeline(rec.handle(), S, batch, centers.data(), ring, read, write, &ctx, io_threads) != CTAS_OK)
+      throw_error("ct", ctas_last_error());
   } catch (const CtasErr &) {
     return 1;
   }

@@ -33,6 +33,7 @@ typedef struct ctas_fbp ctas_fbp;
 #define CTAS_E_NOMEM -6    /* device allocation failed                      */
 #define CTAS_E_STATE -7    /* call order (e.g. repeat before sino)           */
 #define CTAS_E_AXIS -8     /* axis search failed (lct.cpp:1136-1210 throws)  */
+#define CTAS_E_IO -9       /* a reader or writer callback reported failure   */
 
 /* contrast codes = Contrast::Cntype (physical.world.h:22-27) */
 #define CTAS_CONTRAST_ABS 0
@@ -158,6 +159,25 @@ typedef struct {
  * flags shared). */
 int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int filter,
                           float filter_param, float anarc, int flags, void *stream);
+
+/* ---- pipelined reconstruction of a series --------------------------------
+ * The loop of `ctas ct` over its sinograms (RecInThread, ct.cpp:156-236:
+ * read, ring filter, reconstruct, write) as a pipeline: `read` fills
+ * sinogram `index` ([N][w] float32) on a reader thread, batches of `batch`
+ * slices are uploaded from pinned buffers, ring-filtered (`ring_box` > 0,
+ * RingFilter::apply) and reconstructed on the device at centers[index], and
+ * `write` receives slice `index` ([w][w]) on a writer thread. Reading batch
+ * k+1, computing batch k and writing batch k-1 overlap. With io_threads = 1
+ * the callbacks run in index order on one reader and one writer thread;
+ * with io_threads > 1 the slices of a batch are read (and written) by that
+ * many threads at once, in no fixed order, so the callbacks must be safe
+ * to call concurrently for different indices. Callbacks return 0 on
+ * success; any other value stops the pipeline and the call returns
+ * CTAS_E_IO. The filtered sinograms are not returned. */
+typedef int (*ctas_read_fn)(void *user, long index, float *sino);
+typedef int (*ctas_write_fn)(void *user, long index, const float *slice);
+int ctas_fbp_pipeline(ctas_fbp *h, long n_slices, int batch, const float *centers, int ring_box,
+                      ctas_read_fn read, ctas_write_fn write, void *user, int io_threads);
 
 /* ---- ring filter (RingFilter::apply, lct.cpp:295-327) ----------------- */
 int ctas_ring_filter(float *sino, int N, int w, int box);
