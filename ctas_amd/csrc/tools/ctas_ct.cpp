@@ -6,6 +6,7 @@
 #include <getopt.h>
 
 #include <atomic>
+#include <chrono>
 #include <cstdlib>
 #include <iostream>
 #include <memory>
@@ -97,6 +98,8 @@ int main(int argc, char **argv) {
     return 0;
   }
   if (arc <= 0.0f) { std::cerr << "CT arc (given by -a) must be strictly positive.\n"; return 1; }
+  const auto t_start = std::chrono::steady_clock::now();
+  auto secs = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
   try {
     const std::vector<std::string> paths(argv + optind, argv + argc);
     Shape2 ish;
@@ -150,8 +153,11 @@ int main(int argc, char **argv) {
       }
     };
     const std::vector<float> centers(S, center);
+    const double t_setup = secs();
     if (ctas_fbp_pipeline(rec.handle(), S, batch, centers.data(), ring, read, write, &ctx, io_threads) != CTAS_OK)
       throw_error("ct", ctas_last_error());
+    if (verbose)
+      std::cerr << "setup " << t_setup << " s (device and handle), reconstruction " << (secs() - t_setup) << " s\n";
   } catch (const CtasErr &) {
     return 1;
   }
