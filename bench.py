@@ -225,7 +225,23 @@ def bench_ts(torch, np, ca, rows=2048, pixels=2048, n_proj=64, reps=5):
         out[name] = {"ms_per_batch": round(ms, 3), "us_per_projection": round(ms * 1e3 / n_proj, 2),
                      "hbm_GBs": round(hbm / (ms / 1e3) / 1e9, 1),
                      "hbm_frac": round(hbm / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
-    return {"workload": f"{n_proj} projections of {rows} x {pixels}, ABS, RAMP, one plane", **out}
+    # CPU baseline: the numpy port of ts_add (FFT per row, O(n log n) like the
+    # reference's FFTW path), one thread, on a sample of the same projections.
+    import oracle
+    n_cpu = 4
+    host = d_p[:n_cpu].cpu().numpy()
+    W = oracle.filter_fill(1, 0.0, pixels)
+    R = np.zeros((rows, pixels), np.float32)
+    t0 = time.perf_counter()
+    for k in range(n_cpu):
+        _, R = oracle.ts_add_numpy(host[k], R, W, 1.0, oracle.ABS, float(ang[k]), 10)
+    dt = time.perf_counter() - t0
+    gpu_pps = n_proj / (out["plane_only"]["ms_per_batch"] / 1e3)
+    return {"workload": f"{n_proj} projections of {rows} x {pixels}, ABS, RAMP, one plane", **out,
+            "value": round(gpu_pps, 1), "unit": "projections/s",
+            "cpu_baseline": {"value": round(n_cpu / dt, 2), "unit": "projections/s", "cores": 1, "kind": "port",
+                             "sample": f"{n_cpu} projections, numpy FFT port of ts_add (oracle.ts_add_numpy), "
+                                       f"{dt:.2f} s"}}
 
 
 def cpu_baseline(d_sino, w, N, c, threads):

@@ -245,3 +245,30 @@ def ts_add(proj: np.ndarray, result: np.ndarray, W: np.ndarray, center: float, c
     if rc:
         raise ValueError(f"orc_ts_add failed ({rc})")
     return P, R
+
+
+def ts_add_numpy(proj: np.ndarray, result: np.ndarray, W: np.ndarray, center: float, contrast: int,
+                 angle: float, plane: int) -> tuple[np.ndarray, np.ndarray]:
+    """ts_add (lct.cpp:739-787) vectorised with numpy's FFT, the CPU-baseline
+    port: -log (ABS), every row filtered by an n-point real FFT with the
+    window W (FFTW's halfcomplex product for an even window), running sums
+    (REF), and the column map in float32 in the reference's order. Returns
+    (filtered projection, result) as new arrays."""
+    P = np.asarray(proj, np.float32)
+    n = P.shape[1]
+    if contrast == ABS:
+        P = -np.log(P)
+    F = np.fft.irfft(np.fft.rfft(P, axis=1) * np.asarray(W, np.float32)[: n // 2 + 1], n, axis=1).astype(np.float32)
+    if contrast == REF:
+        F = np.cumsum(F, axis=1, dtype=np.float32)
+    f32 = np.float32
+    a = f32(angle)
+    cur_sin, cur_cos = f32(np.sin(a)), f32(np.cos(a))
+    plane_cos = f32(f32(plane) - f32(center)) * cur_sin
+    rc = f32(n * 0.5 + float(f32(center)))
+    x = np.arange(n, dtype=np.float32)
+    di = ((x - rc) * cur_cos - plane_cos + rc).astype(np.float32)
+    di = np.where(di < 0, f32(0), np.where(di >= f32(n), f32(n - 1), di))
+    cols = di.astype(np.int64)
+    return F, (np.asarray(result, np.float32) + F[:, cols]).astype(np.float32)
+

@@ -123,3 +123,21 @@ def test_ts_host_argument_errors():
         ca.TsPlan(2, 9000)
     with pytest.raises(ca.CtasError, match="LDS"):
         ca.TsPlan(2, 5000)  # padded to 16384 points
+
+
+@pytest.mark.parametrize("n", [64, 100, 257])
+@pytest.mark.parametrize("contrast", [ABS, PHS, REF])
+def test_numpy_port_matches_oracle(n, contrast):
+    """The vectorised port timed as the CPU baseline against the oracle."""
+    rng = np.random.default_rng(n + contrast)
+    P = rng.uniform(0.2, 1.0, (6, n)).astype(np.float32)
+    R0 = rng.standard_normal((6, n)).astype(np.float32)
+    W = oracle.filter_fill(5, 0.0, n)
+    for angle, c, plane in ((0.0, 0.0, 0), (0.3, 3.37, 17), (np.pi, 0.0, 0), (np.pi / 2, 0.25, 4), (-0.7, -5.25, -9)):
+        a = np.float32(angle)
+        fo, ro = oracle.ts_add(P, R0, W, c, contrast, a, plane)
+        fn, rn = oracle.ts_add_numpy(P, R0, W, c, contrast, a, plane)
+        scale = float(np.abs(fo).max())
+        assert float(np.abs(fn - fo).max()) <= 1e-5 * scale
+        assert float(np.abs(rn - ro).max()) <= 1e-4 * scale + 4 * np.finfo(np.float32).eps * float(np.abs(ro).max())
+
