@@ -204,56 +204,6 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
 // conjugate factors, so that thread t ends with the filtered points
 // t + 256 j in natural order: loads and stores of the rows are coalesced and
 // no index is bit-reversed.
-constexpr float kCos16[16] = {1.0f, 0.92387953f, 0.70710678f, 0.38268343f, 0.0f, -0.38268343f, -0.70710678f,
-                              -0.92387953f, -1.0f, -0.92387953f, -0.70710678f, -0.38268343f, 0.0f, 0.38268343f,
-                              0.70710678f, 0.92387953f};
-constexpr float kSin16[16] = {0.0f, 0.38268343f, 0.70710678f, 0.92387953f, 1.0f, 0.92387953f, 0.70710678f,
-                              0.38268343f, 0.0f, -0.38268343f, -0.70710678f, -0.92387953f, -1.0f, -0.92387953f,
-                              -0.70710678f, -0.38268343f};
-
-// 4-point DFT, exp(-2 pi i/4) per step (forward) or its conjugate (INV).
-template <bool INV>
-__device__ __forceinline__ void dft4(float2 &x0, float2 &x1, float2 &x2, float2 &x3) {
-  const float2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3), a3 = csub(x1, x3);
-  x0 = cadd(a0, a2);
-  x2 = csub(a0, a2);
-  if (INV) {
-    x1 = make_float2(a1.x - a3.y, a1.y + a3.x);
-    x3 = make_float2(a1.x + a3.y, a1.y - a3.x);
-  } else {
-    x1 = make_float2(a1.x + a3.y, a1.y - a3.x);
-    x3 = make_float2(a1.x - a3.y, a1.y + a3.x);
-  }
-}
-
-// 16-point DFT in registers, natural order in and out: j = 4a + b,
-// k = c + 4d; inner DFT4 over a, factor exp(-+2 pi i b c / 16), outer DFT4
-// over b.
-template <bool INV>
-__device__ __forceinline__ void dft16(float2 (&v)[16]) {
-#pragma unroll
-  for (int b = 0; b < 4; ++b) dft4<INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);  // v[b + 4c] = u_b[c]
-#pragma unroll
-  for (int b = 1; b < 4; ++b)
-#pragma unroll
-    for (int c = 1; c < 4; ++c) {
-      const float co = kCos16[b * c], si = INV ? kSin16[b * c] : -kSin16[b * c];
-      v[b + 4 * c] = cmul(v[b + 4 * c], make_float2(co, si));
-    }
-  float2 r[16];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    float2 y0 = v[4 * c], y1 = v[4 * c + 1], y2 = v[4 * c + 2], y3 = v[4 * c + 3];
-    dft4<INV>(y0, y1, y2, y3);  // over b -> d
-    r[c] = y0;
-    r[c + 4] = y1;
-    r[c + 8] = y2;
-    r[c + 12] = y3;
-  }
-#pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = r[k];
-}
-
 // exp(-+2 pi i m / 4096) from the half-turn table tw[m] (m < 2048).
 template <bool INV>
 __device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
@@ -261,29 +211,6 @@ __device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
   float2 t = tw[m & 2047];
   if (m & 2048) t = make_float2(-t.x, -t.y);
   return INV ? make_float2(t.x, -t.y) : t;
-}
-
-// v[k] *= r^k, k = 1..15, with r^2, r^4, r^8 from squarings and the other
-// powers as products of those (at most four factors, a few float ulps).
-__device__ __forceinline__ void twiddle_powers(float2 (&v)[16], float2 r) {
-  float2 p[16];
-  p[1] = r;
-  p[2] = cmul(r, r);
-  p[4] = cmul(p[2], p[2]);
-  p[8] = cmul(p[4], p[4]);
-  p[3] = cmul(p[2], r);
-  p[5] = cmul(p[4], r);
-  p[6] = cmul(p[4], p[2]);
-  p[7] = cmul(p[4], p[3]);
-  p[9] = cmul(p[8], r);
-  p[10] = cmul(p[8], p[2]);
-  p[11] = cmul(p[8], p[3]);
-  p[12] = cmul(p[8], p[4]);
-  p[13] = cmul(p[8], p[5]);
-  p[14] = cmul(p[8], p[6]);
-  p[15] = cmul(p[8], p[7]);
-#pragma unroll
-  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], p[k]);
 }
 
 template <int ST>

@@ -144,5 +144,119 @@ __device__ __forceinline__ void line_filter(float2 *buf, const float2 *__restric
 
 }
 
+
+// ---- register-resident radix-8 / radix-16 pieces -----------------------------
+// Small DFTs on values held in registers, natural order in and out, forward
+// (exp(-2 pi i jk/n)) or inverse (INV, conjugate factors, unscaled).
+constexpr float kCos16[16] = {1.0f, 0.92387953f, 0.70710678f, 0.38268343f, 0.0f, -0.38268343f, -0.70710678f,
+                              -0.92387953f, -1.0f, -0.92387953f, -0.70710678f, -0.38268343f, 0.0f, 0.38268343f,
+                              0.70710678f, 0.92387953f};
+constexpr float kSin16[16] = {0.0f, 0.38268343f, 0.70710678f, 0.92387953f, 1.0f, 0.92387953f, 0.70710678f,
+                              0.38268343f, 0.0f, -0.38268343f, -0.70710678f, -0.92387953f, -1.0f, -0.92387953f,
+                              -0.70710678f, -0.38268343f};
+
+// 4-point DFT, exp(-2 pi i/4) per step (forward) or its conjugate (INV).
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &x0, float2 &x1, float2 &x2, float2 &x3) {
+  const float2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3), a3 = csub(x1, x3);
+  x0 = cadd(a0, a2);
+  x2 = csub(a0, a2);
+  if (INV) {
+    x1 = make_float2(a1.x - a3.y, a1.y + a3.x);
+    x3 = make_float2(a1.x + a3.y, a1.y - a3.x);
+  } else {
+    x1 = make_float2(a1.x + a3.y, a1.y - a3.x);
+    x3 = make_float2(a1.x - a3.y, a1.y + a3.x);
+  }
+}
+
+// 16-point DFT in registers, natural order in and out: j = 4a + b,
+// k = c + 4d; inner DFT4 over a, factor exp(-+2 pi i b c / 16), outer DFT4
+// over b.
+template <bool INV>
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) dft4<INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);  // v[b + 4c] = u_b[c]
+#pragma unroll
+  for (int b = 1; b < 4; ++b)
+#pragma unroll
+    for (int c = 1; c < 4; ++c) {
+      const float co = kCos16[b * c], si = INV ? kSin16[b * c] : -kSin16[b * c];
+      v[b + 4 * c] = cmul(v[b + 4 * c], make_float2(co, si));
+    }
+  float2 r[16];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float2 y0 = v[4 * c], y1 = v[4 * c + 1], y2 = v[4 * c + 2], y3 = v[4 * c + 3];
+    dft4<INV>(y0, y1, y2, y3);  // over b -> d
+    r[c] = y0;
+    r[c + 4] = y1;
+    r[c + 8] = y2;
+    r[c + 12] = y3;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = r[k];
+}
+
+// v[k] *= r^k, k = 1..15, with r^2, r^4, r^8 from squarings and the other
+// powers as products of those (at most four factors, a few float ulps).
+__device__ __forceinline__ void twiddle_powers(float2 (&v)[16], float2 r) {
+  float2 p[16];
+  p[1] = r;
+  p[2] = cmul(r, r);
+  p[4] = cmul(p[2], p[2]);
+  p[8] = cmul(p[4], p[4]);
+  p[3] = cmul(p[2], r);
+  p[5] = cmul(p[4], r);
+  p[6] = cmul(p[4], p[2]);
+  p[7] = cmul(p[4], p[3]);
+  p[9] = cmul(p[8], r);
+  p[10] = cmul(p[8], p[2]);
+  p[11] = cmul(p[8], p[3]);
+  p[12] = cmul(p[8], p[4]);
+  p[13] = cmul(p[8], p[5]);
+  p[14] = cmul(p[8], p[6]);
+  p[15] = cmul(p[8], p[7]);
+#pragma unroll
+  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], p[k]);
+}
+
+
+// 8-point DFT in registers: two 4-point DFTs of the even and odd points,
+// combined with exp(-+2 pi i k / 8).
+template <bool INV>
+__device__ __forceinline__ void dft8(float2 (&v)[8]) {
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  constexpr float h = 0.70710678f;
+  const float2 w1 = make_float2(h, INV ? h : -h), w2 = make_float2(0.0f, INV ? 1.0f : -1.0f),
+               w3 = make_float2(-h, INV ? h : -h);
+  o1 = cmul(o1, w1);
+  o2 = cmul(o2, w2);
+  o3 = cmul(o3, w3);
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+// v[k] *= r^k, k = 1..7 (powers by squarings and products).
+__device__ __forceinline__ void twiddle_powers8(float2 (&v)[8], float2 r) {
+  const float2 p2 = cmul(r, r), p4 = cmul(p2, p2), p3 = cmul(p2, r);
+  v[1] = cmul(v[1], r);
+  v[2] = cmul(v[2], p2);
+  v[3] = cmul(v[3], p3);
+  v[4] = cmul(v[4], p4);
+  v[5] = cmul(v[5], cmul(p4, r));
+  v[6] = cmul(v[6], cmul(p4, p2));
+  v[7] = cmul(v[7], cmul(p4, p3));
+}
+
 }  // namespace
 }  // namespace ctas

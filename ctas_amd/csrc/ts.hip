@@ -170,6 +170,212 @@ __global__ __launch_bounds__(256 * K) void ts_kernel(const float *proj, long lon
   }
 }
 
+// exp(-+2 pi i m / 2048) from the half-turn table tw[m] (m < 1024).
+template <bool INV>
+__device__ __forceinline__ float2 w2048(const float2 *__restrict__ tw, int m) {
+  m &= 2047;
+  float2 t = tw[m & 1023];
+  if (m & 1024) t = make_float2(-t.x, -t.y);
+  return INV ? make_float2(t.x, -t.y) : t;
+}
+
+// 2048-point lines (Z = 2048: widths 2048, and 513-1023 padded): two lines
+// of two rows each (four rows) per workgroup of 256 threads, transformed in
+// registers as 8 x 16 x 16. Thread t loads x[t + 256 j] (j < 8) of both
+// lines, takes 8-point DFTs and multiplies by w^(t k) (w = exp(-2 pi i /
+// 2048)); after an exchange through LDS, thread (line, k, t1) takes the
+// 16-point DFT over t2 of the values at t = t1 + 16 t2 and multiplies by
+// w^(8 t1 m2); after another exchange, thread (line, k, m2) takes the
+// 16-point DFT over t1 and holds X[k + 8 m2 + 128 m1], m1 < 16, where the
+// window multiplies it. Each thread's twiddles are read once from the
+// FP64-computed table and kept in registers. The inverse runs the same steps backwards with
+// conjugate factors and leaves thread t with the filtered points t + 256 j
+// in natural order: coalesced loads, two exchanges each way, no bit
+// reversal. The filtered lines then go to LDS for the resampling.
+constexpr int kA = 9, kB = 17;               // padded row lengths of the exchanges
+constexpr int kLineA = 256 * kA;             // [t][k], k < 8
+constexpr int kLineB = 128 * kB;             // [k * 16 + m2][t1]
+constexpr size_t kTs2048Lds = sizeof(float2) * (2 * kLineA + 2 * kLineB);
+
+template <int J>
+__global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long long proj_stride, int n_proj,
+                                                     TsConsts consts, float *filtered,
+                                                     float *__restrict__ result, const float *__restrict__ W,
+                                                     const float2 *__restrict__ tw, int rows, int n, int contrast) {
+  constexpr int Z = 2048;
+  extern __shared__ float2 lds[];
+  float2 *A = lds;                 // exchange 1, then the filtered lines [line][Z]
+  float2 *B = lds + 2 * kLineA;    // exchange 2
+  const int tid = threadIdx.x;
+  const int r0 = 4 * blockIdx.x;   // rows r0 .. r0 + 3
+  const int s0 = (Z - n) >> 1;
+  const bool absorb = contrast == CTAS_CONTRAST_ABS;
+  const int ln = tid >> 7, kk = (tid >> 4) & 7, lo = tid & 15;  // line, k, t1 or m2
+
+  // The row index of each of the four rows, clamped for loads that are
+  // never stored (rows past the end).
+  int rr[4];
+  bool rv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    rv[q] = r0 + q < rows;
+    rr[q] = rv[q] ? r0 + q : rows - 1;
+  }
+  float acc[4][J];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < J; ++j) acc[q][j] = result[static_cast<size_t>(rr[q]) * n + min(tid + 256 * j, n - 1)];
+
+  // window values this thread multiplies, fixed for the whole batch
+  float wv[16];
+#pragma unroll
+  for (int m1 = 0; m1 < 16; ++m1) wv[m1] = W[kk + 8 * lo + 128 * m1];
+  // this thread's twiddles, fixed for the whole batch and read from the
+  // table (each rounded once): w^(t k), k < 8, and w^(8 t1 m2), m2 < 16
+  float2 tw1[8], tw2[16];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) tw1[k] = w2048<false>(tw, tid * k);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) tw2[m] = w2048<false>(tw, 8 * lo * m);
+
+  for (int p = 0; p < n_proj; ++p) {
+    const float *base = proj + static_cast<size_t>(p) * proj_stride;
+    // pass 1: thread t, both lines, x[t + 256 j]
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const float *ra = base + static_cast<size_t>(rr[2 * l]) * n;
+      const float *rb = base + static_cast<size_t>(rr[2 * l + 1]) * n;
+      float2 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int x = tid + 256 * j - s0;
+        float a = 0.0f, c = 0.0f;
+        if (x >= 0 && x < n) {
+          a = ra[x];
+          c = rb[x];
+          if (absorb) {
+            a = -__logf(a);
+            c = -__logf(c);
+          }
+        }
+        v[j] = make_float2(rv[2 * l] ? a : 0.0f, rv[2 * l + 1] ? c : 0.0f);
+      }
+      dft8<false>(v);
+#pragma unroll
+      for (int k = 1; k < 8; ++k) v[k] = cmul(v[k], tw1[k]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) A[l * kLineA + tid * kA + k] = v[k];
+    }
+    __syncthreads();
+    // pass 2: thread (line, k, t1), the values at t = t1 + 16 t2
+    {
+      float2 v[16];
+#pragma unroll
+      for (int t2 = 0; t2 < 16; ++t2) v[t2] = A[ln * kLineA + (lo + 16 * t2) * kA + kk];
+      dft16<false>(v);
+#pragma unroll
+      for (int m2 = 1; m2 < 16; ++m2) v[m2] = cmul(v[m2], tw2[m2]);
+#pragma unroll
+      for (int m2 = 0; m2 < 16; ++m2) B[ln * kLineB + (kk * 16 + m2) * kB + lo] = v[m2];
+    }
+    __syncthreads();
+    // pass 3: thread (line, k, m2) -> X[k + 8 m2 + 128 m1]; window; inverse
+    {
+      float2 v[16];
+#pragma unroll
+      for (int t1 = 0; t1 < 16; ++t1) v[t1] = B[ln * kLineB + (kk * 16 + lo) * kB + t1];
+      dft16<false>(v);
+#pragma unroll
+      for (int m1 = 0; m1 < 16; ++m1) {
+        v[m1].x *= wv[m1];
+        v[m1].y *= wv[m1];
+      }
+      dft16<true>(v);
+#pragma unroll
+      for (int t1 = 0; t1 < 16; ++t1) B[ln * kLineB + (kk * 16 + lo) * kB + t1] = v[t1];
+    }
+    __syncthreads();
+    // inverse pass 2: thread (line, k, t1)
+    {
+      float2 v[16];
+#pragma unroll
+      for (int m2 = 0; m2 < 16; ++m2) v[m2] = B[ln * kLineB + (kk * 16 + m2) * kB + lo];
+#pragma unroll
+      for (int m2 = 1; m2 < 16; ++m2) v[m2] = cmulc(v[m2], tw2[m2]);
+      dft16<true>(v);
+#pragma unroll
+      for (int t2 = 0; t2 < 16; ++t2) A[ln * kLineA + (lo + 16 * t2) * kA + kk] = v[t2];
+    }
+    __syncthreads();
+    // inverse pass 1: thread t -> Z x the filtered points t + 256 j
+    float2 y[2][8];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) y[l][k] = A[l * kLineA + tid * kA + k];
+#pragma unroll
+      for (int k = 1; k < 8; ++k) y[l][k] = cmulc(y[l][k], tw1[k]);
+      dft8<true>(y[l]);
+    }
+    __syncthreads();  // exchange 1 is read; the lines go in its place
+    constexpr float scale0 = 1.0f / static_cast<float>(Z);
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) A[l * Z + tid + 256 * j] = y[l][j];
+    __syncthreads();
+    float scale = scale0;
+    if (contrast == CTAS_CONTRAST_REF) {
+      // std::partial_sum of each filtered row (lct.cpp:774-775), one lane
+      // per row in sequential order, written back into its line.
+      if ((tid & 63) == 0) {
+        const int q = tid >> 6;  // row of the workgroup
+        float *line = reinterpret_cast<float *>(A + (q >> 1) * Z) + (q & 1);
+        float run = 0.0f;
+        for (int x = 0; x < n; ++x) {
+          const float f = line[2 * (s0 + x)] * scale0;
+          run = (x == 0) ? f : run + f;
+          line[2 * (s0 + x)] = run;
+        }
+      }
+      __syncthreads();
+      scale = 1.0f;
+    }
+    if (filtered) {
+      float *fb = filtered + static_cast<size_t>(p) * proj_stride;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!rv[q]) continue;
+        const float *line = reinterpret_cast<const float *>(A + (q >> 1) * Z) + (q & 1);
+        for (int x = tid; x < n; x += 256) fb[static_cast<size_t>(r0 + q) * n + x] = line[2 * (s0 + x)] * scale;
+      }
+    }
+    const float4 kc = consts.k[p];  // {cos a, P, R, -}
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = min(tid + 256 * j, n - 1);
+      const int i = ts_column(x, kc.x, kc.y, kc.z, n);
+      const float2 a = A[s0 + i], b = A[Z + s0 + i];
+      acc[0][j] = acc[0][j] + a.x * scale;
+      acc[1][j] = acc[1][j] + a.y * scale;
+      acc[2][j] = acc[2][j] + b.x * scale;
+      acc[3][j] = acc[3][j] + b.y * scale;
+    }
+    __syncthreads();  // the next projection overwrites exchange 1 and the lines
+  }
+
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!rv[q]) continue;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = tid + 256 * j;
+      if (x < n) result[static_cast<size_t>(r0 + q) * n + x] = acc[q][j];
+    }
+  }
+}
+
 }  // namespace
 
 hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
@@ -177,6 +383,17 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
                      int logZ, int contrast, int cus, int lds_max, hipStream_t s) {
   if (rows <= 0 || n_proj <= 0) return hipSuccess;
   if (n_proj > kTsBatch || n > kTsMaxWidth || Z > kTsMaxWidth) return hipErrorInvalidValue;
+  if (Z == 2048 && kTs2048Lds <= static_cast<size_t>(lds_max)) {
+    const dim3 grid((rows + 3) / 4), block(256);
+    const int J = (n + 255) / 256;  // n <= 2048: J <= 8
+    if (J <= 4)
+      hipLaunchKernelGGL(ts2048_kernel<4>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
+                         result, W, tw, rows, n, contrast);
+    else
+      hipLaunchKernelGGL(ts2048_kernel<8>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
+                         result, W, tw, rows, n, contrast);
+    return hipGetLastError();
+  }
   // Projections in flight per workgroup (K): up to 4, at most the batch, as
   // long as every workgroup of the launch stays resident (160 KiB of LDS per
   // CU; a second round of workgroups would wait for the first to finish the

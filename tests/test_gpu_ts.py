@@ -51,7 +51,8 @@ def _check(filt_dev, res_dev, filt_ref, res_ref, tag, tol_f=1e-5):
     assert em <= 1e-4
 
 
-@pytest.mark.parametrize("rows,n", [(1, 64), (7, 256), (16, 300), (5, 257), (12, 1000), (4, 2048)])
+@pytest.mark.parametrize("rows,n", [(1, 64), (7, 256), (16, 300), (5, 257), (12, 1000), (4, 2048), (7, 2048),
+                                    (6, 700)])
 @pytest.mark.parametrize("contrast", [ca.ABS, ca.PHS, ca.REF])
 def test_ts_add_matches_oracle(rows, n, contrast):
     P = _proj(rows, n, rows * 1000 + n, contrast)
@@ -94,12 +95,14 @@ def test_ts_add_widest_lines(rows, n):
     _check(Pd, Rd, fref, rref, f"n={n}")
 
 
-@pytest.mark.parametrize("rows,n,n_proj", [(9, 200, 3), (9, 200, 70), (3, 2000, 3), (2, 4000, 2)])
+@pytest.mark.parametrize("rows,n,n_proj", [(9, 200, 3), (9, 200, 70), (3, 2000, 3), (2, 4000, 2), (6, 2048, 3),
+                                           (5, 1000, 66)])
 @pytest.mark.parametrize("contrast", [ca.ABS, ca.REF])
 def test_ts_batch_equals_successive_calls(torch_cuda, rows, n, n_proj, contrast):
-    """Batches run 4 (lines of up to 2048 points), 2 (4096) or 1 (8192)
-    projections per workgroup at once, with short last groups and a split
-    over launches past 64 projections."""
+    """Batches run 4 (lines of up to 1024 points), 2 (4096) or 1 (8192)
+    projections per workgroup at once, or, for 2048-point lines, four rows
+    per workgroup through the register FFT; short last groups, partial row
+    groups and a split over launches past 64 projections."""
     torch = torch_cuda
     Ps = np.stack([_proj(rows, n, 100 + i, contrast) for i in range(n_proj)])
     angles = np.linspace(-0.6, 0.6, n_proj).astype(np.float32)
