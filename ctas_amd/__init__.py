@@ -114,6 +114,8 @@ def lib() -> C.CDLL:
         "ctas_ts_add_dev": (i, [vp, vp, i, fp, f, i, vp, vp, vp]),
         "ctas_ts_accumulate": (i, [vp, fp, i, fp, f, i, fp]),
         "ctas_fbp_pipeline": (i, [vp, C.c_long, i, fp, i, _READ_FN, _WRITE_FN, vp, i]),
+        "ctas_denoise": (i, [fp, i, i, i, f]),
+        "ctas_denoise_dev": (i, [vp, vp, i, i, i, f, vp]),
         "ctas_ring_filter": (i, [fp, i, i, i]),
         "ctas_ring_filter_dev": (i, [vp, i, i, i, i, vp]),
         "ctas_axis_scores": (i, [vp, fp, i, fp]),
@@ -397,6 +399,18 @@ def batch_uneven(items, contrast=ABS, filt=1, param=0.0, anarc=180.0, flags=0, s
         arr[k] = _Item(s.shape[1], s.shape[0], float(c), _ptr(s), _ptr(o))
     _check(lib().ctas_fbp_batch_uneven(arr, len(items), contrast, filt, param, anarc, flags,
                                        _stream_ptr(stream)))
+
+
+def denoise(img: np.ndarray, rad: int, threshold: float = 0.0) -> np.ndarray:
+    """Denoiser(shape, rad, threshold).proc (matrix.world.cpp:952-1037) on a copy."""
+    a = np.array(img, np.float32, copy=True, order="C")
+    _check(lib().ctas_denoise(_f32p(a), a.shape[0], a.shape[1], rad, threshold))
+    return a
+
+
+def denoise_dev(d_in, d_out, rad: int, threshold: float = 0.0, stream=None) -> None:
+    _check(lib().ctas_denoise_dev(_ptr(d_in), _ptr(d_out), d_in.shape[-2], d_in.shape[-1], rad, threshold,
+                                  _current_stream(stream)))
 
 
 def ring_filter(sino: np.ndarray, box: int) -> np.ndarray:

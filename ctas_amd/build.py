@@ -28,7 +28,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CTAS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["window.cpp", "filter.hip", "backproject.hip", "ring.hip", "axis.hip",
-               "phantom.hip", "cnn.hip", "ts.hip", "capi.hip"]
+               "phantom.hip", "cnn.hip", "ts.hip", "denoise.hip", "capi.hip"]
 # Packed-FP32 instructions are disabled for all device code.
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

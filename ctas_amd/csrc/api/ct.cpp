@@ -183,6 +183,12 @@ void ts_add(Map &projection, Map &result, const Filter &filter, const float cent
         "tomosynthesis");
 }
 
+void Denoiser::proc(Map &iom) const {
+  if (!sh_[0] || !sh_[1] || !rad_) return;
+  if (iom.shape() != sh_) throw_error("denoiser", "Non matching shape of input array.");
+  check(ctas_denoise(iom.data(), static_cast<int>(sh_[0]), static_cast<int>(sh_[1]), rad_, thr_), "denoiser");
+}
+
 float raxis(Map &sino, float anarc, float maxDev, int algo) {
   float r = 0.0f;
   check(ctas_raxis(sino.data(), static_cast<int>(sino.rows()), static_cast<int>(sino.columns()), anarc, maxDev, algo,

@@ -162,6 +162,19 @@ class RingFilter {
 void ts_add(Map &projection, Map &result, const Filter &filter, const float center, const Contrast contrast,
             const float angle, const int plane);
 
+// Denoiser (matrix.world.h:468-481) without the mask: proc() replaces
+// pixels by their disc mean under the threshold rule, on the GPU.
+class Denoiser {
+ public:
+  Denoiser(const Shape2 &sh, int rad, float threshold) : sh_(sh), rad_(rad < 0 ? -rad : rad), thr_(threshold) {}
+  void proc(Map &iom) const;
+
+ private:
+  Shape2 sh_;
+  int rad_;
+  float thr_;
+};
+
 float raxis(Map &sino, float anarc, float maxDev, int algo = 0);
 float raxis(Map &proj0, Map &proj180, float maxDev);
 

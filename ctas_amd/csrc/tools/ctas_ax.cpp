@@ -35,20 +35,16 @@ int main(int argc, char **argv) {
       case 'm': maxShift = std::strtof(optarg, nullptr); break;
       case 'v': break;
       default:
-        std::cout << "Usage: " << argv[0] << " [-a ARC] [-m MAXSHIFT] [-o MASK] sinogram | proj0 proj180\n";
+        std::cout << "Usage: " << argv[0] << " [-a ARC] [-m MAXSHIFT] [-o MASK] [-n RAD] [-N THR] sinogram | proj0 proj180\n";
         return ch == 'h' ? 0 : 1;
     }
   }
   if (optind >= argc) {
-    std::cout << "Usage: " << argv[0] << " [-a ARC] [-m MAXSHIFT] [-o MASK] sinogram | proj0 proj180\n";
+    std::cout << "Usage: " << argv[0] << " [-a ARC] [-m MAXSHIFT] [-o MASK] [-n RAD] [-N THR] sinogram | proj0 proj180\n";
     return 0;
   }
   if (arc <= 0.0f) { std::cerr << "CT arc (given by -a): must be strictly positive.\n"; return 1; }
   if (maxShift <= 0.0f) { std::cerr << "Impossible maximum shift " << maxShift << ": must be strictly positive.\n"; return 1; }
-  if (denoiseRad || denoiseThr != 0.0f) {
-    std::cerr << "The denoiser (-n/-N) is not part of this engine.\n";
-    return 1;
-  }
   const int nimg = argc - optind;
   try {
     Map img0 = read_images(argv[optind]).at(0);
@@ -64,7 +60,10 @@ int main(int argc, char **argv) {
         std::cerr << "Failed to calculate COR.\n" << std::flush;
       }
     };
+    // axis.cpp:139-142, 176-180: the images go through the denoiser first
+    const Denoiser dnz(ish, denoiseRad, denoiseThr);
     if (nimg == 1) {
+      if (denoiseRad) dnz.proc(img0);
       const float ang = arc / (std::fabs(arc) <= 1.0f ? 1.0f : static_cast<float>(ish[0] - 1));
       CTrec *rec = nullptr;
       Map recr;
@@ -86,6 +85,10 @@ int main(int argc, char **argv) {
     } else if (nimg == 2) {
       Map p180 = read_images(argv[optind + 1]).at(0);
       if (p180.shape() != ish) throw_error(argv[0], "Different shapes of input projections.");
+      if (denoiseRad) {
+        dnz.proc(img0);
+        dnz.proc(p180);
+      }
       rotax([&] { return raxis(img0, p180, maxShift); });
     } else {
       throw_error(argv[0], "Can accept only one (sinogram) or two (opposit projections) images.");

@@ -219,6 +219,17 @@ int ctas_ts_add_dev(ctas_ts *h, const float *d_proj, int n_proj, const float *an
 int ctas_ts_accumulate(ctas_ts *h, const float *projections, int n_proj, const float *angles,
                        float center, int plane, float *result);
 
+/* ---- denoiser (Denoiser, matrix.world.cpp:952-1037; `ax -n/-N`) -------- */
+/* Denoiser(shape, rad, threshold).proc(image) without a mask: the disc mean
+ * of radius |rad| (offsets inside the image only) replaces each pixel
+ * always (threshold 0) or where the pixel is 0 or deviates from it by more
+ * than threshold (absolute, > 0) or -threshold relative to the mean (< 0).
+ * rad 0 leaves the image unchanged. */
+int ctas_denoise(float *image, int rows, int cols, int rad, float threshold);
+/* The same on device arrays (d_in and d_out distinct), on `stream`. */
+int ctas_denoise_dev(const float *d_in, float *d_out, int rows, int cols, int rad, float threshold,
+                     void *stream);
+
 /* ---- rotation axis (lct.cpp:804-1326, axis.cpp:116-208) -------------- */
 /* AverageGradient::apply for n trial axis positions axisR (pixels from the
  * left edge) on the sinogram kept by ctas_fbp_sino of an ABS/RAMP handle. */

@@ -54,6 +54,7 @@ def lib(fast: bool = False) -> C.CDLL:
         "orc_raxis": (i, [fp, i, i, f, f, i, fp]),
         "orc_raxis_pair": (i, [fp, fp, i, i, f, fp]),
         "orc_ts_add": (i, [fp, fp, i, i, fp, f, i, f, i]),
+        "orc_denoise": (i, [fp, i, i, i, f]),
         "orc_phantom_sino": (None, [i, i, d, i, dp, d, C.c_ulonglong, fp]),
     }
     for n, (r, a) in sig.items():
@@ -271,4 +272,12 @@ def ts_add_numpy(proj: np.ndarray, result: np.ndarray, W: np.ndarray, center: fl
     di = np.where(di < 0, f32(0), np.where(di >= f32(n), f32(n - 1), di))
     cols = di.astype(np.int64)
     return F, (np.asarray(result, np.float32) + F[:, cols]).astype(np.float32)
+
+
+def denoise(img: np.ndarray, rad: int, thr: float = 0.0) -> np.ndarray:
+    """Denoiser(shape, rad, thr).proc on a copy (no mask)."""
+    a = np.array(img, np.float32, copy=True, order="C")
+    if lib().orc_denoise(_f(a), a.shape[0], a.shape[1], rad, thr):
+        raise ValueError("orc_denoise failed")
+    return a
 

@@ -988,3 +988,48 @@ int orc_ts_add(float *proj, float *result, int rows, int n, const float *W, floa
   free(y);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* denoiser                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Denoiser (matrix.world.cpp:952-1037) without a mask, in the reference's
+ * array form: for every offset (i, j) of the disc, taken in order (i outer,
+ * j inner, from -rad), tarr(dst) += iom(src) over the overlapping ranges
+ * dst = [max(0, i), size - 1 + min(0, i)], src = dst - (i, j)
+ * (matrix.world.h:261-272), and swghts counts the offsets; tarr is scaled by
+ * invert(swghts) = 1.0 / n (double, stored as float) and the threshold rule
+ * of proc() picks between the mean and the pixel. */
+int orc_denoise(float *a, int H, int W, int rad, float thr) {
+  if (rad < 0) rad = -rad;
+  if (H <= 0 || W <= 0 || rad == 0) return 0;
+  const size_t n = (size_t)H * W;
+  float *tarr = (float *)calloc(n, sizeof(float));
+  float *sw = (float *)calloc(n, sizeof(float));
+  if (!tarr || !sw) { free(tarr); free(sw); return -3; }
+  const unsigned r2 = (unsigned)(rad * rad);
+  for (int i = -rad; i <= rad; i++)
+    for (int j = -rad; j <= rad; j++) {
+      if ((unsigned)(i * i + j * j) > r2) continue;
+      const int y0 = i > 0 ? i : 0, y1 = H - 1 + (i < 0 ? i : 0);
+      const int x0 = j > 0 ? j : 0, x1 = W - 1 + (j < 0 ? j : 0);
+      for (int y = y0; y <= y1; y++)
+        for (int x = x0; x <= x1; x++) {
+          tarr[(size_t)y * W + x] += a[(size_t)(y - i) * W + (x - j)];
+          sw[(size_t)y * W + x] += 1.0f;
+        }
+    }
+  for (size_t k = 0; k < n; k++) {
+    const float inv = sw[k] == 0.0f ? 0.0f : (float)(1.0 / sw[k]);
+    const float t = tarr[k] * inv;
+    float *v = a + k;
+    if (thr == 0.0f) *v = t;
+    else if (thr < 0.0f) {
+      if (*v == 0.0f || (t != 0.0f && fabsf((*v - t) / t) > -thr)) *v = t;
+    } else if (*v == 0.0f || fabsf(*v - t) > thr) *v = t;
+  }
+  free(tarr);
+  free(sw);
+  return 0;
+}
+

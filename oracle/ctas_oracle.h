@@ -68,6 +68,7 @@ int   orc_raxis_pair(const float *p0, const float *p180, int rows, int w,
                      float maxDev, float *result);
 int   orc_ts_add(float *proj, float *result, int rows, int n, const float *W,
                  float center, int contrast, float angle, int plane);
+int   orc_denoise(float *a, int H, int W, int rad, float thr);
 void  orc_phantom_sino(int N, int w, double c, int n_ell, const double *ell,
                        double N0, unsigned long long seed, float *out);
 

@@ -143,3 +143,31 @@ def test_ax_tool_matches_oracle(tmp_path):
     for wv in want:
         if wv is not None:
             assert abs(next(got) - wv) <= 0.25
+
+
+@pytest.mark.gpu
+def test_ax_tool_denoiser(tmp_path):
+    """`ax -n 2 -N 0.05`: the sinogram goes through the denoiser before the
+    axis search (axis.cpp:139-142)."""
+    if ca.device_count() < 1:
+        pytest.skip("no HIP device")
+    N, w, c0 = 241, 128, -2.0
+    rng = np.random.default_rng(3)
+    dots = np.array([[x, y, 2.0, 2.0, 0.0, 2.0 / w] for x, y in rng.uniform(-0.31 * w, 0.31 * w, (8, 2))])
+    P = np.exp(-oracle.phantom_sino(N, w, c0, dots, 0.0, 0)).astype(np.float32)
+    P[rng.random(P.shape) < 0.002] = 0.0  # dead pixels for the denoiser to fill
+    np.save(tmp_path / "sino.npy", P)
+    r = run("ctas-ax", tmp_path / "sino.npy", "-n", 2, "-N", 0.05)
+    got = [float(x) for x in r.stdout.split()]
+    Pd = oracle.denoise(P, 2, 0.05)
+    step = 180.0 / (N - 1)
+    want = []
+    for algo in range(4):
+        try:
+            want.append(oracle.raxis(Pd, step, 0.25, algo) - 0.5 * (w - 1))
+        except ValueError:
+            pass
+    assert len(got) == len(want)
+    for g, wv in zip(got, want):
+        assert abs(g - wv) <= 0.25
+
