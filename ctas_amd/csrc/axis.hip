@@ -263,57 +263,144 @@ extern "C" int ctas_axis_scores(ctas_fbp *h, const float *axisR, int n, float *s
   return CTAS_OK;
 }
 
+namespace ctas {
+namespace {
+
+// ---- 0/180-degree registration (raxis(proj0, proj180, maxDev)) ----------
+// Column profile sum_r |p0 - 1| + |p180 - 1| (double terms added to a float
+// per column, rows in order), its centre of mass and the mean of the two
+// edge columns of the mirrored p180, each a sequential sum as in
+// lct.cpp:1226-1251; one workgroup, w <= 16384.
+__global__ __launch_bounds__(256) void pair_profile_kernel(const float *__restrict__ p0,
+                                                           const float *__restrict__ p180, int rows, int w,
+                                                           float *__restrict__ out) {
+  extern __shared__ float prof[];
+  for (int x = threadIdx.x; x < w; x += 256) {
+    float acc = 0.0f;
+    for (int r = 0; r < rows; ++r) {
+      const size_t i = static_cast<size_t>(r) * w + x;
+      acc = static_cast<float>(acc + (fabs(static_cast<double>(p0[i]) - 1.0) + fabs(static_cast<double>(p180[i]) - 1.0)));
+    }
+    prof[x] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double ssum = 0.0, wsum = 0.0;
+  for (int x = 0; x < w; ++x) {
+    ssum += prof[x];
+    wsum += static_cast<double>(static_cast<float>(x) * prof[x]);
+  }
+  // p180 mirrored: its first column is p180's last and vice versa
+  double a0 = 0.0, a1 = 0.0;
+  for (int r = 0; r < rows; ++r) {
+    a0 += p180[static_cast<size_t>(r) * w + (w - 1)];
+    a1 += p180[static_cast<size_t>(r) * w];
+  }
+  out[0] = static_cast<float>(ssum == 0.0 ? 0.0 : 1.0);
+  out[1] = ssum == 0.0 ? 0.0f : static_cast<float>(wsum) / static_cast<float>(ssum);
+  out[2] = (static_cast<float>(a0) + static_cast<float>(a1)) / static_cast<float>(2 * rows);
+}
+
+// Registration error of trial shift x = shifts[blockIdx.x] (calcErr of
+// lct.cpp:1255-1270): sum over rows and columns of (p180r(col - sh) - p0)^2,
+// p180r the mirrored p180, `aver` outside the overlap, each square in float
+// and summed in double; the block's partial sums are reduced in a fixed order.
+__global__ __launch_bounds__(256) void pair_err_kernel(const float *__restrict__ p0, const float *__restrict__ p180,
+                                                       int rows, int w, const float *__restrict__ shifts, float aver,
+                                                       double *__restrict__ out) {
+  __shared__ double part[256];
+  const long sh = static_cast<long>(shifts[blockIdx.x]);
+  const long lo = sh > 0 ? sh : 0, hi = w - 1 + (sh < 0 ? sh : 0);
+  const size_t n = static_cast<size_t>(rows) * w;
+  double acc = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 256) {
+    const long col = static_cast<long>(i % w), r = static_cast<long>(i / w);
+    const long src = col - sh;
+    float d = aver;
+    if (col >= lo && col <= hi && src >= 0 && src < w) d = p180[r * w + (w - 1 - src)];
+    d = d - p0[i];
+    acc += static_cast<double>(d * d);
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (static_cast<int>(threadIdx.x) < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+}  // namespace
+}  // namespace ctas
+
+using namespace ctas;
+
 // raxis(proj0, proj180, maxDev) (lct.cpp:1217-1294): registration of the
-// 0 and 180 degree projections; a few rows, evaluated on the host.
+// 0 and 180 degree projections. The profile and the error sums of every
+// trial shift run on the GPU; the sweep and the bisection are host logic.
 extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int w, float maxDev, float *result) {
   if (!p0 || !p180 || !result) return fail(CTAS_E_ARG, "null argument");
   if (maxDev < 0 || maxDev > 0.5f)
     return fail(CTAS_E_ARG, "Invalid maximum relative shift of the rotation axis " + std::to_string(maxDev) +
                                 ". Must be within [0, 0.5] range.");
   if (rows <= 0 || w <= 0) return fail(CTAS_E_ARG, "Empty input projections.");
+  if (w > 16384) return fail(CTAS_E_ARG, "projection width above 16384");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
   const float cent = static_cast<float>(0.5 * (w - 1));
-  std::vector<float> proj(w, 0.0f);
-  for (int r = 0; r < rows; ++r)
-    for (int x = 0; x < w; ++x)
-      proj[x] = static_cast<float>(proj[x] + (std::fabs(static_cast<double>(p0[static_cast<size_t>(r) * w + x]) - 1.0) +
-                                              std::fabs(static_cast<double>(p180[static_cast<size_t>(r) * w + x]) - 1.0)));
-  double ssum = 0.0, wsum = 0.0;
-  for (int x = 0; x < w; ++x) {
-    ssum += proj[x];
-    wsum += static_cast<double>(static_cast<float>(x) * proj[x]);
-  }
-  if (ssum == 0.0) return fail(CTAS_E_AXIS, "Impossible input projection averaging to zero.");
-  const float fcr = static_cast<float>(wsum) / static_cast<float>(ssum);
+  const size_t n = static_cast<size_t>(rows) * w;
   const int stpsz = 16;
   const int steps = static_cast<int>(1 + 2 * (maxDev == 0.0f ? 1.0f : maxDev * static_cast<float>(w) / stpsz));
-  std::vector<float> p180r(static_cast<size_t>(rows) * w);
-  for (int r = 0; r < rows; ++r)
-    for (int x = 0; x < w; ++x) p180r[static_cast<size_t>(r) * w + x] = p180[static_cast<size_t>(r) * w + (w - 1 - x)];
-  double a0 = 0.0, a1 = 0.0;
-  for (int r = 0; r < rows; ++r) {
-    a0 += p180r[static_cast<size_t>(r) * w];
-    a1 += p180r[static_cast<size_t>(r) * w + w - 1];
-  }
-  const float aver = (static_cast<float>(a0) + static_cast<float>(a1)) / static_cast<float>(2 * rows);
+  // device: p0, p180, profile results, trial shifts, their sums
+  const int max_trials = std::max(steps, 8) + w / stpsz + 4;
+  struct Buf {
+    void *p = nullptr;
+    ~Buf() {
+      if (p) (void)hipFree(p);
+    }
+  } buf;
+  const size_t off_res = 2 * n * sizeof(float), off_sh = off_res + 256, off_acc = off_sh + 4 * ((max_trials + 63) / 64 * 64);
+  if (hipMalloc(&buf.p, off_acc + sizeof(double) * max_trials) != hipSuccess)
+    return fail(CTAS_E_NOMEM, "device allocation failed");
+  char *base = static_cast<char *>(buf.p);
+  float *d_p0 = reinterpret_cast<float *>(base), *d_p180 = d_p0 + n;
+  float *d_res = reinterpret_cast<float *>(base + off_res), *d_sh = reinterpret_cast<float *>(base + off_sh);
+  double *d_acc = reinterpret_cast<double *>(base + off_acc);
+  CTAS_HIP_TRY(hipMemcpy(d_p0, p0, n * sizeof(float), hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(d_p180, p180, n * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(pair_profile_kernel, dim3(1), dim3(256), sizeof(float) * w, nullptr, d_p0, d_p180, rows, w,
+                     d_res);
+  CTAS_HIP_TRY(hipGetLastError());
+  float res[3];
+  CTAS_HIP_TRY(hipMemcpy(res, d_res, sizeof res, hipMemcpyDeviceToHost));
+  if (res[0] == 0.0f) return fail(CTAS_E_AXIS, "Impossible input projection averaging to zero.");
+  const float fcr = res[1], aver = res[2];
+  int dev_rc = CTAS_OK;
+  // errors of trial shifts xs[0..k), one workgroup each
+  auto errs = [&](const float *xs, int k, float *e) {
+    if (dev_rc != CTAS_OK) return;
+    std::vector<double> acc(k);
+    if (hipMemcpy(d_sh, xs, sizeof(float) * k, hipMemcpyHostToDevice) != hipSuccess) {
+      dev_rc = fail(CTAS_E_DEVICE, "registration: upload failed");
+      return;
+    }
+    hipLaunchKernelGGL(pair_err_kernel, dim3(k), dim3(256), 0, nullptr, d_p0, d_p180, rows, w, d_sh, aver, d_acc);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(acc.data(), d_acc, sizeof(double) * k, hipMemcpyDeviceToHost) != hipSuccess) {
+      dev_rc = fail(CTAS_E_DEVICE, "registration: error sums failed on the device");
+      return;
+    }
+    for (int i = 0; i < k; ++i) e[i] = std::sqrt(static_cast<float>(acc[i]));
+  };
   auto calc_err = [&](float x) -> float {
-    const long sh = static_cast<long>(x);
-    double acc = 0.0;
-    for (int r = 0; r < rows; ++r)
-      for (int col = 0; col < w; ++col) {
-        float d = aver;
-        const long src = col - sh;
-        if (col >= std::max(0L, sh) && col <= w - 1 + std::min(0L, sh) && src >= 0 && src < w)
-          d = p180r[static_cast<size_t>(r) * w + src];
-        d = d - p0[static_cast<size_t>(r) * w + col];
-        acc += static_cast<double>(d * d);
-      }
-    return std::sqrt(static_cast<float>(acc));
+    float e = 0.0f;
+    errs(&x, 1, &e);
+    return e;
   };
   std::vector<float> vax(steps), err(steps);
-  for (int ii = 0; ii < steps; ++ii) {
-    vax[ii] = 2 * (fcr - cent) + static_cast<float>(stpsz * (ii - (steps - 1) / 2));
-    err[ii] = calc_err(vax[ii]);
-  }
+  for (int ii = 0; ii < steps; ++ii) vax[ii] = 2 * (fcr - cent) + static_cast<float>(stpsz * (ii - (steps - 1) / 2));
+  errs(vax.data(), steps, err.data());
+  if (dev_rc != CTAS_OK) return dev_rc;
   int idx = -1;
   float minval = FLT_MAX;
   for (int i = 1; i < steps - 1; ++i)
@@ -321,7 +408,7 @@ extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int
   int rc = CTAS_OK;
   if (idx >= 0) {
     *result = cent + 0.5f * improve_me(1.0f, vax, err, idx, 1, calc_err, &rc);
-    return rc;
+    return dev_rc != CTAS_OK ? dev_rc : rc;
   }
   idx = err[0] < err[steps - 1] ? 0 : steps - 1;
   const int cdir = idx == 0 ? -1 : 1;
@@ -337,6 +424,7 @@ extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int
   for (int i = 0; i < maxSteps; ++i) {
     v2[i + 2] = vax[idx] + static_cast<float>(cdir * (i + 1) * stpsz);
     e2[i + 2] = calc_err(v2[i + 2]);
+    if (dev_rc != CTAS_OK) return dev_rc;
     if (e2[i + 2] < minval) {
       minval = e2[i + 2];
       idx2 = i + 2;
@@ -346,7 +434,7 @@ extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int
   }
   if (idx2 == maxSteps + 1) return fail(CTAS_E_AXIS, "Failed to calculate rotation centre.");
   *result = cent + 0.5f * improve_me(1.0f, v2, e2, idx2, cdir, calc_err, &rc);
-  return rc;
+  return dev_rc != CTAS_OK ? dev_rc : rc;
 }
 
 // raxis(sino, anarc, maxDev, algo) (lct.cpp:1133-1212). The trial

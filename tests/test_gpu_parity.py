@@ -378,3 +378,23 @@ def test_pipeline_reader_failure_is_reported():
 
     with pytest.raises(ca.CtasError, match="reading sinogram 3"):
         rec.pipeline(read, lambda i, sl: None, 6, 0.0, batch=2)
+
+
+@pytest.mark.parametrize("c0", [3.0, -5.5])
+def test_raxis_pair_matches_oracle(c0):
+    """raxis(proj0, proj180): profile and trial errors on the GPU, the search
+    on the host. The 180-degree projection is the 0-degree one mirrored
+    about the axis at (w-1)/2 + c0, so the registration recovers c0."""
+    N, w = 64, 256
+    P = np.exp(-oracle.phantom_sino(N, w, 0.0, _dots(w), 0.0, 0)).astype(np.float32)
+    s = int(round(2 * c0))
+    src = (w - 1 + s) - np.arange(w)
+    ok = (src >= 0) & (src < w)
+    p180 = np.ones_like(P)
+    p180[:, ok] = P[:, src[ok]]
+    got = ca.raxis_pair(P, p180, 0.25)
+    ref = oracle.raxis_pair(P, p180, 0.25)
+    assert abs(got - ref) <= 0.5, (got, ref)
+    assert abs(got - (0.5 * (w - 1) + c0)) <= 0.5
+    with pytest.raises(ca.CtasError, match="averaging to zero"):
+        ca.raxis_pair(np.ones((4, 32), np.float32), np.ones((4, 32), np.float32), 0.25)
