@@ -239,21 +239,42 @@ __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long lon
 #pragma unroll
   for (int m = 0; m < 16; ++m) tw2[m] = w2048<false>(tw, 8 * lo * m);
 
-  for (int p = 0; p < n_proj; ++p) {
+  // The raw samples of the next projection are loaded while the current
+  // one is transformed (x = t + 256 j - s0 inside the row; 0 outside).
+  float2 raw[2][8];
+  auto fetch = [&](int p) {
     const float *base = proj + static_cast<size_t>(p) * proj_stride;
-    // pass 1: thread t, both lines, x[t + 256 j]
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
       const float *ra = base + static_cast<size_t>(rr[2 * l]) * n;
       const float *rb = base + static_cast<size_t>(rr[2 * l + 1]) * n;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int x = tid + 256 * j - s0;
+        raw[l][j] = (x >= 0 && x < n) ? make_float2(ra[x], rb[x]) : make_float2(1.0f, 1.0f);
+      }
+    }
+  };
+  fetch(0);
+
+  for (int p = 0; p < n_proj; ++p) {
+    float2 cur[2][8];
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) cur[l][j] = raw[l][j];
+    if (p + 1 < n_proj) fetch(p + 1);
+    // pass 1: thread t, both lines, x[t + 256 j]
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
       float2 v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int x = tid + 256 * j - s0;
         float a = 0.0f, c = 0.0f;
         if (x >= 0 && x < n) {
-          a = ra[x];
-          c = rb[x];
+          a = cur[l][j].x;
+          c = cur[l][j].y;
           if (absorb) {
             a = -__logf(a);
             c = -__logf(c);
