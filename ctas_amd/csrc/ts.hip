@@ -397,6 +397,183 @@ __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long lon
   }
 }
 
+// exp(-+2 pi i m / 4096) from the half-turn table tw[m] (m < 2048).
+template <bool INV>
+__device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
+  m &= 4095;
+  float2 t = tw[m & 2047];
+  if (m & 2048) t = make_float2(-t.x, -t.y);
+  return INV ? make_float2(t.x, -t.y) : t;
+}
+
+// 4096-point lines (Z = 4096: width 4096, and 1025-2047 padded): one line
+// (two rows) per workgroup of 256 threads, transformed in registers as
+// 16 x 16 x 16, the steps of the sinogram filter's radix-16 kernel: thread t
+// holds x[t + 256 j], takes a 16-point DFT and multiplies by w^(t k); after
+// an exchange thread (t1 = t % 16, k1 = t / 16) takes the DFT over t2 of the
+// values at t1 + 16 t2 and multiplies by w^(16 t1 k2); after another, thread
+// (k1, k2) holds X[k1 + 16 k2 + 256 k3], k3 < 16, for the window; the inverse
+// runs back with conjugate factors to the filtered x[t + 256 j]. Twiddles
+// (from the FP64 table) and window values stay in registers for the batch.
+constexpr int kP16 = 17;  // padded row length of the exchanges
+constexpr size_t kTs4096Lds = sizeof(float2) * 256 * kP16;
+
+template <int J>
+__global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long long proj_stride, int n_proj,
+                                                     TsConsts consts, float *filtered,
+                                                     float *__restrict__ result, const float *__restrict__ W,
+                                                     const float2 *__restrict__ tw, int rows, int n, int contrast) {
+  constexpr int Z = 4096;
+  extern __shared__ float2 xch[];
+  const int tid = threadIdx.x, lo4 = tid & 15, hi4 = tid >> 4;
+  const int t0 = 2 * blockIdx.x;
+  const bool has1 = t0 + 1 < rows;
+  const int s0 = (Z - n) >> 1;
+  const bool absorb = contrast == CTAS_CONTRAST_ABS;
+  const size_t row0 = static_cast<size_t>(t0) * n, row1c = has1 ? row0 + n : row0;
+
+  float acc0[J], acc1[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = min(tid + 256 * j, n - 1);
+    acc0[j] = result[row0 + x];
+    acc1[j] = result[row1c + x];
+  }
+  float2 tw1[16], tw2[16];
+  float wv[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    tw1[k] = w4096<false>(tw, tid * k);
+    tw2[k] = w4096<false>(tw, 16 * lo4 * k);
+    wv[k] = W[hi4 + 16 * lo4 + 256 * k];
+  }
+
+  // The next projection's rows load during the current transform, except
+  // at 16 columns per thread and row, where the registers would not keep
+  // two waves per SIMD.
+  constexpr bool kPrefetch = J <= 8;
+  float2 raw[16];
+  auto fetch = [&](int p) {
+    const float *r0 = proj + static_cast<size_t>(p) * proj_stride + row0;
+    const float *r1 = proj + static_cast<size_t>(p) * proj_stride + row1c;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int x = tid + 256 * j - s0;
+      raw[j] = (x >= 0 && x < n) ? make_float2(r0[x], r1[x]) : make_float2(1.0f, 1.0f);
+    }
+  };
+  if (kPrefetch) fetch(0);
+
+  for (int p = 0; p < n_proj; ++p) {
+    if (!kPrefetch) fetch(p);
+    float2 v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int x = tid + 256 * j - s0;
+      float a = 0.0f, c = 0.0f;
+      if (x >= 0 && x < n) {
+        a = raw[j].x;
+        c = raw[j].y;
+        if (absorb) {
+          a = -__logf(a);
+          c = -__logf(c);
+        }
+      }
+      v[j] = make_float2(a, has1 ? c : 0.0f);
+    }
+    if (kPrefetch && p + 1 < n_proj) fetch(p + 1);
+
+    dft16<false>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], tw1[k]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) xch[tid * kP16 + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = xch[(lo4 + 16 * j) * kP16 + hi4];
+    __syncthreads();
+    dft16<false>(v);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], tw2[k]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) xch[(hi4 * 16 + k) * kP16 + lo4] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = xch[tid * kP16 + j];
+    dft16<false>(v);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      v[k].x *= wv[k];
+      v[k].y *= wv[k];
+    }
+    dft16<true>(v);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) xch[tid * kP16 + j] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = xch[(hi4 * 16 + k) * kP16 + lo4];
+    __syncthreads();
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], tw2[k]);
+    dft16<true>(v);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) xch[(lo4 + 16 * j) * kP16 + hi4] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = xch[tid * kP16 + k];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], tw1[k]);
+    dft16<true>(v);
+    __syncthreads();  // the exchange is read; the line goes in its place
+#pragma unroll
+    for (int j = 0; j < 16; ++j) xch[tid + 256 * j] = v[j];
+    __syncthreads();
+
+    float scale = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
+    if (contrast == CTAS_CONTRAST_REF) {
+      // std::partial_sum of each filtered row (lct.cpp:774-775), one lane per
+      // row in sequential order, written back into the line.
+      if (tid == 0 || (tid == 64 && has1)) {
+        float *line = reinterpret_cast<float *>(xch) + (tid == 64 ? 1 : 0);
+        float run = 0.0f;
+        for (int x = 0; x < n; ++x) {
+          const float f = line[2 * (s0 + x)] * scale;
+          run = (x == 0) ? f : run + f;
+          line[2 * (s0 + x)] = run;
+        }
+      }
+      __syncthreads();
+      scale = 1.0f;
+    }
+    if (filtered) {
+      float *f0 = filtered + static_cast<size_t>(p) * proj_stride + row0;
+      for (int x = tid; x < n; x += 256) {
+        const float2 q = xch[s0 + x];
+        f0[x] = q.x * scale;
+        if (has1) f0[n + x] = q.y * scale;
+      }
+    }
+    const float4 kc = consts.k[p];  // {cos a, P, R, -}
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = min(tid + 256 * j, n - 1);
+      const float2 q = xch[s0 + ts_column(x, kc.x, kc.y, kc.z, n)];
+      acc0[j] = acc0[j] + q.x * scale;
+      acc1[j] = acc1[j] + q.y * scale;
+    }
+    __syncthreads();  // the next projection overwrites the exchange
+  }
+
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = tid + 256 * j;
+    if (x < n) {
+      result[row0 + x] = acc0[j];
+      if (has1) result[row0 + n + x] = acc1[j];
+    }
+  }
+}
+
 }  // namespace
 
 hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
@@ -412,6 +589,17 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
                          result, W, tw, rows, n, contrast);
     else
       hipLaunchKernelGGL(ts2048_kernel<8>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
+                         result, W, tw, rows, n, contrast);
+    return hipGetLastError();
+  }
+  if (Z == 4096 && kTs4096Lds <= static_cast<size_t>(lds_max)) {
+    const dim3 grid((rows + 1) / 2), block(256);
+    const int J = (n + 255) / 256;  // n <= 4096: J <= 16
+    if (J <= 8)
+      hipLaunchKernelGGL(ts4096_kernel<8>, grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
+                         result, W, tw, rows, n, contrast);
+    else
+      hipLaunchKernelGGL(ts4096_kernel<16>, grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
                          result, W, tw, rows, n, contrast);
     return hipGetLastError();
   }
