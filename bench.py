@@ -6,7 +6,8 @@ padded to 4096, centre offset +3.37 px, output 2048^2 x 256 float32, run as
 one hipGraph-captured launch sequence (min/max -> filter -> backprojection).
 
 One step = one reconstruction of the whole stack with the inputs already in
-HBM. Prints one JSON line (see DESIGN.md for the roofline accounting).
+HBM. Prints one JSON line; the roofline object prices the backprojection's
+LDS sample bytes (4 per voxel-update) against 256 CUs x 128 B/clk x 2.4 GHz.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--slices S] [--no-cpu]
 """
@@ -279,7 +280,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); slices are "
-              "independent and would shard without collectives, see DESIGN.md", file=sys.stderr)
+              "independent and would shard across GPUs without collectives", file=sys.stderr)
         return 2
 
     import numpy as np
