@@ -171,3 +171,24 @@ def test_ax_tool_denoiser(tmp_path):
     for g, wv in zip(got, want):
         assert abs(g - wv) <= 0.25
 
+
+
+@pytest.mark.gpu
+def test_ct_tool_npy_stack_pipeline(tmp_path):
+    """A .npy stack in, a .npy stack out, written plane by plane at their
+    positions by four I/O threads, batches of 2 over 7 slices with the ring
+    filter: equal to one batched reconstruction of the ring-filtered stack."""
+    if ca.device_count() < 1:
+        pytest.skip("no HIP device")
+    S, N, w, c = 7, 180, 256, -1.5
+    sinos = np.stack([oracle.phantom_sino(N, w, c, oracle.random_ellipses(6, w, 1.0, 60 + s), 1e5, 60 + s)
+                      for s in range(S)]).astype(np.float32)
+    np.save(tmp_path / "stack.npy", sinos)
+    r = run("ctas-ct", tmp_path / "stack.npy", "-o", tmp_path / "rec.npy", "-c", c, "-f", "HANN", "-R", 2,
+            "--batch", 2, "--io-threads", 4)
+    assert r.returncode == 0, r.stderr
+    got = np.load(tmp_path / "rec.npy")
+    ref_in = np.stack([ca.ring_filter(s, 2) for s in sinos])
+    _, ref = ca.CTrec(w, N, ca.ABS, "HANN").reconstruct(ref_in, [c] * S)
+    assert got.shape == (S, w, w)
+    assert np.array_equal(got, ref)
