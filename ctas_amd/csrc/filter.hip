@@ -363,6 +363,216 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   (void)LOGZ;
 }
 
+// ---- Z = 8192: 32 x 16 x 16 in registers --------------------------------------
+// Thread t holds the 32 points t + 256 j, takes a 32-point DFT (two 16-point
+// DFTs and a butterfly) and multiplies by w^(t k), w = exp(-2 pi i / 8192);
+// after an exchange through LDS each thread takes two items (k, t1) and the
+// 16-point DFT over t2 of the values at t = t1 + 16 t2, times w^(32 t1 m2);
+// after another, items (k, m2) take the DFT over t1 and hold
+// X[k + 32 m2 + 512 m1], m1 < 16, for the window. The inverse runs back with
+// conjugate factors to the filtered points t + 256 j in natural order.
+// Twiddles are read from the FP64-computed table. One exchange buffer of
+// 512 x 17 entries (69632 B) serves both exchanges.
+constexpr float kCos32[16] = {1.0f, 0.98078528f, 0.92387953f, 0.83146961f, 0.70710678f, 0.55557023f,
+                              0.38268343f, 0.19509032f, 0.0f, -0.19509032f, -0.38268343f, -0.55557023f,
+                              -0.70710678f, -0.83146961f, -0.92387953f, -0.98078528f};
+constexpr float kSin32[16] = {0.0f, 0.19509032f, 0.38268343f, 0.55557023f, 0.70710678f, 0.83146961f,
+                              0.92387953f, 0.98078528f, 1.0f, 0.98078528f, 0.92387953f, 0.83146961f,
+                              0.70710678f, 0.55557023f, 0.38268343f, 0.19509032f};
+
+template <bool INV>
+__device__ __forceinline__ void dft32(float2 (&v)[32]) {
+  float2 e[16], o[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    e[j] = v[2 * j];
+    o[j] = v[2 * j + 1];
+  }
+  dft16<INV>(e);
+  dft16<INV>(o);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const float2 t = cmul(o[k], make_float2(kCos32[k], INV ? kSin32[k] : -kSin32[k]));
+    v[k] = cadd(e[k], t);
+    v[k + 16] = csub(e[k], t);
+  }
+}
+
+// exp(-2 pi i m / 8192) from the half-turn table tw[m] (m < 4096).
+__device__ __forceinline__ float2 w8192(const float2 *__restrict__ tw, int m) {
+  m &= 8191;
+  const float2 t = tw[m & 4095];
+  return (m & 4096) ? make_float2(-t.x, -t.y) : t;
+}
+
+constexpr int kPA8 = 33, kPB8 = 17;
+constexpr size_t kFilter8192Lds = sizeof(float2) * 512 * kPB8;
+
+template <int ST>
+__global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict__ items, int n_items,
+                                                         int contrast) {
+  constexpr int Z = 8192;
+  extern __shared__ float2 xb[];
+  const int b = blockIdx.x;
+  const Item &it = items[find_item_by_pair(items, n_items, b)];
+  const int pair = b - it.pair_base;
+  const int w = it.w, N = it.N;
+  const int s0 = (Z - w) >> 1;
+  const int t0 = 2 * pair, t1r = t0 + 1;
+  const bool has1 = t1r < N;
+  const float *__restrict__ r0 = it.sino + static_cast<size_t>(t0) * w;
+  const float *__restrict__ r1 = r0 + w;
+  const float2 *__restrict__ tw = it.tw;
+  const int tid = threadIdx.x;
+
+  // deAbs (physical.world.cpp:88-100), as in filter_kernel.
+  const bool absorb = (contrast == CTAS_CONTRAST_ABS);
+  bool clamp = false;
+  float mina = 0.0f;
+  if (absorb) {
+    const float2 m = it.minmax[tid & 63];
+    float mn = m.x, mx = m.y;
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, o));
+      mx = fmaxf(mx, __shfl_xor(mx, o));
+    }
+    clamp = (mn <= 0.0f);
+    mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
+    if (pair == 0 && tid == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
+  }
+  auto deabs = [&](float v) {
+    if (clamp && v <= mina) v = mina;
+    return -__logf(v);
+  };
+
+  // Padded line: row samples at s0..s0+w-1 and the next row's first sample
+  // at s0+w (lct.cpp:580-584; 0 after the last row).
+  float2 v[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const int x = tid + 256 * j - s0;
+    float a = 0.0f, c = 0.0f;
+    if (x >= 0 && x < w) {
+      a = r0[x];
+      if (absorb) a = deabs(a);
+      if (has1) {
+        c = r1[x];
+        if (absorb) c = deabs(c);
+      }
+    } else if (x == w) {
+      if (t0 + 1 < N) {
+        a = r0[w];
+        if (absorb) a = deabs(a);
+      }
+      if (has1 && t1r + 1 < N) {
+        c = r1[w];
+        if (absorb) c = deabs(c);
+      }
+    }
+    v[j] = make_float2(a, c);
+  }
+
+  dft32<false>(v);
+#pragma unroll
+  for (int k = 1; k < 32; ++k) v[k] = cmul(v[k], w8192(tw, tid * k));
+#pragma unroll
+  for (int k = 0; k < 32; ++k) xb[tid * kPA8 + k] = v[k];
+  __syncthreads();
+  float2 u[2][16];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int i = tid + 256 * s, k = i >> 4, q = i & 15;  // q = t1
+#pragma unroll
+    for (int t2 = 0; t2 < 16; ++t2) u[s][t2] = xb[(q + 16 * t2) * kPA8 + k];
+  }
+  __syncthreads();  // the first exchange is read; the second takes its place
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int i = tid + 256 * s, k = i >> 4, q = i & 15;
+    dft16<false>(u[s]);
+#pragma unroll
+    for (int m2 = 1; m2 < 16; ++m2) u[s][m2] = cmul(u[s][m2], w8192(tw, 32 * q * m2));
+#pragma unroll
+    for (int m2 = 0; m2 < 16; ++m2) xb[(k * 16 + m2) * kPB8 + q] = u[s][m2];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int i = tid + 256 * s, k = i >> 4, q = i & 15;  // q = m2
+#pragma unroll
+    for (int t1 = 0; t1 < 16; ++t1) u[s][t1] = xb[(k * 16 + q) * kPB8 + t1];
+    dft16<false>(u[s]);
+#pragma unroll
+    for (int m1 = 0; m1 < 16; ++m1) {
+      const float wk = it.W[k + 32 * q + 512 * m1];
+      u[s][m1].x *= wk;
+      u[s][m1].y *= wk;
+    }
+    dft16<true>(u[s]);
+#pragma unroll
+    for (int t1 = 0; t1 < 16; ++t1) xb[(k * 16 + q) * kPB8 + t1] = u[s][t1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int i = tid + 256 * s, k = i >> 4, q = i & 15;  // q = t1
+#pragma unroll
+    for (int m2 = 0; m2 < 16; ++m2) u[s][m2] = xb[(k * 16 + m2) * kPB8 + q];
+  }
+  __syncthreads();  // the second exchange is read; the first takes its place
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int i = tid + 256 * s, k = i >> 4, q = i & 15;
+#pragma unroll
+    for (int m2 = 1; m2 < 16; ++m2) u[s][m2] = cmulc(u[s][m2], w8192(tw, 32 * q * m2));
+    dft16<true>(u[s]);
+#pragma unroll
+    for (int t2 = 0; t2 < 16; ++t2) xb[(q + 16 * t2) * kPA8 + k] = u[s][t2];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; ++k) v[k] = xb[tid * kPA8 + k];
+#pragma unroll
+  for (int k = 1; k < 32; ++k) v[k] = cmulc(v[k], w8192(tw, tid * k));
+  dft32<true>(v);
+
+  const float invZ = 1.0f / static_cast<float>(Z);
+  const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
+  if (contrast != CTAS_CONTRAST_REF) {
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const int x = tid + 256 * j - s0;
+      if (x < 0 || x >= w) continue;
+      const float a = v[j].x * invZ, c = v[j].y * invZ;
+      store_val<ST>(it.filt, row0 + x, a);
+      if (it.filt_f32) it.filt_f32[row0 + x] = a;
+      if (has1) {
+        store_val<ST>(it.filt, row1 + x, c);
+        if (it.filt_f32) it.filt_f32[row1 + x] = c;
+      }
+    }
+    return;
+  }
+  // REF (lct.cpp:590-595): the line in natural order through LDS, then the
+  // prefix sum along each row, one lane per row, in sequential order.
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 32; ++j) xb[tid + 256 * j] = v[j];
+  __syncthreads();
+  if (tid == 0 || (tid == 64 && has1)) {
+    const bool im = tid == 64;
+    const size_t rowb = im ? row1 : row0;
+    float run = 0.0f;
+    for (int x = 0; x < w; ++x) {
+      const float2 q = xb[s0 + x];
+      const float f = (im ? q.y : q.x) * invZ;
+      run = (x == 0) ? f : run + f;
+      store_val<ST>(it.filt, rowb + x, run);
+      if (it.filt_f32) it.filt_f32[rowb + x] = run;
+    }
+  }
+}
+
 // FLT contrast: no filtering, only the storage conversion (and float copy).
 template <int ST>
 __global__ __launch_bounds__(256) void passthrough_kernel(const Item *__restrict__ items,
@@ -422,6 +632,14 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
       case ST_F32: hipLaunchKernelGGL(filter16_kernel<ST_F32>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
       case ST_F16: hipLaunchKernelGGL(filter16_kernel<ST_F16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
       default: hipLaunchKernelGGL(filter16_kernel<ST_BF16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
+    }
+    return hipGetLastError();
+  }
+  if (min_logZ == 13 && max_logZ == 13) {
+    switch (store) {
+      case ST_F32: hipLaunchKernelGGL(filter8192_kernel<ST_F32>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
+      case ST_F16: hipLaunchKernelGGL(filter8192_kernel<ST_F16>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
+      default: hipLaunchKernelGGL(filter8192_kernel<ST_BF16>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
     }
     return hipGetLastError();
   }
