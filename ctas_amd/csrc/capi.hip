@@ -397,6 +397,7 @@ struct BatchWork {
   float2 *d_mm = nullptr;  size_t mm_cap = 0;
   int *d_status = nullptr; size_t st_cap = 0;
   Item *d_items = nullptr; size_t it_cap = 0;
+  Item *d_items_f = nullptr; size_t itf_cap = 0;  // the same items grouped by line length
   unsigned *d_counter = nullptr; size_t ctr_cap = 0;
 };
 std::mutex g_batch_mu;
@@ -891,15 +892,37 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   if (int rc = grow(bw.d_mm, bw.mm_cap, sizeof(float2) * kMinMaxBlocks * n)) return rc;
   if (int rc = grow(bw.d_status, bw.st_cap, sizeof(int) * n)) return rc;
   if (int rc = grow(bw.d_items, bw.it_cap, sizeof(Item) * n)) return rc;
+  if (int rc = grow(bw.d_items_f, bw.itf_cap, sizeof(Item) * n)) return rc;
   if (int rc = grow(bw.d_counter, bw.ctr_cap, sizeof(unsigned) * 64)) return rc;
   for (int oi = 0; oi < n; ++oi) {
     its[oi].filt = static_cast<unsigned char *>(bw.d_filt) + filt_off[oi];
     its[oi].minmax = bw.d_mm + static_cast<size_t>(kMinMaxBlocks) * oi;
     its[oi].status = bw.d_status + oi;
   }
+  // The filter runs once per line length, each with its own kernel (the
+  // register-resident FFTs for 4096 and 8192 points): the items again,
+  // grouped by logZ, with their row pairs counted from each group's start.
+  std::vector<Item> fits(its);
+  std::stable_sort(fits.begin(), fits.end(), [](const Item &a, const Item &b) { return a.logZ < b.logZ; });
+  std::vector<std::tuple<int, int, int, int>> groups;  // first item, items, pairs, logZ
+  for (int i = 0; i < n;) {
+    int j = i, gp = 0;
+    for (; j < n && fits[j].logZ == fits[i].logZ; ++j) {
+      fits[j].pair_base = gp;
+      gp += (fits[j].N + 1) / 2;
+    }
+    groups.emplace_back(i, j - i, gp, fits[i].logZ);
+    i = j;
+  }
   CTAS_HIP_TRY(hipMemcpy(bw.d_items, its.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(bw.d_items_f, fits.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
   if (contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(bw.d_items, n, s));
-  CTAS_HIP_TRY(launch_filter(bw.d_items, n, pairs, min_logZ, max_logZ, contrast, st, s));
+  for (const auto &g : groups)
+    CTAS_HIP_TRY(launch_filter(bw.d_items_f + std::get<0>(g), std::get<1>(g), std::get<2>(g), std::get<3>(g),
+                               std::get<3>(g), contrast, st, s));
+  (void)pairs;
+  (void)min_logZ;
+  (void)max_logZ;
   CTAS_HIP_TRY(launch_backproject_persistent(bw.d_items, n, tiles, mode, st, bw.d_counter,
                                              persistent_workgroups(), s));
   CTAS_HIP_TRY(hipStreamSynchronize(s));

@@ -248,10 +248,13 @@ def test_device_graph_and_streams(torch_cuda):
 
 def test_uneven_batch_persistent(torch_cuda):
     torch = torch_cuda
-    shapes = [(360, 512), (90, 768), (180, 256), (240, 1024), (45, 300)]
+    """Items of five line lengths (512 to 8192 points) in one batch: the
+    filter runs once per length, and every slice equals the one its own
+    handle reconstructs."""
+    shapes = [(360, 512), (90, 768), (180, 256), (240, 1024), (45, 300), (60, 1500), (30, 2500)]
     items, refs = [], []
     for k, (N, w) in enumerate(shapes):
-        c = [0.0, 1.5, -2.0, 3.37, 0.5][k]
+        c = [0.0, 1.5, -2.0, 3.37, 0.5, -1.25, 2.0][k]
         P = _phantom(N, w, c, 6, 1e5, seed=60 + k)
         d_in = torch.from_numpy(P).cuda()
         d_out = torch.zeros((w, w), dtype=torch.float32, device="cuda")
