@@ -160,3 +160,20 @@ def test_ts_accumulate_host_batch():
         _, R = oracle.ts_add(Ps[i], R, W, -2.0, ca.PHS, angles[i], 3)
     assert rel_rms(got, R) <= 1e-5
 
+
+
+@pytest.mark.parametrize("rows,n", [(3, 1), (2, 2), (4, 3), (1, 2048), (1, 4096)])
+@pytest.mark.parametrize("contrast", [ca.ABS, ca.REF])
+def test_ts_add_edge_widths(rows, n, contrast):
+    """Widths 1-3 (a 2-point line for width 1) and single rows on the
+    register-FFT paths."""
+    P = _proj(rows, n, 17 + n, contrast)
+    R0 = np.zeros((rows, n), np.float32)
+    W = oracle.filter_fill(1, 0.0, n)
+    for angle, c, plane in ((0.0, 0.0, 0), (0.4, 0.25, 1), (math.pi, 0.0, 0)):
+        a = np.float32(angle)
+        fref, rref = oracle.ts_add(P, R0, W, c, contrast, a, plane)
+        Pd, Rd = P.copy(), R0.copy()
+        ca.ts_add(Pd, Rd, "RAMP", 0.0, c, contrast, float(a), plane)
+        _check(Pd, Rd, fref, rref, f"rows={rows} n={n} contrast={contrast} angle={angle}",
+               1e-4 if contrast == ca.REF else 1e-5)
