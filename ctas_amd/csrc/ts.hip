@@ -10,20 +10,23 @@
 // is evaluated in float and clamped to [0, n - 1]. The column map d(x) is
 // the same for every row.
 //
-// Layout: one workgroup of 256 threads owns a pair of rows and walks through
-// a batch of projections in order. The two rows travel through the FFT as
-// the real and imaginary parts of one complex line in LDS (the window is
-// real and even); the filtered line stays in LDS, so the resampling reads it
-// there and the only HBM traffic per projection and pixel is the raw sample
-// in and, when asked for, the filtered sample out. Thread t keeps result
-// columns t + 256 j of both rows in registers across the whole batch, so
-// the result is read and written once per batch; the per-pixel sums happen
-// in projection order, as successive ts_add calls make them.
+// Layout: a workgroup of 256 threads owns two rows (or four) and walks
+// through a batch of projections in order. Two rows travel through the FFT
+// as the real and imaginary parts of one complex line (the window is real
+// and even); the filtered line ends in LDS, so the resampling reads it there
+// and the only HBM traffic per projection and pixel is the raw sample in
+// and, when asked for, the filtered sample out. Thread t keeps its result
+// columns t + 256 j in registers across the whole batch, so the result is
+// read and written once per batch; the per-pixel sums happen in projection
+// order, as successive ts_add calls make them. Three kernels:
+//   ts2048_kernel  2048-point lines, four rows, FFT in registers (8x16x16);
+//   ts4096_kernel  4096-point lines, two rows, FFT in registers (16x16x16);
+//   ts_kernel      other lengths, two rows, radix-2 FFT in LDS, up to four
+//                  projections in flight per workgroup.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
-#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
