@@ -373,38 +373,6 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
 // conjugate factors to the filtered points t + 256 j in natural order.
 // Twiddles are read from the FP64-computed table. One exchange buffer of
 // 512 x 17 entries (69632 B) serves both exchanges.
-constexpr float kCos32[16] = {1.0f, 0.98078528f, 0.92387953f, 0.83146961f, 0.70710678f, 0.55557023f,
-                              0.38268343f, 0.19509032f, 0.0f, -0.19509032f, -0.38268343f, -0.55557023f,
-                              -0.70710678f, -0.83146961f, -0.92387953f, -0.98078528f};
-constexpr float kSin32[16] = {0.0f, 0.19509032f, 0.38268343f, 0.55557023f, 0.70710678f, 0.83146961f,
-                              0.92387953f, 0.98078528f, 1.0f, 0.98078528f, 0.92387953f, 0.83146961f,
-                              0.70710678f, 0.55557023f, 0.38268343f, 0.19509032f};
-
-template <bool INV>
-__device__ __forceinline__ void dft32(float2 (&v)[32]) {
-  float2 e[16], o[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    e[j] = v[2 * j];
-    o[j] = v[2 * j + 1];
-  }
-  dft16<INV>(e);
-  dft16<INV>(o);
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const float2 t = cmul(o[k], make_float2(kCos32[k], INV ? kSin32[k] : -kSin32[k]));
-    v[k] = cadd(e[k], t);
-    v[k + 16] = csub(e[k], t);
-  }
-}
-
-// exp(-2 pi i m / 8192) from the half-turn table tw[m] (m < 4096).
-__device__ __forceinline__ float2 w8192(const float2 *__restrict__ tw, int m) {
-  m &= 8191;
-  const float2 t = tw[m & 4095];
-  return (m & 4096) ? make_float2(-t.x, -t.y) : t;
-}
-
 constexpr int kPA8 = 33, kPB8 = 17;
 constexpr size_t kFilter8192Lds = sizeof(float2) * 512 * kPB8;
 

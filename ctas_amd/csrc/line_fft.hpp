@@ -258,5 +258,39 @@ __device__ __forceinline__ void twiddle_powers8(float2 (&v)[8], float2 r) {
   v[7] = cmul(v[7], cmul(p4, p3));
 }
 
+// 32-point DFT in registers: two 16-point DFTs of the even and odd points,
+// combined with exp(-+2 pi i k / 32).
+constexpr float kCos32[16] = {1.0f, 0.98078528f, 0.92387953f, 0.83146961f, 0.70710678f, 0.55557023f,
+                              0.38268343f, 0.19509032f, 0.0f, -0.19509032f, -0.38268343f, -0.55557023f,
+                              -0.70710678f, -0.83146961f, -0.92387953f, -0.98078528f};
+constexpr float kSin32[16] = {0.0f, 0.19509032f, 0.38268343f, 0.55557023f, 0.70710678f, 0.83146961f,
+                              0.92387953f, 0.98078528f, 1.0f, 0.98078528f, 0.92387953f, 0.83146961f,
+                              0.70710678f, 0.55557023f, 0.38268343f, 0.19509032f};
+
+template <bool INV>
+__device__ __forceinline__ void dft32(float2 (&v)[32]) {
+  float2 e[16], o[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    e[j] = v[2 * j];
+    o[j] = v[2 * j + 1];
+  }
+  dft16<INV>(e);
+  dft16<INV>(o);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const float2 t = cmul(o[k], make_float2(kCos32[k], INV ? kSin32[k] : -kSin32[k]));
+    v[k] = cadd(e[k], t);
+    v[k + 16] = csub(e[k], t);
+  }
+}
+
+// exp(-2 pi i m / 8192) from the half-turn table tw[m] (m < 4096).
+__device__ __forceinline__ float2 w8192(const float2 *__restrict__ tw, int m) {
+  m &= 8191;
+  const float2 t = tw[m & 4095];
+  return (m & 4096) ? make_float2(-t.x, -t.y) : t;
+}
+
 }  // namespace
 }  // namespace ctas

@@ -18,9 +18,11 @@
 // and, when asked for, the filtered sample out. Thread t keeps its result
 // columns t + 256 j in registers across the whole batch, so the result is
 // read and written once per batch; the per-pixel sums happen in projection
-// order, as successive ts_add calls make them. Three kernels:
+// order, as successive ts_add calls make them. The kernels:
 //   ts2048_kernel  2048-point lines, four rows, FFT in registers (8x16x16);
 //   ts4096_kernel  4096-point lines, two rows, FFT in registers (16x16x16);
+//   ts8192_kernel  8192-point lines of widths up to 4096, two rows, FFT in
+//                  registers (32x16x16);
 //   ts_kernel      other lengths, two rows, radix-2 FFT in LDS, up to four
 //                  projections in flight per workgroup.
 #include <hip/hip_runtime.h>
@@ -577,6 +579,167 @@ __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long lon
   }
 }
 
+// 8192-point lines of widths 2049-4095 (padded): one line (two rows) per
+// workgroup, transformed in registers as 32 x 16 x 16 with the steps of the
+// sinogram filter's 8192-point kernel (thread t holds x[t + 256 j], j < 32;
+// two items per thread in each radix-16 pass; one exchange buffer of
+// 512 x 17 entries, which then holds the filtered line).
+constexpr int kTsPA8 = 33, kTsPB8 = 17;
+constexpr size_t kTs8192Lds = sizeof(float2) * 512 * kTsPB8;
+
+template <int J>
+__global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long long proj_stride, int n_proj,
+                                                     TsConsts consts, float *filtered,
+                                                     float *__restrict__ result, const float *__restrict__ W,
+                                                     const float2 *__restrict__ tw, int rows, int n, int contrast) {
+  constexpr int Z = 8192;
+  extern __shared__ float2 xb[];
+  const int tid = threadIdx.x;
+  const int t0 = 2 * blockIdx.x;
+  const bool has1 = t0 + 1 < rows;
+  const int s0 = (Z - n) >> 1;
+  const bool absorb = contrast == CTAS_CONTRAST_ABS;
+  const size_t row0 = static_cast<size_t>(t0) * n, row1c = has1 ? row0 + n : row0;
+
+  float acc0[J], acc1[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = min(tid + 256 * j, n - 1);
+    acc0[j] = result[row0 + x];
+    acc1[j] = result[row1c + x];
+  }
+
+  for (int p = 0; p < n_proj; ++p) {
+    const float *r0 = proj + static_cast<size_t>(p) * proj_stride + row0;
+    const float *r1 = proj + static_cast<size_t>(p) * proj_stride + row1c;
+    float2 v[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const int x = tid + 256 * j - s0;
+      float a = 0.0f, c = 0.0f;
+      if (x >= 0 && x < n) {
+        a = r0[x];
+        c = r1[x];
+        if (absorb) {
+          a = -__logf(a);
+          c = -__logf(c);
+        }
+      }
+      v[j] = make_float2(a, has1 ? c : 0.0f);
+    }
+    dft32<false>(v);
+#pragma unroll
+    for (int k = 1; k < 32; ++k) v[k] = cmul(v[k], w8192(tw, tid * k));
+#pragma unroll
+    for (int k = 0; k < 32; ++k) xb[tid * kTsPA8 + k] = v[k];
+    __syncthreads();
+    float2 u[2][16];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int i = tid + 256 * s, k = i >> 4, q = i & 15;  // q = t1
+#pragma unroll
+      for (int t2 = 0; t2 < 16; ++t2) u[s][t2] = xb[(q + 16 * t2) * kTsPA8 + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int i = tid + 256 * s, k = i >> 4, q = i & 15;
+      dft16<false>(u[s]);
+#pragma unroll
+      for (int m2 = 1; m2 < 16; ++m2) u[s][m2] = cmul(u[s][m2], w8192(tw, 32 * q * m2));
+#pragma unroll
+      for (int m2 = 0; m2 < 16; ++m2) xb[(k * 16 + m2) * kTsPB8 + q] = u[s][m2];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int i = tid + 256 * s, k = i >> 4, q = i & 15;  // q = m2
+#pragma unroll
+      for (int t1 = 0; t1 < 16; ++t1) u[s][t1] = xb[(k * 16 + q) * kTsPB8 + t1];
+      dft16<false>(u[s]);
+#pragma unroll
+      for (int m1 = 0; m1 < 16; ++m1) {
+        const float wk = W[k + 32 * q + 512 * m1];
+        u[s][m1].x *= wk;
+        u[s][m1].y *= wk;
+      }
+      dft16<true>(u[s]);
+#pragma unroll
+      for (int t1 = 0; t1 < 16; ++t1) xb[(k * 16 + q) * kTsPB8 + t1] = u[s][t1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int i = tid + 256 * s, k = i >> 4, q = i & 15;  // q = t1
+#pragma unroll
+      for (int m2 = 0; m2 < 16; ++m2) u[s][m2] = xb[(k * 16 + m2) * kTsPB8 + q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int i = tid + 256 * s, k = i >> 4, q = i & 15;
+#pragma unroll
+      for (int m2 = 1; m2 < 16; ++m2) u[s][m2] = cmulc(u[s][m2], w8192(tw, 32 * q * m2));
+      dft16<true>(u[s]);
+#pragma unroll
+      for (int t2 = 0; t2 < 16; ++t2) xb[(q + 16 * t2) * kTsPA8 + k] = u[s][t2];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v[k] = xb[tid * kTsPA8 + k];
+#pragma unroll
+    for (int k = 1; k < 32; ++k) v[k] = cmulc(v[k], w8192(tw, tid * k));
+    dft32<true>(v);
+    __syncthreads();  // the exchange is read; the line goes in its place
+#pragma unroll
+    for (int j = 0; j < 32; ++j) xb[tid + 256 * j] = v[j];
+    __syncthreads();
+
+    float scale = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
+    if (contrast == CTAS_CONTRAST_REF) {
+      // std::partial_sum of each filtered row (lct.cpp:774-775), one lane per
+      // row in sequential order, written back into the line.
+      if (tid == 0 || (tid == 64 && has1)) {
+        float *line = reinterpret_cast<float *>(xb) + (tid == 64 ? 1 : 0);
+        float run = 0.0f;
+        for (int x = 0; x < n; ++x) {
+          const float f = line[2 * (s0 + x)] * scale;
+          run = (x == 0) ? f : run + f;
+          line[2 * (s0 + x)] = run;
+        }
+      }
+      __syncthreads();
+      scale = 1.0f;
+    }
+    if (filtered) {
+      float *f0 = filtered + static_cast<size_t>(p) * proj_stride + row0;
+      for (int x = tid; x < n; x += 256) {
+        const float2 q = xb[s0 + x];
+        f0[x] = q.x * scale;
+        if (has1) f0[n + x] = q.y * scale;
+      }
+    }
+    const float4 kc = consts.k[p];  // {cos a, P, R, -}
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int x = min(tid + 256 * j, n - 1);
+      const float2 q = xb[s0 + ts_column(x, kc.x, kc.y, kc.z, n)];
+      acc0[j] = acc0[j] + q.x * scale;
+      acc1[j] = acc1[j] + q.y * scale;
+    }
+    __syncthreads();  // the next projection overwrites the buffer
+  }
+
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int x = tid + 256 * j;
+    if (x < n) {
+      result[row0 + x] = acc0[j];
+      if (has1) result[row0 + n + x] = acc1[j];
+    }
+  }
+}
+
 }  // namespace
 
 hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
@@ -593,6 +756,12 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
     else
       hipLaunchKernelGGL(ts2048_kernel<8>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
                          result, W, tw, rows, n, contrast);
+    return hipGetLastError();
+  }
+  if (Z == 8192 && n <= 4096 && kTs8192Lds <= static_cast<size_t>(lds_max)) {
+    const dim3 grid((rows + 1) / 2), block(256);
+    hipLaunchKernelGGL(ts8192_kernel<16>, grid, block, kTs8192Lds, s, proj, proj_stride, n_proj, consts, filtered,
+                       result, W, tw, rows, n, contrast);
     return hipGetLastError();
   }
   if (Z == 4096 && kTs4096Lds <= static_cast<size_t>(lds_max)) {

@@ -52,7 +52,7 @@ def _check(filt_dev, res_dev, filt_ref, res_ref, tag, tol_f=1e-5):
 
 
 @pytest.mark.parametrize("rows,n", [(1, 64), (7, 256), (16, 300), (5, 257), (12, 1000), (4, 2048), (7, 2048),
-                                    (6, 700), (5, 1500), (3, 4096)])
+                                    (6, 700), (5, 1500), (3, 4096), (5, 3000), (2, 2049)])
 @pytest.mark.parametrize("contrast", [ca.ABS, ca.PHS, ca.REF])
 def test_ts_add_matches_oracle(rows, n, contrast):
     P = _proj(rows, n, rows * 1000 + n, contrast)
