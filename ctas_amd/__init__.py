@@ -85,6 +85,8 @@ def lib() -> C.CDLL:
         "ctas_fbp_clone": (i, [vp, C.POINTER(vp)]),
         "ctas_fbp_destroy": (i, [vp]),
         "ctas_fbp_info": (i, [vp, ip, ip, ip, ip]),
+        "ctas_fbp_rec_shape": (i, [vp, f, ip, ip]),
+        "ctas_fbp_rec_crop": (i, [vp, f, ip, ip]),
         "ctas_fbp_set_physics": (i, [vp, f, f]),
         "ctas_fbp_get_reccof": (i, [vp, fp]),
         "ctas_fbp_set_angles": (i, [vp, fp]),
@@ -256,6 +258,18 @@ class CTrec:
         z = C.c_int()
         _check(lib().ctas_fbp_info(self._h, None, None, C.byref(z), None))
         return z.value
+
+    def rec_shape(self, center: float = 0.0) -> tuple[int, int]:
+        """recShape (ct.h:272-274): rows and columns of the cropped slice."""
+        r, c = C.c_int(), C.c_int()
+        _check(lib().ctas_fbp_rec_shape(self._h, center, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
+    def rec_crop(self, center: float = 0.0) -> slice:
+        """recCrop (ct.h:276-279) as the slice of the rows kept."""
+        b, e = C.c_int(), C.c_int()
+        _check(lib().ctas_fbp_rec_crop(self._h, center, C.byref(b), C.byref(e)))
+        return slice(b.value, e.value)
 
     def window(self) -> np.ndarray:
         W = np.zeros(self.zidth, np.float32)

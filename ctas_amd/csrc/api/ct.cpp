@@ -113,13 +113,15 @@ CTrec::~CTrec() {
 }
 
 Shape2 CTrec::recShape(float center) const {
-  const long r = static_cast<long>(std::ceil(std::fabs(center)));
-  return {ish_[1] - 2 * r, ish_[1]};
+  int rows = 0, cols = 0;
+  check(ctas_fbp_rec_shape(h_, center, &rows, &cols), kRecMod);
+  return {rows, cols};
 }
 
 Crop2 CTrec::recCrop(float center) const {
-  const long r = static_cast<long>(std::ceil(std::fabs(center)));
-  return Crop2{r, r};
+  int b = 0, e = 0;
+  check(ctas_fbp_rec_crop(h_, center, &b, &e), kRecMod);
+  return Crop2{b, ish_[1] - e};
 }
 
 void CTrec::setPhysics(float pixelSize, float lambda) { check(ctas_fbp_set_physics(h_, pixelSize, lambda), kRecMod); }

@@ -441,12 +441,17 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
 // ~1e-4 px of the FP64 coordinate, finer than the float arithmetic of the
 // parity path).
 // The fixed-point paths index windows by floor(t); the reference reads
-// sample (long)t, truncated (lct.cpp:710-713), which for t < 0 is
-// floor(t) + 1. Window position k therefore holds the sample at trunc_pos(k).
-// Only a coordinate that is exactly a negative integer (a sample outside the
-// detector, at an angle whose constants are exact) reads one bin higher than
-// the truncation would.
+// sample (long)t, truncated (lct.cpp:710-713), which for a non-integer t < 0
+// is floor(t) + 1. Window position k therefore holds the sample at
+// trunc_pos(k). An integer t is its own truncation: at an angle whose sine
+// and cosine are 0 or +-1 (multiples of 90 degrees) every coordinate of a
+// tile shares the fraction of the tile's base, and when that fraction is 0
+// the window holds sample k at position k (`exact`), so that a coordinate
+// that is exactly a negative integer reads that sample. At other angles a
+// fixed-point coordinate lands on an integer only by rounding, where the
+// FP64 coordinate may lie on either side of it.
 __device__ __forceinline__ int trunc_pos(int k) { return k < 0 ? k + 1 : k; }
+__device__ __forceinline__ int window_pos(int k, bool exact) { return exact ? k : trunc_pos(k); }
 
 template <int ST>
 __device__ __forceinline__ float half_to_float(uint32_t h) {
@@ -489,8 +494,11 @@ struct StagePair {
     const double tc = T0 + sd.x * (0.5 * (G::RT - 1)) + sd.y * (0.5 * (G::CT - 1));
     const int lo = static_cast<int>(floor(tc)) - G::HALF - 2;
     const bool neg = sd.x < 0.0;
-    const int bt = static_cast<int>(llrint((T0 - lo + static_cast<double>(a) * L) * 65536.0)) - (neg ? 65536 : 0);
+    const int b0 = static_cast<int>(llrint((T0 - lo + static_cast<double>(a) * L) * 65536.0));
+    const int bt = b0 - (neg ? 65536 : 0);
     const int S = static_cast<int>(llrint(sd.x * 65536.0)), C = static_cast<int>(llrint(sd.y * 65536.0));
+    // every coordinate of the tile an integer: sample k at position k
+    const bool exact = ((b0 | S | C) & 0xFFFF) == 0;
     const int thr = neg ? -S : 65536 - S;
     prm0 = make_int4(bt, S, C, thr | (neg ? (1 << 30) : 0));
     prm1 = make_int4(8 * S, 16 * C, 0, 0);
@@ -500,8 +508,8 @@ struct StagePair {
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int k = lo + sub + j * TPA;
-      v[j][0] = ldg(filt, rb + trunc_pos(k));
-      v[j][1] = ldg(filt, rb + trunc_pos(k + 1));
+      v[j][0] = ldg(filt, rb + window_pos(k, exact));
+      v[j][1] = ldg(filt, rb + window_pos(k + 1, exact));
     }
   }
 
@@ -677,8 +685,10 @@ __device__ __forceinline__ float add_pair(float acc, uint32_t x) {
 // Integer constants of one slot for one tile: window starts, the 16.16
 // bases of the row terms (word offsets within the slot included) and the
 // sinogram offsets of the two rows (rb < 0: no partner).
+// exact: bit 0 (1) every coordinate of window A is an integer, bit 1 (2)
+// every coordinate of window B.
 struct SlotConsts {
-  int loA, loB, ba, bb, ra, rb, unused0, unused1;
+  int loA, loB, ba, bb, ra, rb, exact, unused;
 };
 
 template <int KIND, int AC, int ST>
@@ -713,6 +723,7 @@ struct StageMirror {
     raw.c = ldg(d, 1);
     raw.row_a = ldg(r, 0);
     raw.row_b = ldg(r, 1);
+    raw.integral = ldg(r, 2);
   }
   // Deriving wave: the constants of its lanes' slots for the tile at
   // (ty0, tx0) (the mirrored tile's columns are 2hp - x), formed in FP64,
@@ -732,7 +743,10 @@ struct StageMirror {
     c.bb = static_cast<int>(rint((TB - c.loB) * 65536.0));
     c.ra = it.pad + raw.row_a * w;
     c.rb = raw.row_b >= 0 ? it.pad + raw.row_b * w : -1;
-    c.unused0 = c.unused1 = 0;
+    // the row and column terms are whole pixels: the base's fraction is
+    // that of every coordinate of the window
+    c.exact = raw.integral ? ((c.ba & 0xFFFF) == 0 ? 1 : 0) | ((c.bb & 0xFFFF) == 0 ? 2 : 0) : 0;
+    c.unused = 0;
     if (l < AC) reinterpret_cast<SlotConsts *>(lds + ANGS)[par * AC + l] = c;
   }
   __device__ __forceinline__ void get(const unsigned char *lds, int par, int na) {
@@ -753,7 +767,7 @@ struct StageMirror {
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;  // word e of the slot: [0, L) window A, [L, 2L) window B
-      const int kk = trunc_pos(e < L ? k.loA + e : k.loB + (e - L));
+      const int kk = e < L ? window_pos(k.loA + e, k.exact & 1) : window_pos(k.loB + (e - L), k.exact & 2);
       v[j].x = ldg32(base16, static_cast<unsigned>(k.ra + kk));
       v[j].y = ldg32(base16, static_cast<unsigned>((k.rb + kk) & mb));
     }

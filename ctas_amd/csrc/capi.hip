@@ -139,13 +139,24 @@ int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
   if (!h->d_slot_tab && hipMalloc(&h->d_slot_tab, sizeof(int) * kSlotTab * std::max(h->N, 1)) != hipSuccess)
     return fail(CTAS_E_NOMEM, "device allocation of the mirror slot tables failed");
   std::vector<int> tab(static_cast<size_t>(kSlotTab) * slots.size());
-  for (size_t i = 0; i < slots.size(); ++i)
+  for (size_t i = 0; i < slots.size(); ++i) {
+    int frac = 0;  // OR of the fractional bits of every term
     for (int kind = TK_64x64; kind <= TK_128x128; ++kind) {
       const int RT = tile_rows(kind), CT = tile_cols(kind), TR = RT / 16, TC = CT / 16;
       int *t = tab.data() + i * kSlotTab + lattice_offset(kind);
-      for (int y = 0; y < RT; ++y) t[lattice_row(y, TR)] = static_cast<int>(std::lrint(slots[i].s * y * 65536.0));
-      for (int x = 0; x < CT; ++x) t[RT + lattice_col(x, TC)] = static_cast<int>(std::lrint(slots[i].c * x * 65536.0));
+      for (int y = 0; y < RT; ++y) {
+        const int v = static_cast<int>(std::lrint(slots[i].s * y * 65536.0));
+        t[lattice_row(y, TR)] = v;
+        frac |= v & 0xFFFF;
+      }
+      for (int x = 0; x < CT; ++x) {
+        const int v = static_cast<int>(std::lrint(slots[i].c * x * 65536.0));
+        t[RT + lattice_col(x, TC)] = v;
+        frac |= v & 0xFFFF;
+      }
     }
+    slots[i].integral = frac == 0;
+  }
   CTAS_HIP_TRY(hipMemcpy(h->d_slots, slots.data(), sizeof(MirrorSlot) * slots.size(), hipMemcpyHostToDevice));
   CTAS_HIP_TRY(hipMemcpy(h->d_slot_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
   return CTAS_OK;
@@ -568,6 +579,22 @@ int ctas_fbp_destroy(ctas_fbp *h) {
   if (h->stream) (void)hipStreamDestroy(h->stream);
   drop_extra(h);
   delete h;
+  return CTAS_OK;
+}
+
+int ctas_fbp_rec_shape(const ctas_fbp *h, float center, int *rows, int *cols) {
+  if (!h || !rows || !cols) return fail(CTAS_E_ARG, "null argument");
+  const int r = static_cast<int>(std::ceil(std::fabs(center)));
+  *rows = h->w - 2 * r;
+  *cols = h->w;
+  return CTAS_OK;
+}
+
+int ctas_fbp_rec_crop(const ctas_fbp *h, float center, int *row_begin, int *row_end) {
+  if (!h || !row_begin || !row_end) return fail(CTAS_E_ARG, "null argument");
+  const int r = static_cast<int>(std::ceil(std::fabs(center)));
+  *row_begin = r;
+  *row_end = h->w - r;
   return CTAS_OK;
 }
 

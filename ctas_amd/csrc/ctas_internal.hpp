@@ -56,7 +56,10 @@ float improve_me(float eps, const std::vector<float> &vax, const std::vector<flo
 struct MirrorSlot {
   double s, c;  // FP64 sin and cos of angle row_a
   int row_a, row_b;
-  int pad[2];
+  int integral;  // every row and column term of every tile kind is a whole
+                 // multiple of 2^16 (sin and cos in {0, +-1}): a tile's
+                 // coordinates are then all integers or all non-integers
+  int pad;
 };
 // Per slot and tile kind, the row terms round(s y 2^16) (y < rows of the
 // tile) and then the column terms round(c x 2^16) (x < columns), each in
@@ -158,9 +161,15 @@ constexpr int kTsMaxWidth = 8192;
 struct TsConsts {
   float4 k[kTsBatch];
 };
+// odd (null for even widths): (u[x], v[x]) of the odd-width correction of
+// the row filter, odd_g its gain (ts.hip).
 hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
                      float *filtered, float *result, const float *W, const float2 *tw, int rows, int n, int Z,
-                     int logZ, int contrast, int cus, int lds_max, hipStream_t s);
+                     int logZ, int contrast, int cus, int lds_max, const float2 *odd, float odd_g, hipStream_t s);
+// For odd n >= 3: the basis (u[x], v[x]) = (-1)^x (cos(pi x/n), sin(pi x/n)),
+// x < n, and the gain Z (W[m] - W[m+1])/n (W = Filter::fill(n), m = (n-1)/2)
+// of the odd-width correction of ts_add's row filter; false for other n.
+bool ts_odd_basis(int filter, float param, int n, int Z, std::vector<float2> &uv, float &gain);
 // Axis-score kernels (axis.hip).
 // AverageGradient::apply scores of n slices [n][w][w] reconstructed at the
 // given centres; d_tmp holds n*w*w floats, d_partial n*kScoreBlocks*2.

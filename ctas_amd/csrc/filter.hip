@@ -603,7 +603,13 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
     }
     return hipGetLastError();
   }
-  if (min_logZ == 13 && max_logZ == 13) {
+  // The 8192-point register kernel's exchange buffer must fit the device's
+  // LDS per workgroup; otherwise the radix-2 LDS kernel filters these lines.
+  int dev = 0, lds_max = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return hipErrorInvalidDevice;
+  if (min_logZ == 13 && max_logZ == 13 && kFilter8192Lds <= static_cast<size_t>(lds_max)) {
     switch (store) {
       case ST_F32: hipLaunchKernelGGL(filter8192_kernel<ST_F32>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
       case ST_F16: hipLaunchKernelGGL(filter8192_kernel<ST_F16>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
@@ -613,6 +619,7 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
   }
   // a padded line of any item fits in that of the largest
   const size_t lds = sizeof(float2) * padded_len(1 << max_logZ, line_psh(1 << max_logZ));
+  if (lds > static_cast<size_t>(lds_max)) return hipErrorInvalidValue;
   switch (store) {
     case ST_F32: hipLaunchKernelGGL(filter_kernel<ST_F32>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
     case ST_F16: hipLaunchKernelGGL(filter_kernel<ST_F16>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;

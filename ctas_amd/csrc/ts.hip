@@ -54,6 +54,75 @@ __device__ __forceinline__ int ts_column(int x, float cs, float pc, float rc, in
   return i;
 }
 
+// Odd widths. FFTW's half-complex product of filter_line (lct.cpp:280-291)
+// multiplies the real part of bin k by W[k] and its imaginary part by
+// W[n-k]. Filter::fill's window is even except at the middle bins of an odd
+// n: f = 2(n-k)/n at k = m = (n-1)/2 and 2(n-m-1)/n at k = m+1
+// (lct.cpp:147-148), so W[m] != W[m+1]. The device runs the circular
+// convolution of the even window, whose bin m takes their mean; the
+// difference is the rank-two term
+//   y[x] += (W[m] - W[m+1])/n (Re X_m cos(phi x) + Im X_m sin(phi x)),
+//   phi = 2 pi m/n, X_m = sum_t x[t] exp(-i phi t).
+// With u[t] = (-1)^t cos(pi t/n) = cos(phi t) and v[t] = (-1)^t sin(pi t/n)
+// = -sin(phi t): y[x] += g (S_u u[x] - S_v v[x]), S_u = sum_t x[t] u[t],
+// S_v = sum_t x[t] v[t]; g = Z (W[m] - W[m+1])/n for a line scaled by Z.
+// `odd` holds (u[x], v[x]), x < n, for odd n >= 3 and is null otherwise.
+//
+// Sums S_u, S_v of the rows r0 and r1 of one projection (after -log for
+// ABS): thread lt of a group of 256, reduced over its wave; lane 0 of wave
+// w leaves {S_u row 0, S_u row 1, S_v row 0, S_v row 1} at stash[4 w].
+__device__ __forceinline__ void odd_sums(const float *r0, const float *r1, bool has0, bool has1, int n,
+                                         bool absorb, const float2 *__restrict__ odd, int lt, float *stash) {
+  float su0 = 0.0f, su1 = 0.0f, sv0 = 0.0f, sv1 = 0.0f;
+  for (int x = lt; x < n; x += 256) {
+    float a = has0 ? r0[x] : 1.0f, c = has1 ? r1[x] : 1.0f;
+    if (absorb) {
+      a = -__logf(a);
+      c = -__logf(c);
+    }
+    a = has0 ? a : 0.0f;
+    c = has1 ? c : 0.0f;
+    const float2 uv = odd[x];
+    su0 += a * uv.x;
+    su1 += c * uv.x;
+    sv0 += a * uv.y;
+    sv1 += c * uv.y;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    su0 += __shfl_xor(su0, o);
+    su1 += __shfl_xor(su1, o);
+    sv0 += __shfl_xor(sv0, o);
+    sv1 += __shfl_xor(sv1, o);
+  }
+  if ((lt & 63) == 0) {
+    float *d = stash + 4 * (lt >> 6);
+    d[0] = su0;
+    d[1] = su1;
+    d[2] = sv0;
+    d[3] = sv1;
+  }
+}
+
+// The four waves' stashes summed: {S_u row 0, S_u row 1, S_v row 0, S_v row 1}.
+__device__ __forceinline__ float4 odd_total(const float *stash) {
+  float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    t.x += stash[4 * w];
+    t.y += stash[4 * w + 1];
+    t.z += stash[4 * w + 2];
+    t.w += stash[4 * w + 3];
+  }
+  return t;
+}
+
+// The correction of point x (< n) of a row pair's line scaled by Z.
+__device__ __forceinline__ float2 odd_term(const float2 *__restrict__ odd, float g, float4 t, int x) {
+  const float2 uv = odd[x];
+  return make_float2(g * (t.x * uv.x - t.z * uv.y), g * (t.y * uv.x - t.w * uv.y));
+}
+
 // K projections are filtered at once, one per 256-thread quarter of the
 // workgroup, each in its own line of LDS; all 256 K threads then add them
 // to their columns (x = tid + 256 K j, j < J) in projection order.
@@ -62,7 +131,8 @@ __global__ __launch_bounds__(256 * K) void ts_kernel(const float *proj, long lon
                                                      TsConsts consts, float *filtered,
                                                      float *__restrict__ result, const float *__restrict__ W,
                                                      const float2 *__restrict__ tw, int rows, int n, int Z,
-                                                     int logZ, int contrast) {
+                                                     int logZ, int contrast, const float2 *__restrict__ odd,
+                                                     float odd_g) {
   extern __shared__ float2 lines[];
   const int tid = threadIdx.x;
   const int sub = tid >> 8, lt = tid & 255;  // quarter and thread within it
@@ -72,6 +142,8 @@ __global__ __launch_bounds__(256 * K) void ts_kernel(const float *proj, long lon
   // projection, are staged in LDS behind the lines.
   const float2 *twp = tw;
   const float *Wp = W;
+  // odd widths: 16 floats of wave sums per quarter, behind the tables
+  float *ostash = reinterpret_cast<float *>(lines + K * L + (TAB ? Z : 0)) + 16 * sub;
   if constexpr (TAB) {
     float2 *tw_s = lines + K * L;
     float *W_s = reinterpret_cast<float *>(tw_s + Z / 2);
@@ -119,8 +191,19 @@ __global__ __launch_bounds__(256 * K) void ts_kernel(const float *proj, long lon
       }
       buf[lds_pad(__brev(static_cast<unsigned>(m)) >> rsh, psh)] = make_float2(a, c);
     }
+    if (odd) odd_sums(r0, r1, live, live && has1, n, absorb, odd, lt, ostash);
     __syncthreads();
     line_filter(buf, twp, Wp, Z, logZ, lt, psh);
+    if (odd) {
+      const float4 t = odd_total(ostash);
+      for (int x = lt; x < n; x += 256) {
+        float2 &e = buf[lds_pad(__brev(static_cast<unsigned>(s0 + x)) >> rsh, psh)];
+        const float2 d = odd_term(odd, odd_g, t, x);
+        e.x += d.x;
+        e.y += d.y;
+      }
+      __syncthreads();
+    }
 
     float scale = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
     if (contrast == CTAS_CONTRAST_REF) {
@@ -200,17 +283,19 @@ __device__ __forceinline__ float2 w2048(const float2 *__restrict__ tw, int m) {
 constexpr int kA = 9, kB = 17;               // padded row lengths of the exchanges
 constexpr int kLineA = 256 * kA;             // [t][k], k < 8
 constexpr int kLineB = 128 * kB;             // [k * 16 + m2][t1]
-constexpr size_t kTs2048Lds = sizeof(float2) * (2 * kLineA + 2 * kLineB);
+constexpr size_t kTs2048Lds = sizeof(float2) * (2 * kLineA + 2 * kLineB) + sizeof(float) * 32;  // + odd sums
 
 template <int J>
 __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long long proj_stride, int n_proj,
                                                      TsConsts consts, float *filtered,
                                                      float *__restrict__ result, const float *__restrict__ W,
-                                                     const float2 *__restrict__ tw, int rows, int n, int contrast) {
+                                                     const float2 *__restrict__ tw, int rows, int n, int contrast,
+                                                     const float2 *__restrict__ odd, float odd_g) {
   constexpr int Z = 2048;
   extern __shared__ float2 lds[];
   float2 *A = lds;                 // exchange 1, then the filtered lines [line][Z]
   float2 *B = lds + 2 * kLineA;    // exchange 2
+  float *ostash = reinterpret_cast<float *>(B + 2 * kLineB);  // odd widths: 16 floats per line
   const int tid = threadIdx.x;
   const int r0 = 4 * blockIdx.x;   // rows r0 .. r0 + 3
   const int s0 = (Z - n) >> 1;
@@ -269,6 +354,13 @@ __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long lon
 #pragma unroll
       for (int j = 0; j < 8; ++j) cur[l][j] = raw[l][j];
     if (p + 1 < n_proj) fetch(p + 1);
+    if (odd) {
+      const float *base = proj + static_cast<size_t>(p) * proj_stride;
+#pragma unroll
+      for (int l = 0; l < 2; ++l)
+        odd_sums(base + static_cast<size_t>(rr[2 * l]) * n, base + static_cast<size_t>(rr[2 * l + 1]) * n,
+                 rv[2 * l], rv[2 * l + 1], n, absorb, odd, tid, ostash + 16 * l);
+    }
     // pass 1: thread t, both lines, x[t + 256 j]
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
@@ -343,6 +435,21 @@ __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long lon
 #pragma unroll
       for (int k = 1; k < 8; ++k) y[l][k] = cmulc(y[l][k], tw1[k]);
       dft8<true>(y[l]);
+    }
+    if (odd) {
+#pragma unroll
+      for (int l = 0; l < 2; ++l) {
+        const float4 t = odd_total(ostash + 16 * l);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int x = tid + 256 * j - s0;
+          if (x >= 0 && x < n) {
+            const float2 d = odd_term(odd, odd_g, t, x);
+            y[l][j].x += d.x;
+            y[l][j].y += d.y;
+          }
+        }
+      }
     }
     __syncthreads();  // exchange 1 is read; the lines go in its place
     constexpr float scale0 = 1.0f / static_cast<float>(Z);
@@ -421,15 +528,17 @@ __device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
 // runs back with conjugate factors to the filtered x[t + 256 j]. Twiddles
 // (from the FP64 table) and window values stay in registers for the batch.
 constexpr int kP16 = 17;  // padded row length of the exchanges
-constexpr size_t kTs4096Lds = sizeof(float2) * 256 * kP16;
+constexpr size_t kTs4096Lds = sizeof(float2) * 256 * kP16 + sizeof(float) * 16;  // + odd sums
 
 template <int J>
 __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long long proj_stride, int n_proj,
                                                      TsConsts consts, float *filtered,
                                                      float *__restrict__ result, const float *__restrict__ W,
-                                                     const float2 *__restrict__ tw, int rows, int n, int contrast) {
+                                                     const float2 *__restrict__ tw, int rows, int n, int contrast,
+                                                     const float2 *__restrict__ odd, float odd_g) {
   constexpr int Z = 4096;
   extern __shared__ float2 xch[];
+  float *ostash = reinterpret_cast<float *>(xch + 256 * kP16);
   const int tid = threadIdx.x, lo4 = tid & 15, hi4 = tid >> 4;
   const int t0 = 2 * blockIdx.x;
   const bool has1 = t0 + 1 < rows;
@@ -471,6 +580,9 @@ __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long lon
 
   for (int p = 0; p < n_proj; ++p) {
     if (!kPrefetch) fetch(p);
+    if (odd)
+      odd_sums(proj + static_cast<size_t>(p) * proj_stride + row0, proj + static_cast<size_t>(p) * proj_stride + row1c,
+               true, has1, n, absorb, odd, tid, ostash);
     float2 v[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -529,6 +641,18 @@ __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long lon
 #pragma unroll
     for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], tw1[k]);
     dft16<true>(v);
+    if (odd) {
+      const float4 t = odd_total(ostash);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int x = tid + 256 * j - s0;
+        if (x >= 0 && x < n) {
+          const float2 d = odd_term(odd, odd_g, t, x);
+          v[j].x += d.x;
+          v[j].y += d.y;
+        }
+      }
+    }
     __syncthreads();  // the exchange is read; the line goes in its place
 #pragma unroll
     for (int j = 0; j < 16; ++j) xch[tid + 256 * j] = v[j];
@@ -585,15 +709,17 @@ __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long lon
 // two items per thread in each radix-16 pass; one exchange buffer of
 // 512 x 17 entries, which then holds the filtered line).
 constexpr int kTsPA8 = 33, kTsPB8 = 17;
-constexpr size_t kTs8192Lds = sizeof(float2) * 512 * kTsPB8;
+constexpr size_t kTs8192Lds = sizeof(float2) * 512 * kTsPB8 + sizeof(float) * 16;  // + odd sums
 
 template <int J>
 __global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long long proj_stride, int n_proj,
                                                      TsConsts consts, float *filtered,
                                                      float *__restrict__ result, const float *__restrict__ W,
-                                                     const float2 *__restrict__ tw, int rows, int n, int contrast) {
+                                                     const float2 *__restrict__ tw, int rows, int n, int contrast,
+                                                     const float2 *__restrict__ odd, float odd_g) {
   constexpr int Z = 8192;
   extern __shared__ float2 xb[];
+  float *ostash = reinterpret_cast<float *>(xb + 512 * kTsPB8);
   const int tid = threadIdx.x;
   const int t0 = 2 * blockIdx.x;
   const bool has1 = t0 + 1 < rows;
@@ -612,6 +738,7 @@ __global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long lon
   for (int p = 0; p < n_proj; ++p) {
     const float *r0 = proj + static_cast<size_t>(p) * proj_stride + row0;
     const float *r1 = proj + static_cast<size_t>(p) * proj_stride + row1c;
+    if (odd) odd_sums(r0, r1, true, has1, n, absorb, odd, tid, ostash);
     float2 v[32];
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
@@ -690,6 +817,18 @@ __global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long lon
 #pragma unroll
     for (int k = 1; k < 32; ++k) v[k] = cmulc(v[k], w8192(tw, tid * k));
     dft32<true>(v);
+    if (odd) {
+      const float4 t = odd_total(ostash);
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        const int x = tid + 256 * j - s0;
+        if (x >= 0 && x < n) {
+          const float2 d = odd_term(odd, odd_g, t, x);
+          v[j].x += d.x;
+          v[j].y += d.y;
+        }
+      }
+    }
     __syncthreads();  // the exchange is read; the line goes in its place
 #pragma unroll
     for (int j = 0; j < 32; ++j) xb[tid + 256 * j] = v[j];
@@ -744,7 +883,8 @@ __global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long lon
 
 hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const TsConsts &consts,
                      float *filtered, float *result, const float *W, const float2 *tw, int rows, int n, int Z,
-                     int logZ, int contrast, int cus, int lds_max, hipStream_t s) {
+                     int logZ, int contrast, int cus, int lds_max, const float2 *odd, float odd_g,
+                     hipStream_t s) {
   if (rows <= 0 || n_proj <= 0) return hipSuccess;
   if (n_proj > kTsBatch || n > kTsMaxWidth || Z > kTsMaxWidth) return hipErrorInvalidValue;
   if (Z == 2048 && kTs2048Lds <= static_cast<size_t>(lds_max)) {
@@ -752,16 +892,16 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
     const int J = (n + 255) / 256;  // n <= 2048: J <= 8
     if (J <= 4)
       hipLaunchKernelGGL(ts2048_kernel<4>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
-                         result, W, tw, rows, n, contrast);
+                         result, W, tw, rows, n, contrast, odd, odd_g);
     else
       hipLaunchKernelGGL(ts2048_kernel<8>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
-                         result, W, tw, rows, n, contrast);
+                         result, W, tw, rows, n, contrast, odd, odd_g);
     return hipGetLastError();
   }
   if (Z == 8192 && n <= 4096 && kTs8192Lds <= static_cast<size_t>(lds_max)) {
     const dim3 grid((rows + 1) / 2), block(256);
     hipLaunchKernelGGL(ts8192_kernel<16>, grid, block, kTs8192Lds, s, proj, proj_stride, n_proj, consts, filtered,
-                       result, W, tw, rows, n, contrast);
+                       result, W, tw, rows, n, contrast, odd, odd_g);
     return hipGetLastError();
   }
   if (Z == 4096 && kTs4096Lds <= static_cast<size_t>(lds_max)) {
@@ -769,10 +909,10 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
     const int J = (n + 255) / 256;  // n <= 4096: J <= 16
     if (J <= 8)
       hipLaunchKernelGGL(ts4096_kernel<8>, grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
-                         result, W, tw, rows, n, contrast);
+                         result, W, tw, rows, n, contrast, odd, odd_g);
     else
       hipLaunchKernelGGL(ts4096_kernel<16>, grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
-                         result, W, tw, rows, n, contrast);
+                         result, W, tw, rows, n, contrast, odd, odd_g);
     return hipGetLastError();
   }
   // Projections in flight per workgroup (K): up to 4, at most the batch, as
@@ -784,14 +924,15 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
   const int psh = line_psh(Z);
   const size_t line_bytes = sizeof(float2) * padded_len(Z, psh);
   const size_t tab_bytes = (sizeof(float2) / 2 + sizeof(float)) * static_cast<size_t>(Z);
+  const size_t stash_bytes = sizeof(float) * 16;  // odd-width wave sums per line
   const int pairs = (rows + 1) / 2;
   const size_t per_cu = (pairs + cus - 1) / cus;
   auto fits = [&](size_t b) { return b <= static_cast<size_t>(lds_max) && per_cu * b <= 160 * 1024; };
-  const bool tab = fits(line_bytes + tab_bytes);
+  const bool tab = fits(line_bytes + stash_bytes + tab_bytes);
   const size_t extra = tab ? tab_bytes : 0;
   int K = 1;
-  while (K < 4 && K < n_proj && 2 * K * Z <= 8192 && fits(2 * K * line_bytes + extra)) K *= 2;
-  const size_t lds = line_bytes * K + extra;
+  while (K < 4 && K < n_proj && 2 * K * Z <= 8192 && fits(2 * K * (line_bytes + stash_bytes) + extra)) K *= 2;
+  const size_t lds = (line_bytes + stash_bytes) * K + extra;
   if (lds > static_cast<size_t>(lds_max)) return hipErrorInvalidValue;
   const dim3 grid(pairs), block(256 * K);
   const int J = (n + 256 * K - 1) / (256 * K);
@@ -799,10 +940,10 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
   do {                                                                                                        \
     if (tab)                                                                                                  \
       hipLaunchKernelGGL((ts_kernel<JJ, KK, true>), grid, block, lds, s, proj, proj_stride, n_proj, consts,   \
-                         filtered, result, W, tw, rows, n, Z, logZ, contrast);                                \
+                         filtered, result, W, tw, rows, n, Z, logZ, contrast, odd, odd_g);                    \
     else                                                                                                      \
       hipLaunchKernelGGL((ts_kernel<JJ, KK, false>), grid, block, lds, s, proj, proj_stride, n_proj, consts,  \
-                         filtered, result, W, tw, rows, n, Z, logZ, contrast);                                \
+                         filtered, result, W, tw, rows, n, Z, logZ, contrast, odd, odd_g);                    \
   } while (0)
   // K = 4 only for Z <= 2048 (so J <= 2), K = 2 for Z <= 4096 (J <= 8).
   if (K == 4) {
@@ -848,6 +989,8 @@ struct ctas_ts {
   int cus = 1, lds_max = 65536;  // compute units, LDS bytes per workgroup
   float *d_W = nullptr;
   float2 *d_tw = nullptr;
+  float2 *d_odd = nullptr;  // odd widths: basis of the middle-bin correction
+  float odd_g = 0.0f;
 };
 
 extern "C" {
@@ -903,13 +1046,18 @@ int ctas_ts_create(ctas_ts **ph, int rows, int pixels, int contrast, int filter,
     const double a = -2.0 * M_PI * m / h->Z;
     tw[m] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
   }
+  std::vector<float2> uv;
+  const bool odd = ts_odd_basis(filter, filter_param, pixels, h->Z, uv, h->odd_g);
   if (hipMalloc(&h->d_W, sizeof(float) * W.size()) != hipSuccess ||
-      hipMalloc(&h->d_tw, sizeof(float2) * tw.size()) != hipSuccess) {
+      hipMalloc(&h->d_tw, sizeof(float2) * tw.size()) != hipSuccess ||
+      (odd && hipMalloc(&h->d_odd, sizeof(float2) * uv.size()) != hipSuccess)) {
     (void)hipFree(h->d_W);
+    (void)hipFree(h->d_tw);
     return fail(CTAS_E_NOMEM, "device allocation failed");
   }
   CTAS_HIP_TRY(hipMemcpy(h->d_W, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice));
   CTAS_HIP_TRY(hipMemcpy(h->d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
+  if (odd) CTAS_HIP_TRY(hipMemcpy(h->d_odd, uv.data(), sizeof(float2) * uv.size(), hipMemcpyHostToDevice));
   *ph = h.release();
   return CTAS_OK;
 }
@@ -918,6 +1066,7 @@ int ctas_ts_destroy(ctas_ts *h) {
   if (!h) return CTAS_OK;
   (void)hipFree(h->d_W);
   (void)hipFree(h->d_tw);
+  (void)hipFree(h->d_odd);
   delete h;
   return CTAS_OK;
 }
@@ -941,7 +1090,7 @@ int ctas_ts_add_dev(ctas_ts *h, const float *d_proj, int n_proj, const float *an
     for (int i = 0; i < np; ++i) k.k[i] = ts_consts(h->n, center, angles[p0 + i], plane);
     const hipError_t e = launch_ts(d_proj + p0 * stride, stride, np, k, d_filtered ? d_filtered + p0 * stride : nullptr,
                                    d_result, h->d_W, h->d_tw, h->rows, h->n, h->Z, h->logZ, h->contrast, h->cus,
-                                   h->lds_max, s);
+                                   h->lds_max, h->d_odd, h->odd_g, s);
     if (e != hipSuccess) return fail(CTAS_E_DEVICE, std::string("tomosynthesis: ") + hipGetErrorString(e));
   }
   return CTAS_OK;

@@ -86,10 +86,10 @@ int mirror_slots(const std::vector<double> &sc64, std::vector<MirrorSlot> &slots
   }
   slots.clear();
   for (int i = 0; i < n; ++i)
-    if (partner[i] > i) slots.push_back(MirrorSlot{sc64[2 * i], sc64[2 * i + 1], i, partner[i], {0, 0}});
+    if (partner[i] > i) slots.push_back(MirrorSlot{sc64[2 * i], sc64[2 * i + 1], i, partner[i], 0, 0});
   const int paired = static_cast<int>(slots.size());
   for (int i = 0; i < n; ++i)
-    if (partner[i] < 0) slots.push_back(MirrorSlot{sc64[2 * i], sc64[2 * i + 1], i, -1, {0, 0}});
+    if (partner[i] < 0) slots.push_back(MirrorSlot{sc64[2 * i], sc64[2 * i + 1], i, -1, 0, 0});
   return paired;
 }
 
@@ -254,6 +254,22 @@ int ts_window(int filter, float param, int n, int &Z, std::vector<float> &out) {
   out.resize(Z);
   for (int k = 0; k < Z; ++k) out[k] = static_cast<float>(a[k].real());
   return CTAS_OK;
+}
+
+bool ts_odd_basis(int filter, float param, int n, int Z, std::vector<float2> &uv, float &gain) {
+  uv.clear();
+  gain = 0.0f;
+  if (n < 3 || (n & 1) == 0) return false;
+  std::vector<float> W;
+  if (filter_fill(filter, param, n, W) != CTAS_OK) return false;
+  const int m = (n - 1) / 2;
+  gain = static_cast<float>(static_cast<double>(Z) * (static_cast<double>(W[m]) - W[m + 1]) / n);
+  uv.resize(n);
+  for (int x = 0; x < n; ++x) {
+    const double sg = (x & 1) ? -1.0 : 1.0, a = M_PI * x / n;
+    uv[x] = make_float2(static_cast<float>(sg * std::cos(a)), static_cast<float>(sg * std::sin(a)));
+  }
+  return true;
 }
 
 }  // namespace ctas

@@ -96,6 +96,12 @@ int ctas_fbp_clone(const ctas_fbp *src, ctas_fbp **h);
 /* ~CTrec (lct.cpp:549-555) */
 int ctas_fbp_destroy(ctas_fbp *h);
 int ctas_fbp_info(const ctas_fbp *h, int *w, int *n_angles, int *zidth, int *flags);
+/* CTrec::recShape (ct.h:272-274): the slice rows left after cropping
+ * ceil(|center|) rows at the top and at the bottom, and all w columns. */
+int ctas_fbp_rec_shape(const ctas_fbp *h, float center, int *rows, int *cols);
+/* CTrec::recCrop (ct.h:276-279): that crop as the half-open row range
+ * [row_begin, row_end) of the w x w slice; columns are not cropped. */
+int ctas_fbp_rec_crop(const ctas_fbp *h, float center, int *row_begin, int *row_end);
 /* CTrec::setPhysics (lct.cpp:558-566): pixel size and wavelength in m. */
 int ctas_fbp_set_physics(ctas_fbp *h, float pixel_size, float lambda);
 int ctas_fbp_get_reccof(const ctas_fbp *h, float *recCof);

@@ -252,14 +252,19 @@ def ts_add_numpy(proj: np.ndarray, result: np.ndarray, W: np.ndarray, center: fl
                  angle: float, plane: int) -> tuple[np.ndarray, np.ndarray]:
     """ts_add (lct.cpp:739-787) vectorised with numpy's FFT, the CPU-baseline
     port: -log (ABS), every row filtered by an n-point real FFT with the
-    window W (FFTW's halfcomplex product for an even window), running sums
-    (REF), and the column map in float32 in the reference's order. Returns
-    (filtered projection, result) as new arrays."""
+    window W applied as FFTW's halfcomplex product (the real part of bin k
+    times W[k], its imaginary part times W[n-k]), running sums (REF), and the
+    column map in float32 in the reference's order. Returns (filtered
+    projection, result) as new arrays."""
     P = np.asarray(proj, np.float32)
     n = P.shape[1]
     if contrast == ABS:
         P = -np.log(P)
-    F = np.fft.irfft(np.fft.rfft(P, axis=1) * np.asarray(W, np.float32)[: n // 2 + 1], n, axis=1).astype(np.float32)
+    Wf = np.asarray(W, np.float32)
+    k = np.arange(n // 2 + 1)
+    X = np.fft.rfft(P, axis=1)
+    X = X.real * Wf[k] + 1j * (X.imag * Wf[(n - k) % n])
+    F = np.fft.irfft(X, n, axis=1).astype(np.float32)
     if contrast == REF:
         F = np.cumsum(F, axis=1, dtype=np.float32)
     f32 = np.float32

@@ -1,0 +1,142 @@
+"""Parity at the benchmark workloads' own shapes, through the C ABI.
+
+* Batched FP32 FBP, 2048 px x 1800 angles, HANN, ABS, c = +3.37: two slices
+  through the captured hipGraph with the autotuned tile, against the
+  oracle's CTAS CPU path (prepare_sino + reconstructOnCPU, lct.cpp:569-597,
+  688-729) on 32 slice rows per slice.
+* Uneven batch: one sinogram of each width 512 .. 4096 with angle counts
+  from {360, 900, 1440, 1800}, in one persistent launch, against the oracle.
+* The 81-centre sweep (-20 .. 20 px, step 0.5) at 2048 px x 1800 angles:
+  AverageGradient scores (lct.cpp:872-915) against the oracle's at three of
+  the centres.
+
+Bars: 1e-4 relative RMS end to end (the device FFT filter differs from the
+oracle's FP64 FFT at ~1e-6); bit-identical rows when the backprojection is
+given the oracle's filtered sinogram (FLT contrast).
+"""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import ctas_amd as ca
+import oracle
+from tests.conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+ROWS = np.arange(16, 2048, 64)  # 32 slice rows spread over the slice
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if ca.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+_CFG2 = {}
+
+
+def _config2():
+    """Two slices of the batched FP32 workload: 20 seeded ellipses (semi-axes
+    x4), Poisson 1e5, HANN padded to 4096, c = +3.37."""
+    if not _CFG2:
+        N, w, c, S = 1800, 2048, 3.37, 2
+        P = np.stack([oracle.phantom_sino(N, w, c, oracle.random_ellipses(20, w, 4.0, 200 + s), 1e5, 200 + s)
+                      for s in range(S)])
+        W = oracle.filter_window(5, 0.0, w, ca.ABS)
+        Pf = np.stack([oracle.prepare(P[s], W, ca.ABS) for s in range(S)])
+        ref = np.stack([oracle.bp_ctas_rows(Pf[s], oracle.angles(N), c, ROWS) for s in range(S)])
+        _CFG2.update(P=P, Pf=Pf, ref=ref, N=N, w=w, c=c, S=S)
+    return _CFG2
+
+
+def test_config2_shape_graph_autotuned(torch_cuda):
+    """2048 x 1800 x 2 slices through graph_capture / graph_launch with the
+    tile the autotuner picks; then the same tile on the oracle's filtered
+    sinograms (FLT) must give the oracle's rows bit for bit."""
+    torch = torch_cuda
+    d = _config2()
+    N, w, c, S = d["N"], d["w"], d["c"], d["S"]
+    d_in = torch.from_numpy(d["P"]).cuda()
+    d_out = torch.zeros((S, w, w), dtype=torch.float32, device="cuda")
+    rec = ca.CTrec(w, N, ca.ABS, "HANN")
+    rec.autotune(S)
+    kind, ac = rec.tiles()
+    rec.graph_capture(d_in, d_out, [c] * S)
+    for rep in range(2):
+        d_out.zero_()
+        torch.cuda.synchronize()
+        rec.graph_launch()
+        torch.cuda.synchronize()
+        out = d_out.cpu().numpy()
+        for s in range(S):
+            err = rel_rms(out[s][ROWS], d["ref"][s])
+            print(f"tile kind {kind} / {ac} angles per pass, replay {rep}, slice {s}: rel_rms {err:.2e}")
+            assert err <= 1e-4
+    assert (rec.status(S) == 0).all()
+    # the backprojection alone, same tile, on identical filtered input
+    flt = ca.CTrec(w, N, ca.FLT, "HANN")
+    flt.set_tiles(kind, ac)
+    d_f = torch.from_numpy(d["Pf"]).cuda()
+    flt.graph_capture(d_f, d_out, [c] * S)
+    flt.graph_launch()
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy()
+    for s in range(S):
+        assert np.array_equal(out[s][ROWS], d["ref"][s]), f"slice {s}: {(out[s][ROWS] != d['ref'][s]).sum()} pixels"
+
+
+def test_config4_uneven_batch_against_oracle(torch_cuda):
+    """One item of each width class (512 .. 4096: Z from 1024 to 8192, gain
+    4/3 for 768, 1536 and 3072) with angle counts from {360, 900, 1440,
+    1800}, HANN, ABS, in one persistent launch; 8 rows of each slice against
+    the oracle. Then the same batch with FLT on the oracle's filtered
+    sinograms: those rows bit for bit."""
+    torch = torch_cuda
+    widths = [512, 768, 1024, 1536, 2048, 3072, 4096]
+    angles = [360, 900, 1440, 1800, 900, 360, 1440]
+    centres = [1.5, -2.25, 3.37, 0.0, -7.5, 12.0, -0.5]
+    items, flt_items, refs, rows_of = [], [], [], []
+    for k, (w, N, c) in enumerate(zip(widths, angles, centres)):
+        P = oracle.phantom_sino(N, w, c, oracle.random_ellipses(20, w, 4.0 * w / 2048, 700 + k), 1e5, 700 + k)
+        Pf = oracle.prepare(P, oracle.filter_window(5, 0.0, w, ca.ABS), ca.ABS)
+        rows = np.linspace(w // 8, w - w // 8, 8).astype(np.int32)
+        refs.append(oracle.bp_ctas_rows(Pf, oracle.angles(N), c, rows))
+        rows_of.append(rows)
+        items.append((torch.from_numpy(P).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), c))
+        flt_items.append((torch.from_numpy(Pf).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), c))
+    ca.batch_uneven(items, ca.ABS, 5, 0.0, 180.0, 0)
+    ca.batch_uneven(flt_items, ca.FLT, 5, 0.0, 180.0, 0)
+    torch.cuda.synchronize()
+    for k, w in enumerate(widths):
+        got = items[k][1].cpu().numpy()[rows_of[k]]
+        err = rel_rms(got, refs[k])
+        print(f"w={w} N={angles[k]} c={centres[k]}: rel_rms {err:.2e}")
+        assert err <= 1e-4
+        got_f = flt_items[k][1].cpu().numpy()[rows_of[k]]
+        assert np.array_equal(got_f, refs[k]), f"w={w}: {(got_f != refs[k]).sum()} pixels differ"
+
+
+def test_sweep_81_centres_at_2048x1800():
+    """The 81 trial centres of the sweep at 2048 px x 1800 angles (sigma =
+    w/(N-1) > 1: the one-sided Gaussian smoothing is on); three of them
+    against the oracle's AverageGradient score."""
+    N, w = 1800, 2048
+    P = oracle.phantom_sino(N, w, 3.37, oracle.random_ellipses(20, w, 4.0, 811), 1e5, 811)
+    step = 180.0 / (N - 1)
+    rec = ca.CTrec(w, N, ca.ABS, "RAMP", anarc=step)
+    rec.sino(P)
+    axes = (0.5 * (w - 1) + np.arange(-20.0, 20.01, 0.5)).astype(np.float32)
+    got = rec.axis_scores(axes)
+    assert got.shape == (81,) and np.isfinite(got).all()
+    Pf = oracle.prepare(P, oracle.filter_window(1, 0.0, w, ca.ABS), ca.ABS)
+    sigma = float(np.float32(oracle.angO(step, N) * np.float32(w) / np.pi))
+    assert sigma > 1.0
+    sc = oracle.angles(N, step)
+    picks = [0, 47, 80]  # -20, +3.5, +20 px
+    with ThreadPoolExecutor(3) as ex:
+        ref = list(ex.map(lambda k: oracle.ag_score(Pf, sc, sigma, float(axes[k])), picks))
+    for k, r in zip(picks, ref):
+        print(f"axis {axes[k]:.1f}: gpu {got[k]:.6e} oracle {r:.6e}")
+        assert abs(got[k] - r) <= 2e-4 * abs(r)

@@ -149,16 +149,16 @@ def test_reduced_precision_storage(store, interp):
 @pytest.mark.parametrize("store", [ca.STORE_FP16, ca.STORE_BF16])
 @pytest.mark.parametrize("kind", [ca.TILE_64x64, ca.TILE_128x64, ca.TILE_128x128])
 @pytest.mark.parametrize("w,N,c,arc", [(384, 360, 0.0, 180.0), (257, 181, 3.37, 180.0),
-                                       (320, 400, -5.25, 360.0), (256, 240, 2.0, 150.0)])
+                                       (320, 400, -5.5, 360.0), (256, 240, 2.0, 150.0)])
 def test_reduced_precision_ctas_paths(store, kind, w, N, c, arc):
     """CTAS sampling of FP16/BF16 storage against the FP64 oracle. 180 and
     360 degree arcs take the mirrored path (every angle but 0, 90 and 270
     degrees has a partner at 180 - theta), including an odd width, whose
     mirror column is w - 1 - x, and centre offsets; a 150 degree arc has no
-    pairs and takes the row-pair path. Every tile kind. (A coordinate that
-    is exactly a negative integer is read one bin off the reference's
-    truncation by the fixed-point paths; a centre of -5.5 px makes whole rows
-    of such coordinates at 270 degrees, so the 360-degree case uses -5.25.)"""
+    pairs and takes the row-pair path. Every tile kind. A centre of -5.5 px
+    on the 360-degree arc makes whole rows of coordinates that are exactly
+    negative integers at 270 degrees, which the reference truncates to
+    themselves (lct.cpp:711-713)."""
     if arc == 360.0:
         # second half-turn: P(theta + pi, s) = P(theta, -s), s taken about
         # the axis at k + 1/2 = w/2 + c (to the nearest bin)
@@ -182,6 +182,43 @@ def test_reduced_precision_ctas_paths(store, kind, w, N, c, arc):
     # per-pixel bound: the mirrored path forms each coordinate from terms
     # rounded once; the row-pair path accumulates rounded steps (~1e-3 px)
     assert worst <= (1.5e-2 if arc in (180.0, 360.0) else 5e-2)
+
+
+@pytest.mark.parametrize("store", [ca.STORE_FP16, ca.STORE_BF16])
+@pytest.mark.parametrize("kind", [ca.TILE_64x64, ca.TILE_128x64, ca.TILE_128x128])
+@pytest.mark.parametrize("table", ["quarter_turns", "unpaired"])
+@pytest.mark.parametrize("c", [-5.5, -6.0, 4.5])
+def test_fixed_point_exact_integer_coordinates(store, kind, table, c):
+    """Known answer at angles of 0 and +-1 sines and cosines, where every
+    ray coordinate of a pixel is exactly an integer or a half-integer, and
+    at 270 degrees with c < 0 rows of them are negative integers: the
+    reference reads the sample at the flat offset theta*w + (long)t
+    (lct.cpp:711-713), truncation toward zero, so t = -3 reads the previous
+    row's third sample from the end. The samples are small integers (exact
+    in FP16 and BF16) and the contrast FLT (no filter), so every sum is
+    exact and the fixed-point paths must equal the oracle bit for bit. The
+    quarter turns (0, 90, 180, 270 degrees) pair 0 with 180 and take the
+    mirrored path; the unpaired table (270, 0, 90 degrees) takes the row-pair
+    path."""
+    w = 200
+    if table == "quarter_turns":
+        sc = np.array([[0, 1], [1, 0], [0, -1], [-1, 0]], np.float32)
+    else:
+        sc = np.array([[-1, 0], [0, 1], [1, 0]], np.float32)
+    N = sc.shape[0]
+    rng = np.random.default_rng(17)
+    P = rng.integers(-8, 9, (N, w)).astype(np.float32)
+    rec = ca.CTrec(w, N, ca.FLT, "RAMP", store=store, anarc=360.0)
+    rec.set_angles(sc)
+    rec.set_tiles(kind, 32)
+    _, out = rec.reconstruct(P, c)
+    ref = oracle.bp_ctas(P, sc, c)  # float arithmetic, exact for these tables
+    hp = w // 2
+    rows = np.arange(w)[:, None]
+    neg = (2 * hp + 2 * c - rows < 0) & (ref != 0)   # 270 degrees: t = 2hp + 2c - j
+    if c < 0:
+        assert neg.any(), "the case must hold in-mask pixels with negative integer coordinates"
+    assert np.array_equal(out[0], ref), f"{int((out[0] != ref).sum())} pixels differ"
 
 
 @pytest.mark.parametrize("store", [ca.STORE_FP32, ca.STORE_FP16])
@@ -401,3 +438,73 @@ def test_raxis_pair_matches_oracle(c0):
     assert abs(got - (0.5 * (w - 1) + c0)) <= 0.5
     with pytest.raises(ca.CtasError, match="averaging to zero"):
         ca.raxis_pair(np.ones((4, 32), np.float32), np.ones((4, 32), np.float32), 0.25)
+
+
+@pytest.mark.parametrize("c", [0.0, 0.5, -3.0, 3.37, -20.0])
+def test_rec_shape_and_crop(c):
+    """recShape / recCrop (ct.h:272-279): ceil(|c|) rows cropped at the top
+    and at the bottom, no columns, and the crop of a reconstructed slice has
+    the shape recShape gives."""
+    N, w = 90, 128
+    rec = ca.CTrec(w, N, ca.ABS, "RAMP")
+    r = int(np.ceil(abs(np.float32(c))))
+    assert rec.rec_shape(c) == (w - 2 * r, w)
+    keep = rec.rec_crop(c)
+    assert (keep.start, keep.stop) == (r, w - r)
+    _, out = rec.reconstruct(_phantom(N, w, c, 6, 1e5, seed=8), c)
+    assert out[0][keep].shape == rec.rec_shape(c)
+
+
+def test_clone_keeps_geometry_and_state_apart():
+    """CTrec(CTrec&) (lct.cpp:529-545): the clone has the geometry, window,
+    recCof and angle table of its source, reconstructs the same slices, and
+    keeps no reference to the source's filtered sinogram."""
+    N, w = 120, 160
+    rec = ca.CTrec(w, N, ca.ABS, "HANN")
+    rec.set_physics(2e-6, 1e-10)
+    sc = oracle.angles(N, 170.0)
+    rec.set_angles(sc)
+    P = _phantom(N, w, 1.5, 6, 1e5, seed=21)
+    rec.sino(P)
+    cl = rec.clone()
+    assert cl.rec_cof() == rec.rec_cof()
+    assert np.array_equal(cl.window(), rec.window())
+    assert np.array_equal(cl.angles(), sc)
+    with pytest.raises(ca.CtasError) as e:
+        cl.repeat([1.5])          # the source's sino() is not the clone's
+    assert e.value.code == ca.CTAS_E_STATE
+    _, a = rec.reconstruct(P, 1.5)
+    _, b = cl.reconstruct(P, 1.5)
+    assert np.array_equal(a, b)
+    del rec                          # the clone outlives its source
+    _, b2 = cl.reconstruct(P, 1.5)
+    assert np.array_equal(a, b2)
+
+
+def test_raxis_algorithm2_from_the_registration_seed():
+    """raxis algorithm 2 (lct.cpp:1163-1164): the 0/180 registration seeds
+    the 8 px sweep, then bisection; GPU against the oracle and against the
+    phantom's true centre."""
+    N, w, c0 = 361, 256, 6.0
+    P = np.exp(-oracle.phantom_sino(N, w, c0, _dots(w, 10, 4), 0.0, 0)).astype(np.float32)
+    step = 180.0 / (N - 1)
+    got = ca.raxis(P, step, 0.2, 2)    # 0.25 would leave a box below half the width
+    ref = oracle.raxis(P, step, 0.2, 2)
+    assert abs(got - ref) <= 0.25, (got, ref)
+    assert abs(got - 0.5 * (w - 1) - c0) <= 0.5
+
+
+@pytest.mark.parametrize("c0", [18.0, -17.5])
+def test_raxis_directional_walk(c0):
+    """A centre outside the first sweep: maxDev = 0.04 of 256 px gives three
+    trials at -8, 0 and 8 px (lct.cpp:1186-1190), whose minimum lies at an
+    edge, so the search walks from that edge in 8 px steps until the score
+    rises (lct.cpp:1192-1207) and bisects there; GPU against the oracle and
+    the true centre."""
+    N, w = 361, 256
+    P = np.exp(-oracle.phantom_sino(N, w, c0, _dots(w, 10, 5), 0.0, 0)).astype(np.float32)
+    step = 180.0 / (N - 1)
+    got = ca.raxis(P, step, 0.04, 1)
+    ref = oracle.raxis(P, step, 0.04, 1)
+    assert abs(got - ref) <= 0.25, (got, ref)
+    assert abs(got - 0.5 * (w - 1) - c0) <= 0.5

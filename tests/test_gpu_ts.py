@@ -71,6 +71,28 @@ def test_ts_add_matches_oracle(rows, n, contrast):
                1e-4 if contrast == ca.REF else 1e-5)
 
 
+@pytest.mark.parametrize("n", [9, 257, 769, 1501, 2049, 3001])
+@pytest.mark.parametrize("filt", ["WELCH", "PARZEN", "GAUSS:0.4", "LANCKZOS"])
+def test_ts_add_odd_widths(n, filt):
+    """Odd widths with windows that differ between the middle bins m and
+    m + 1 = n - m (f = 2(n-k)/n at k = m, lct.cpp:147-148): FFTW's
+    half-complex product scales the real and imaginary parts of bin m
+    differently, which no even window reproduces (up to 9e-4 of the row
+    maximum at n = 257 with WELCH). Widths on every kernel: 9 and 257 the
+    radix-2 LDS kernel, 769 and 1501 the 2048- and 4096-point register
+    kernels (padded), 2049 and 3001 the 8192-point one."""
+    rows = 3
+    P = _proj(rows, n, n + 5, ca.ABS)
+    R0 = np.zeros((rows, n), np.float32)
+    f, p = ca.parse_filter(filt)
+    W = oracle.filter_fill(f, p, n)
+    a = np.float32(0.3)
+    fref, rref = oracle.ts_add(P, R0, W, 1.5, ca.ABS, a, 7)
+    Pd, Rd = P.copy(), R0.copy()
+    ca.ts_add(Pd, Rd, filt, 0.0, 1.5, ca.ABS, float(a), 7)
+    _check(Pd, Rd, fref, rref, f"n={n} {filt}")
+
+
 @pytest.mark.parametrize("filt", ["RAMP", "KAISER:2.5", "GAUSS:0.4", "BLACKMAN"])
 def test_ts_add_filters(filt):
     rows, n = 6, 384

@@ -332,6 +332,29 @@ def test_ring_filter_invariants():
     assert np.array_equal(oracle.ring_filter(Q, 0), Q)
 
 
+@pytest.mark.parametrize("box", [1, 3, 8])
+def test_ring_filter_edge_columns_closed_form(box):
+    """RingFilter::apply (lct.cpp:309-325) in closed form: each column moves
+    by the box mean of the column means minus its own mean. The box is
+    2 box + 1 columns wide in the interior and clipped at the edges, where
+    the sum over the columns inside is divided by their number box + x + 1
+    (lct.cpp:318-319): column x < box takes the mean of avg[0 .. x + box],
+    column w-1-x the mean of avg[w-1-x-box .. w-1]."""
+    rng = np.random.default_rng(box)
+    N, w = 30, 41
+    P = (rng.random((N, w)) + np.linspace(0, 3, w)[None, :] ** 2).astype(np.float32)
+    avg = P.astype(np.float64).mean(0)
+    tmp = np.empty(w)
+    for x in range(w):
+        lo, hi = max(0, x - box), min(w - 1, x + box)
+        tmp[x] = avg[lo:hi + 1].mean()
+    exp = P.astype(np.float64) + (tmp - avg)[None, :]
+    out = oracle.ring_filter(P, box)
+    assert np.abs(out - exp).max() <= 2e-6 * np.abs(exp).max()
+    # a wrong edge count (e.g. 2 box + 1 there too) moves the edge columns
+    assert np.abs(exp[:, 0] - (P[:, 0] + avg[:box + 1].sum() / (2 * box + 1) - avg[0])).max() > 1e-3
+
+
 def test_phantom_noise_statistics():
     w, N = 64, 200
     ell = np.array([[0, 0, 20, 20, 0, 0.02]])

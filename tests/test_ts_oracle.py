@@ -63,6 +63,53 @@ def test_ts_filter_matches_numpy_fft(n, ft):
     np.testing.assert_array_equal(res, filt)  # angle 0: identity map
 
 
+def _fftw_halfcomplex_filter(P, W):
+    """filter_line (lct.cpp:280-291) from FFTW's documented r2r kinds: R2HC
+    stores r0, r1 .. r_{n/2}, i_{(n+1)/2-1} .. i_1 (X_k = r_k + i i_k, the
+    unnormalised forward DFT); the window multiplies that array position by
+    position; HC2R of the product, divided by n."""
+    n = P.shape[1]
+    X = np.fft.fft(P.astype(np.float64), axis=1)
+    hc = np.zeros(P.shape, np.float64)
+    for k in range(n // 2 + 1):
+        hc[:, k] = X[:, k].real
+    for k in range(1, (n + 1) // 2):
+        hc[:, n - k] = X[:, k].imag
+    hc *= W.astype(np.float64)[None, :]
+    Y = np.zeros(P.shape, np.complex128)
+    Y[:, 0] = hc[:, 0]
+    for k in range(1, (n + 1) // 2):
+        Y[:, k] = hc[:, k] + 1j * hc[:, n - k]
+        Y[:, n - k] = hc[:, k] - 1j * hc[:, n - k]
+    if n % 2 == 0:
+        Y[:, n // 2] = hc[:, n // 2]
+    return np.fft.ifft(Y, axis=1).real
+
+
+@pytest.mark.parametrize("n", [9, 15, 257, 300])
+@pytest.mark.parametrize("ft,p", [(3, 0.0), (4, 0.0), (10, 0.4), (8, 0.0)])
+def test_ts_filter_odd_widths_halfcomplex(n, ft, p):
+    """Windows whose middle bins differ at odd n (WELCH, PARZEN, GAUSS,
+    LANCKZOS: f = 2(n-k)/n at k = (n-1)/2): the oracle's filter against FFTW's
+    half-complex product, and the numpy port against both."""
+    rng = np.random.default_rng(n + ft)
+    P = rng.standard_normal((4, n)).astype(np.float32)
+    W = oracle.filter_fill(ft, p, n)
+    ref = _fftw_halfcomplex_filter(P, W)
+    filt, _ = oracle.ts_add(P, np.zeros_like(P), W, 0.0, PHS, 0.0, 0)
+    np.testing.assert_allclose(filt, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+    fn, _ = oracle.ts_add_numpy(P, np.zeros_like(P), W, 0.0, PHS, 0.0, 0)
+    np.testing.assert_allclose(fn, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+    if n % 2:
+        # the even-window convolution misses the middle-bin term
+        m = (n - 1) // 2
+        We = W.astype(np.float64)
+        We[m] = We[m + 1] = 0.5 * (W[m] + W[m + 1])
+        even = np.fft.ifft(np.fft.fft(P.astype(np.float64), axis=1) * We, axis=1).real
+        if ft in (3, 4):  # WELCH, PARZEN: the term is well above the bar
+            assert np.abs(even - ref).max() > 1e-5 * np.abs(ref).max()
+
+
 def _identity(n):
     return np.ones(n, np.float32)
 
