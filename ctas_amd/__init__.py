@@ -120,6 +120,7 @@ def lib() -> C.CDLL:
         "ctas_denoise_dev": (i, [vp, vp, i, i, i, f, vp]),
         "ctas_ring_filter": (i, [fp, i, i, i]),
         "ctas_ring_filter_dev": (i, [vp, i, i, i, i, vp]),
+        "ctas_ring_filter_dev_noalloc": (i, [vp, i, i, i, i, vp]),
         "ctas_axis_scores": (i, [vp, fp, i, fp]),
         "ctas_raxis": (i, [fp, i, i, f, f, i, fp]),
         "ctas_raxis_pair": (i, [fp, fp, i, i, f, fp]),

@@ -6,10 +6,25 @@
 // sequential float implementation bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "ctas_internal.hpp"
 
 namespace ctas {
 namespace {
+
+// Column means of ctas_ring_filter_dev_noalloc: a file-scope device array,
+// one copy per GPU, held until the process ends. Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): each device's address comes from hipGetSymbolAddress, taken
+// once per device with that device current.
+constexpr long long kRingScratchFloats = 1LL << 20;  // 4 MiB: S * w <= 1,048,576
+__device__ __align__(256) float scratch[kRingScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
 
 __global__ __launch_bounds__(256) void column_mean_kernel(const float *__restrict__ sino, float *__restrict__ avg,
                                                           int N, int w) {
@@ -54,3 +69,31 @@ hipError_t launch_ring(float *d_sino, float *d_avg, int S, int N, int w, int box
 }
 
 }  // namespace ctas
+
+using namespace ctas;
+
+extern "C" int ctas_ring_filter_dev_noalloc(float *d_sino, int S, int N, int w, int box, void *stream) {
+  if (!d_sino || S <= 0 || N <= 0 || w <= 0 || box < 0) return fail(CTAS_E_ARG, "bad arguments");
+  if (static_cast<long long>(S) * w > kRingScratchFloats)
+    return fail(CTAS_E_ARG, "ring filter without allocation: " + std::to_string(S) + " slices x " +
+                                std::to_string(w) + " columns exceed its " + std::to_string(kRingScratchFloats) +
+                                " column means of scratch");
+  if (box == 0) return CTAS_OK;
+  int dev = 0;
+  CTAS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CTAS_E_DEVICE, "device index " + std::to_string(dev) + " out of range");
+  float *avg = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      CTAS_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    avg = g_scratch_ptr[dev];
+  }
+  // column_mean_kernel writes every mean it reads back: nothing to clear
+  const hipError_t e = launch_ring(d_sino, avg, S, N, w, box, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(CTAS_E_DEVICE, std::string("ring filter: ") + hipGetErrorString(e));
+  return CTAS_OK;
+}

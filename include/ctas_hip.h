@@ -188,6 +188,15 @@ int ctas_fbp_pipeline(ctas_fbp *h, long n_slices, int batch, const float *center
 /* ---- ring filter (RingFilter::apply, lct.cpp:295-327) ----------------- */
 int ctas_ring_filter(float *sino, int N, int w, int box);
 int ctas_ring_filter_dev(float *d_sino, int S, int N, int w, int box, void *stream);
+/* No counterpart in the reference. ctas_ring_filter_dev for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * loops): the same kernels, with the S*w column means in a fixed device
+ * array of 1,048,576 floats (4 MiB per GPU, held until the process ends)
+ * instead of a per-call allocation, and no synchronisation (the work is
+ * enqueued on `stream`). S*w above 1,048,576 is rejected with CTAS_E_ARG.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other threads) race. */
+int ctas_ring_filter_dev_noalloc(float *d_sino, int S, int N, int w, int box, void *stream);
 
 /* ---- tomosynthesis (ts_add, ct.h:317 / lct.cpp:739-787) --------------- */
 /* Filter::fill(filt, n) (lct.cpp:122-209): the window of an n-point line,
