@@ -6,10 +6,16 @@ padded to 4096, centre offset +3.37 px, output 2048^2 x 256 float32, run as
 one hipGraph-captured launch sequence (min/max -> filter -> backprojection).
 
 One step = one reconstruction of the whole stack with the inputs already in
-HBM. Prints one JSON line; the roofline object prices the backprojection's
-LDS sample bytes (4 per voxel-update) against 256 CUs x 128 B/clk x 2.4 GHz.
+HBM. Prints one JSON line. The roofline object prices the backprojection's
+LDS sample bytes (4 per voxel-update) against the LDS read peak measured in
+this run (ds_read_b128, the widest read; the b32 peak of the reads the
+kernel issues is reported beside it), and carries the VALU issue roof:
+VALU instructions per voxel-update counted by rocprofv3 (SQ_INSTS_VALU) in
+a child run of the same launch, against the measured v_add_f32 issue rate.
+HBM traffic per launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE child
+runs of the same launch, scaled by what the counters read for a plain copy.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--slices S] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--slices S] [--no-cpu] [--no-pmc]
 """
 from __future__ import annotations
 
@@ -23,21 +29,155 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s
 HBM_PEAK_GBS = 8000.0
+PEAKS_LIB = ROOT / "ctas_amd" / "libctas_peaks.so"
+COPY_BYTES = 1 << 30  # counter calibration: a plain 1 GiB copy
 
 
-# HBM bytes per launch of the backprojection of the default workload (256 x
-# 2048 px x 1800 angles, 64x64 tiles, 32 angles per pass), from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read
-# for a 1 GiB copy (profiles/fbp_fp32_ctas_2048x1800x256.md). PMC counters
-# cannot be read from inside this process, so the figure is quoted for that
-# configuration only.
-PMC_TRAFFIC = {(256, 2048, 1800, 1, 32): 79.4e9}
+def host_cpu_info() -> dict:
+    """nproc, the CPUs this process may run on (affinity and cgroup quota)
+    and the CPU model name."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = os.cpu_count()
+    quota = None
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpus"] = quota
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info["model"] = model
+    usable = info["affinity"] or 1
+    if quota:
+        usable = max(1, min(usable, int(quota)))
+    info["usable"] = usable
+    return info
 
 
-def measured_traffic(S, w, N, kind, ac):
-    return PMC_TRAFFIC.get((S, w, N, kind, ac))
+def measure_peaks() -> dict:
+    """Clock, LDS read and VALU issue peaks on this device
+    (ctas_amd/csrc/measure/peaks.hip)."""
+    import ctypes
+    L = ctypes.CDLL(str(PEAKS_LIB))
+    out = (ctypes.c_double * 12)()
+    if L.ctas_peaks_measure(out, 12) != 0:
+        raise RuntimeError("peak measurement failed")
+    cus, ghz = int(out[0]), out[1]
+    per = lambda b: b * cus * ghz  # B/clk/CU -> GB/s
+    return {"cus": cus, "clock_ghz": round(ghz, 3),
+            "lds_B_per_clk_cu": {"b32": round(out[2], 1), "b64": round(out[3], 1), "b128": round(out[4], 1)},
+            "lds_GBs": {"b32": round(per(out[2]), 0), "b64": round(per(out[3]), 0), "b128": round(per(out[4]), 0)},
+            "lds_instr_per_clk_cu": {"b32": round(out[5], 3), "b64": round(out[6], 3), "b128": round(out[7], 3)},
+            "valu_instr_per_clk_cu": {"v_add_f32": round(out[8], 3), "v_cvt_i32_f32": round(out[9], 3),
+                                      "v_lshl_add_u32": round(out[10], 3)},
+            "lds_bytes_per_workgroup": int(out[11])}
+
+
+def _pmc_pass(args, counters, kernel_re, kind, ac, store, interp):
+    """One rocprofv3 --pmc pass over a child run of the launch; returns
+    {kernel substring: {counter: value}} for the bp kernel and the copy."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    d = tempfile.mkdtemp(prefix="ctas_pmc_", dir="/tmp")
+    try:
+        cmd = [rocprof, "--pmc", *counters, "--kernel-include-regex", f"{kernel_re}|peaks_copy_kernel",
+               "-d", d, "-o", "pmc", "-f", "csv", "--", sys.executable, str(Path(__file__).resolve()),
+               "--pmc-child", "--slices", str(args.slices), "--width", str(args.width),
+               "--angles", str(args.angles), "--center", str(args.center),
+               "--child-tiles", f"{kind},{ac},{store},{interp}"]
+        env = dict(os.environ, TMPDIR="/tmp")
+        with open(os.path.join(d, "log.txt"), "w") as log:
+            subprocess.run(cmd, cwd="/tmp", env=env, stdout=log, stderr=subprocess.STDOUT, timeout=300)
+        # rocprofv3 can end with a fault of its own after the program's
+        # results are written: the files decide
+        found = {}
+        for root, _, files in os.walk(d):
+            for f in files:
+                if not f.endswith("counter_collection.csv"):
+                    continue
+                with open(os.path.join(root, f)) as fh:
+                    for row in csv.DictReader(fh):
+                        name = row.get("Kernel_Name", "")
+                        key = "copy" if "peaks_copy_kernel" in name else "bp" if kernel_re in name else None
+                        if key is None:
+                            continue
+                        c = row.get("Counter_Name", "")
+                        found.setdefault(key, {}).setdefault(c, 0.0)
+                        found[key][c] += float(row.get("Counter_Value", 0) or 0)
+        return found
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def pmc_measure(args, kernel_name, kind, ac, store, interp, updates, bp_ms, peaks):
+    """HBM traffic and VALU instructions of one backprojection launch from
+    two rocprofv3 --pmc passes over a child run of the same launch."""
+    print("[bench] rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, SQ_INSTS_VALU)", file=sys.stderr, flush=True)
+    a = _pmc_pass(args, ["FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS"], kernel_name, kind, ac, store, interp)
+    b = _pmc_pass(args, ["WRITE_SIZE", "SQ_WAVES"], kernel_name, kind, ac, store, interp)
+    bp_a, bp_b, cp_a, cp_b = a.get("bp", {}), b.get("bp", {}), a.get("copy", {}), b.get("copy", {})
+    need = [(bp_a, "FETCH_SIZE"), (bp_a, "SQ_INSTS_VALU"), (bp_b, "WRITE_SIZE"), (cp_a, "FETCH_SIZE"),
+            (cp_b, "WRITE_SIZE")]
+    if any(k not in dct or dct[k] <= 0 for dct, k in need):
+        return {"error": "counters not read", "got": {"pass1": a, "pass2": b}}
+    # counters are in KiB; a copy of COPY_BYTES reads and writes COPY_BYTES
+    f_fetch = COPY_BYTES / (cp_a["FETCH_SIZE"] * 1024.0)
+    f_write = COPY_BYTES / (cp_b["WRITE_SIZE"] * 1024.0)
+    traffic = (bp_a["FETCH_SIZE"] * f_fetch + bp_b["WRITE_SIZE"] * f_write) * 1024.0
+    valu = bp_a["SQ_INSTS_VALU"]
+    clk_cu = peaks["clock_ghz"] * 1e9 * peaks["cus"]
+    achieved = valu / (bp_ms * 1e-3) / clk_cu
+    peak = peaks["valu_instr_per_clk_cu"]["v_add_f32"]
+    return {"traffic": traffic, "fetch_kib": bp_a["FETCH_SIZE"], "write_kib": bp_b["WRITE_SIZE"],
+            "copy_scale": {"fetch": round(f_fetch, 4), "write": round(f_write, 4)},
+            "valu": {"bound": "valu_issue", "sq_insts_valu": valu,
+                     "instr_per_update": round(valu * 64 / updates, 3),
+                     "achieved": round(achieved, 3), "peak": peak, "unit": "wave instructions/clk/CU",
+                     "frac": round(achieved / peak, 4),
+                     "note": "VALU wave instructions of the launch (x64 lanes) per full-grid voxel-update; "
+                             "peak = measured v_add_f32 issue rate"},
+            "lds_instr": bp_a.get("SQ_INSTS_LDS"), "waves": bp_b.get("SQ_WAVES")}
+
+
+def pmc_child(args) -> int:
+    """Runs under rocprofv3: the inputs, then a 1 GiB copy (the counters'
+    calibration) and one backprojection launch of the benchmark workload."""
+    import ctypes
+    import torch
+    import ctas_amd as ca
+    kind, ac, store, interp = (int(x) for x in args.child_tiles.split(","))
+    S, w, N, c = args.slices, args.width, args.angles, args.center
+    d_sino = make_inputs(torch, ca, S, N, w, c)
+    d_out = torch.empty((S, w, w), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.Stream()
+    rec = ca.CTrec(w, N, ca.ABS, "HANN", interp=interp, store=store)
+    rec.set_tiles(kind, ac)
+    rec.prepare_dev(d_sino, stream)
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    L = ctypes.CDLL(str(PEAKS_LIB))
+    L.ctas_peaks_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p]
+    if L.ctas_peaks_copy(src.data_ptr(), dst.data_ptr(), COPY_BYTES, ctypes.c_void_p(stream.cuda_stream)) != 0:
+        return 1
+    rec.backproject_dev(d_out, [c] * S, stream)
+    torch.cuda.synchronize()
+    return 0
 
 
 def in_mask_fraction(w: int, c: float) -> float:
@@ -245,20 +385,51 @@ def bench_ts(torch, np, ca, rows=2048, pixels=2048, n_proj=64, reps=5):
                                        f"{dt:.2f} s"}}
 
 
-def cpu_baseline(d_sino, w, N, c, threads):
+def cpu_baseline(d_sino, w, N, c, host):
     """The oracle's -O3 -ffast-math build (the reference's flags), one slice
-    per thread as CTAS's RecInThread does, on 2 x threads slices."""
-    import numpy as np
+    per thread as CTAS's RecInThread does, with as many threads as this
+    process may use (nof_threads, parallel.world.cpp:8-29: nproc where the
+    CPUs are not restricted), on 2 x threads slices."""
     import oracle
+    threads = host["usable"]
     S = min(d_sino.shape[0], 2 * threads)
-    host = d_sino[:S].cpu().numpy()
+    host_sino = d_sino[:S].cpu().numpy()
     t0 = time.perf_counter()
-    oracle.fbp_batch(host, [c] * S, oracle.ABS, 5, 0.0, 180.0, threads=threads, fast=True)
+    oracle.fbp_batch(host_sino, [c] * S, oracle.ABS, 5, 0.0, 180.0, threads=threads, fast=True)
     dt = time.perf_counter() - t0
     return {"value": S * w * w * N / dt / 1e9, "unit": "Gvoxel-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{S} slices of {w}x{N} (ABS, HANN, c={c}), {threads} threads, "
+            "kind": "port", "host": host,
+            "sample": f"{S} slices of {w}x{N} (ABS, HANN, c={c}), {threads} threads (one slice each), "
                       f"{dt:.1f} s, oracle/liboracle_fast.so (-O3 -ffast-math)"}
+
+
+def cpu_axis_baseline(d_sino, w, N, host, trials_sweep):
+    """raxis on the CPU (the oracle's AverageGradient at the reference's
+    flags): one trial per thread, as many threads as this process may use
+    (AGinThread, lct.cpp:922-1104), timed on one round of concurrent trials;
+    the whole search is estimated as ceil(sweep trials / threads) rounds plus
+    6 sequential bisection trials (improveMe to 0.25 px from 8 px)."""
+    import math
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    import oracle
+    threads = host["usable"]
+    step = 180.0 / (N - 1)
+    P = d_sino[0].cpu().numpy()
+    Pf = oracle.prepare(P, oracle.filter_window(1, 0.0, w, oracle.ABS), oracle.ABS)
+    sc = oracle.angles(N, step)
+    sigma = float(np.float32(oracle.angO(step, N) * np.float32(w) / np.pi))
+    axes = [0.5 * (w - 1) + 8.0 * (k - threads // 2) for k in range(threads)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda a: oracle.ag_score(Pf, sc, sigma, a, fast=True), axes))
+    rnd = time.perf_counter() - t0
+    rounds = math.ceil(trials_sweep / threads) + 6
+    return {"raxis_estimate_ms": round(rounds * rnd * 1e3, 0), "round_of_trials_s": round(rnd, 2),
+            "threads": threads, "trials_per_round": threads,
+            "sample": f"{threads} concurrent AverageGradient trials at {w} px x {N} angles "
+                      f"(oracle/liboracle_fast.so) in {rnd:.1f} s; raxis = {trials_sweep} sweep trials "
+                      f"+ 6 bisection trials estimated as {rounds} such rounds"}
 
 
 def main() -> int:
@@ -272,12 +443,16 @@ def main() -> int:
     ap.add_argument("--center", type=float, default=3.37)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-axis", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-ts", action="store_true")
     ap.add_argument("--no-4096", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--child-tiles", default="1,32,0,0", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        return pmc_child(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); slices are "
               "independent and would shard across GPUs without collectives", file=sys.stderr)
@@ -291,6 +466,9 @@ def main() -> int:
         print("no GPU", file=sys.stderr)
         return 1
     S, w, N, c = args.slices, args.width, args.angles, args.center
+    print("[bench] measuring peaks", file=sys.stderr, flush=True)
+    torch.zeros(1, device="cuda")
+    peaks = measure_peaks()
     print(f"[bench] generating {S} x {N} x {w} phantoms", file=sys.stderr, flush=True)
     d_sino = make_inputs(torch, ca, S, N, w, c)
     d_out = torch.empty((S, w, w), dtype=torch.float32, device="cuda")
@@ -323,7 +501,11 @@ def main() -> int:
     value = updates / (ms / 1e3) / 1e9
     lds_bytes = updates * 4.0  # one 4-byte sample per voxel-update (CTAS mode, FP32)
     achieved = lds_bytes / (bp_ms / 1e3) / 1e9
+    lds_peak, lds_peak32 = peaks["lds_GBs"]["b128"], peaks["lds_GBs"]["b32"]
     frac_in = in_mask_fraction(w, c)
+    pmc = None
+    if not args.no_pmc:
+        pmc = pmc_measure(args, "bp_grid_kernel", kind, ac, ca.STORE_FP32, ca.INTERP_CTAS, updates, bp_ms, peaks)
     result = {
         "metric": "Gvoxel-updates/s FBP 2048^2x1800 FP32 vs FP16; axis-find ms CNN vs sweep",
         "value": round(value, 2),
@@ -341,13 +523,23 @@ def main() -> int:
                                "CTAS floor sampling, one hipGraph",
                    "slices": S, "width": w, "angles": N, "filter": "HANN", "tile_kind": kind,
                    "angles_per_pass": ac, "in_mask_fraction": round(frac_in, 4)},
-        "roofline": {"bound": "lds", "achieved": round(achieved, 1), "peak": LDS_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / LDS_PEAK_GBS, 4),
-                     "traffic": measured_traffic(S, w, N, kind, ac),
+        "roofline": {"bound": "lds", "achieved": round(achieved, 1), "peak": lds_peak,
+                     "unit": "GB/s", "frac": round(achieved / lds_peak, 4),
+                     "traffic": (round(pmc["traffic"]) if pmc and "traffic" in pmc else None),
                      "kernel": "bp_grid_kernel", "kernel_ms": round(bp_ms, 3),
-                     "bytes_per_update": 4, "note": "LDS sample bytes per full-grid voxel-update"},
+                     "bytes_per_update": 4,
+                     "peak_b32": lds_peak32, "frac_b32": round(achieved / lds_peak32, 4),
+                     "binding": "valu_issue",
+                     "valu": (pmc["valu"] if pmc and "valu" in pmc else None),
+                     "note": "LDS sample bytes per full-grid voxel-update (one ds_read_b32 each) over the "
+                             "kernel's HIP-event time; peak = ds_read_b128 bytes/clk/CU x CUs x clock "
+                             "measured in this run (peak_b32: the same for ds_read_b32); the kernel is "
+                             "bound by VALU issue (valu.frac)"},
         "stages_ms": {"prepare": round(prep_ms, 3), "backproject": round(bp_ms, 3)},
+        "peaks": peaks,
     }
+    if pmc:
+        result["pmc"] = {k: v for k, v in pmc.items() if k not in ("valu",)}
     if not args.no_variants:
         var = {}
         for name, store in (("fp16", ca.STORE_FP16), ("bf16", ca.STORE_BF16)):
@@ -360,9 +552,14 @@ def main() -> int:
         var["fp32_linear"] = {"value": round(updates / (lms / 1e3) / 1e9, 2), "ms_per_step": round(lms, 3),
                               "bp_kernel_ms": round(lbp, 3)}
         result["variants"] = var
+    host = host_cpu_info()
     if not args.no_axis:
         result["axis"] = bench_axis(torch, np, ca, d_sino, w, N)
         result["axis_cnn"] = bench_axis_cnn(torch, np, ca)
+        if not args.no_cpu:
+            print("[bench] CPU axis baseline", file=sys.stderr, flush=True)
+            steps = 1 + 2 * int(0.25 * w / 8)  # raxis's sweep (lct.cpp:1186)
+            result["axis"]["cpu"] = cpu_axis_baseline(d_sino, w, N, host, steps)
     if not args.no_uneven:
         result["uneven_batch"] = bench_uneven(torch, np, ca)
     if not args.no_4096:
@@ -371,8 +568,7 @@ def main() -> int:
         result["tomosynthesis"] = bench_ts(torch, np, ca)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        result["cpu_baseline"] = cpu_baseline(d_sino, w, N, c, threads)
+        result["cpu_baseline"] = cpu_baseline(d_sino, w, N, c, host)
     print(json.dumps(result), flush=True)
     return 0
 

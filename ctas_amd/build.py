@@ -9,6 +9,7 @@ tree:
 * ``ctas_amd/libctas.so`` - the C++ reconstruction API (``CTrec``, ``Filter``,
   ``Contrast``, ``raxis``) layered on the C ABI, and the ``ctas-ct`` /
   ``ctas-ax`` tools in ``ctas_amd/bin``;
+* ``ctas_amd/libctas_peaks.so`` - peak measurements for bench.py's roofline;
 * ``oracle/liboracle.so`` - the CPU restatement (test infrastructure).
 """
 from __future__ import annotations
@@ -96,15 +97,17 @@ def build_cpp_api(force: bool = True) -> Path:
     return lib
 
 
-def build_diag(force: bool = True) -> Path:
-    """ctas-peak-probe: measures clock, LDS bandwidth and VALU rate on the
-    device (the peaks of the backprojection roofline)."""
-    src = CSRC / "diag" / "ctas_peak_probe.cpp"
-    exe = PKG / "bin" / "ctas-peak-probe"
-    exe.parent.mkdir(exist_ok=True)
-    if force or _needs(exe, [src]):
-        _run([HIPCC, *HIP_FLAGS, "-x", "hip", str(src), "-o", str(exe)])
-    return exe
+def build_peaks(force: bool = True) -> Path:
+    """libctas_peaks.so: the clock, LDS and VALU peak measurements and the
+    counter-calibration copy kernel that bench.py prices its roofline with
+    (measurement infrastructure; the engine does not use it)."""
+    src = CSRC / "measure" / "peaks.hip"
+    lib = PKG / "libctas_peaks.so"
+    tmp = PKG / "libctas_peaks.so.tmp"
+    if force or _needs(lib, [src]):
+        _run([HIPCC, *HIP_FLAGS, "-shared", "-x", "hip", str(src), "-o", str(tmp)])
+        os.replace(tmp, lib)
+    return lib
 
 
 def build_oracle() -> None:
@@ -115,7 +118,7 @@ def build_oracle() -> None:
 def build_all(force: bool = True) -> None:
     build_engine(force=force)
     build_cpp_api(force=force)
-    build_diag(force=force)
+    build_peaks(force=force)
     build_oracle()
 
 

@@ -177,11 +177,11 @@ def ring_filter(P: np.ndarray, box: int) -> np.ndarray:
     return x
 
 
-def ag_score(Pf: np.ndarray, sincos: np.ndarray, sigma: float, axisR: float) -> float:
+def ag_score(Pf: np.ndarray, sincos: np.ndarray, sigma: float, axisR: float, fast: bool = False) -> float:
     P = np.ascontiguousarray(Pf, np.float32)
     N, w = P.shape
-    return lib().orc_ag_score(_f(P), N, w, _f(np.ascontiguousarray(sincos, np.float32)), sigma,
-                              axisR, None)
+    return lib(fast).orc_ag_score(_f(P), N, w, _f(np.ascontiguousarray(sincos, np.float32)), sigma,
+                                  axisR, None)
 
 
 def raxis(sino: np.ndarray, anarc: float, max_dev: float = 0.25, algo: int = 0) -> float:
