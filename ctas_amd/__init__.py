@@ -144,6 +144,7 @@ def lib() -> C.CDLL:
         "ctas_cnn_predict_dev": (i, [vp, vp, i, i, fp, fp, vp]),
         "ctas_cnn_predict": (i, [vp, fp, i, i, fp, fp]),
         "ctas_axis_find": (i, [vp, vp, vp, f, f, f, fp, ip]),
+        "ctas_axis_find_from": (i, [vp, vp, f, f, f, fp, ip]),
         "ctas_raxis_cnn": (i, [vp, fp, i, i, f, f, f, fp, ip]),
         "ctas_cnn_gemm_dev": (i, [vp, vp, vp, i, i, i, vp]),
     }
@@ -686,6 +687,17 @@ def axis_find(rec: CTrec, d_sino, net: AxisCNN | None = None, half_range: float 
     info = (C.c_int * 4)()
     _check(lib().ctas_axis_find(rec._h, None if net is None else net._h, _ptr(d_sino), half_range,
                                 step, min_confidence, C.byref(off), info))
+    return off.value, {"trials": info[0], "swept": bool(info[1]), "estimate": info[2] / 1000.0,
+                       "confidence": info[3] / 1000.0}
+
+
+def axis_find_from(rec: CTrec, d_sino, start: float, half_range: float = 20.0,
+                   step: float = 0.5) -> tuple[float, dict]:
+    """The windowed search of axis_find from a given start offset (px),
+    without a network. Returns (offset, info)."""
+    off = C.c_float()
+    info = (C.c_int * 4)()
+    _check(lib().ctas_axis_find_from(rec._h, _ptr(d_sino), half_range, step, start, C.byref(off), info))
     return off.value, {"trials": info[0], "swept": bool(info[1]), "estimate": info[2] / 1000.0,
                        "confidence": info[3] / 1000.0}
 

@@ -1038,18 +1038,29 @@ extern "C" int ctas_cnn_predict(ctas_cnn *n, const float *sino, int N, int w, fl
 // network estimate (milli-px), confidence (per mille).
 constexpr int kMaxWindow = 21;
 
-extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, float half_range, float step,
-                              float min_confidence, float *offset, int *info) {
+namespace {
+
+// The sweep of ctas_axis_find on a device sinogram: with `windowed`, a
+// window of 3 grid points around `start` (px), widened one point at a time
+// towards a lower edge score until its minimum is interior, the whole grid
+// past kMaxWindow points; otherwise the whole grid. est and conf only go to
+// info.
+// Argument checks and the filter of the sinogram on the handle's stream
+// (after the work already queued on the null stream, which the caller's
+// NullStreamJoin orders).
+int axis_prepare(ctas_fbp *h, const float *d_sino, float half_range, float step, float *offset) {
   if (!h || !d_sino || !offset || !(step > 0) || !(half_range >= 0)) return fail(CTAS_E_ARG, "bad arguments");
-  const int w = h->w;
   const int K = static_cast<int>(std::floor(2.0 * half_range / step + 0.5)) + 1;
   if (K > 4096) return fail(CTAS_E_ARG, "too many trial centres");
-  if (std::ceil(half_range) >= 0.5f * w) return fail(CTAS_E_CENTER, "trial centres beyond half the width");
-  // filtering on the handle's stream, the network on its own stream, both
-  // after the work already queued on the null stream
-  NullStreamJoin join(h->stream, nullptr);
-  if (join.rc) return join.rc;
-  if (int rc = ctas_fbp_prepare_dev(h, d_sino, 1, h->stream)) return rc;
+  if (std::ceil(half_range) >= 0.5f * h->w) return fail(CTAS_E_CENTER, "trial centres beyond half the width");
+  return ctas_fbp_prepare_dev(h, d_sino, 1, h->stream);
+}
+
+// The search itself, on the filtered sinogram axis_prepare left.
+int axis_search(ctas_fbp *h, float half_range, float step, bool windowed, float start, float est, float conf,
+                float *offset, int *info) {
+  const int w = h->w;
+  const int K = static_cast<int>(std::floor(2.0 * half_range / step + 0.5)) + 1;
   const float cent = static_cast<float>(0.5 * (w - 1));
   auto grid = [&](int k) { return static_cast<float>(-static_cast<double>(half_range) + k * static_cast<double>(step)); };
   std::vector<float> sc(K, 0.0f);
@@ -1079,14 +1090,10 @@ extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, f
       if (sc[k] < sc[m]) m = k;
     return m;
   };
-  float est = 0.0f, conf = 0.0f;
   int swept = 1, km = 0;
-  if (net) {
-    if (int rc = ctas_cnn_predict_dev(net, d_sino, h->N, w, &est, &conf, nullptr)) return rc;
-  }
-  if (net && conf >= min_confidence) {
+  if (windowed) {
     swept = 0;
-    const int k0 = std::max(0, std::min(K - 1, static_cast<int>(std::lround((est + half_range) / step))));
+    const int k0 = std::max(0, std::min(K - 1, static_cast<int>(std::lround((start + half_range) / step))));
     int lo = std::max(0, k0 - 1), hi = std::min(K - 1, k0 + 1);
     if (int rc = eval(lo, hi)) return rc;
     for (;;) {
@@ -1117,6 +1124,32 @@ extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, f
     info[3] = static_cast<int>(std::lround(conf * 1000.0f));
   }
   return CTAS_OK;
+}
+
+}  // namespace
+
+extern "C" int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, float half_range, float step,
+                              float min_confidence, float *offset, int *info) {
+  if (!h) return fail(CTAS_E_ARG, "bad arguments");
+  // filtering on the handle's stream, the network on its own stream, both
+  // after the work already queued on the null stream
+  NullStreamJoin join(h->stream, nullptr);
+  if (join.rc) return join.rc;
+  if (int rc = axis_prepare(h, d_sino, half_range, step, offset)) return rc;
+  float est = 0.0f, conf = 0.0f;
+  if (net) {
+    if (int rc = ctas_cnn_predict_dev(net, d_sino, h->N, h->w, &est, &conf, nullptr)) return rc;
+  }
+  return axis_search(h, half_range, step, net && conf >= min_confidence, est, est, conf, offset, info);
+}
+
+extern "C" int ctas_axis_find_from(ctas_fbp *h, const float *d_sino, float half_range, float step, float start,
+                                   float *offset, int *info) {
+  if (!h) return fail(CTAS_E_ARG, "bad arguments");
+  NullStreamJoin join(h->stream, nullptr);
+  if (join.rc) return join.rc;
+  if (int rc = axis_prepare(h, d_sino, half_range, step, offset)) return rc;
+  return axis_search(h, half_range, step, true, start, start, 1.0f, offset, info);
 }
 
 // bf16 GEMM of the network's kernel on plain row-major operands:

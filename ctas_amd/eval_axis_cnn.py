@@ -11,7 +11,9 @@ device-resident sinogram:
   (81 trial reconstructions), argmin;
 * the network-fronted search (ctas_axis_find with the trained network):
   window around the estimate, the whole sweep when the confidence is below
-  --min-conf.
+  --min-conf;
+* the same windowed search without the network (ctas_axis_find_from),
+  started at offset 0, the only prior a search without one has.
 
 Reported: agreement with the sweep (|difference| <= 0.5 px), the sweep's
 and the estimate's distance to the true offset, milliseconds of each search
@@ -57,17 +59,22 @@ def run(ca, torch, net, seed: int, n: int, half: float = 20.0, step: float = 0.5
             t1 = time.perf_counter()
             cn, info = ca.axis_find(rec, d, net, half, step, min_conf)
             t2 = time.perf_counter()
+            wn, info_n = ca.axis_find_from(rec, d, 0.0, half, step)
+            t3 = time.perf_counter()
             rows.append((i, w, N, c, sw, (t1 - t0) * 1e3, cn, (t2 - t1) * 1e3, info["trials"],
-                         info["swept"], info["estimate"], info["confidence"]))
+                         info["swept"], info["estimate"], info["confidence"], wn, (t3 - t2) * 1e3,
+                         info_n["trials"], info_n["swept"]))
             now = time.perf_counter()
             if now - last > 20:
                 log(f"[eval] {len(rows)}/{n} samples, {now - t_start:.0f} s")
                 last = now
         rec.close()
         del d
-    a = np.array([r[:9] + (float(r[9]), r[10], r[11]) for r in rows], np.float64)
+    a = np.array([r[:9] + (float(r[9]), r[10], r[11], r[12], r[13], r[14], float(r[15])) for r in rows],
+                 np.float64)
     c, sw, sw_ms, cn, cn_ms, trials, swept, est, conf = (a[:, 3], a[:, 4], a[:, 5], a[:, 6], a[:, 7], a[:, 8],
                                                          a[:, 9], a[:, 10], a[:, 11])
+    wn, wn_ms, wn_trials, wn_swept = a[:, 12], a[:, 13], a[:, 14], a[:, 15]
     agree = np.abs(cn - sw) <= 0.5
     res = {
         "n": int(len(rows)), "grid": f"{-half}..{half} step {step}", "min_confidence": min_conf,
@@ -82,6 +89,9 @@ def run(ca, torch, net, seed: int, n: int, half: float = 20.0, step: float = 0.5
         "speedup_mean_time": float(sw_ms.mean() / cn_ms.mean()),
         "speedup_median_ratio": float(np.median(sw_ms / cn_ms)),
         "sweep_ms_2048x1800": None, "cnn_ms_2048x1800": None,
+        "without_network": {"agree_with_sweep_0.5px": float(np.mean(np.abs(wn - sw) <= 0.5)),
+                            "mean_trials": float(wn_trials.mean()), "fallback_rate": float(wn_swept.mean()),
+                            "ms_mean": float(wn_ms.mean())},
     }
     sel = (a[:, 1] == 2048) & (a[:, 2] == 1800)
     if sel.any():
@@ -94,7 +104,7 @@ def run(ca, torch, net, seed: int, n: int, half: float = 20.0, step: float = 0.5
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--weights", type=str, default=None)
-    ap.add_argument("--seed", type=int, default=1000001)
+    ap.add_argument("--seed", type=int, default=3000017)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--min-conf", type=float, default=0.9)
     ap.add_argument("--json", type=str, default=None)

@@ -314,6 +314,11 @@ int ctas_cnn_predict(ctas_cnn *net, const float *sino, int n_angles, int w, floa
  * confidence (per mille). */
 int ctas_axis_find(ctas_fbp *h, ctas_cnn *net, const float *d_sino, float half_range, float step,
                    float min_confidence, float *offset, int *info);
+/* The same windowed search from a given start offset `start` (px) instead
+ * of the network's estimate (e.g. the 0/180 registration's seed, or 0): the
+ * search the network fronts, without it. info as above (estimate = start). */
+int ctas_axis_find_from(ctas_fbp *h, const float *d_sino, float half_range, float step, float start,
+                        float *offset, int *info);
 /* raxis algorithm 1 (lct.cpp:1133-1212) with its 8 px sweep replaced by the
  * network's estimate (improveMe from +-4 px around it); falls back to
  * ctas_raxis(algo 1) below min_confidence. */
