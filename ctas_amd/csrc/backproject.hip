@@ -977,41 +977,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KindGeo<KIN
   bp_tile_mirror<KIND, AC, ST>(it, b - it.tile_base, lds);
 }
 
-template <int A, int B>
-struct Max { static constexpr size_t v = A > B ? A : B; };
-
-constexpr int kPersistAC = 16;
-
-template <int MODE, int ST>
-constexpr size_t persist_lds() {
-  return Max<Max<StageFloat<TK_64x64, kPersistAC, MODE, ST>::BYTES, StageFloat<TK_128x64, kPersistAC, MODE, ST>::BYTES>::v,
-             StageFloat<TK_128x128, kPersistAC, MODE, ST>::BYTES>::v;
-}
-
 __global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
 
-// Persistent launch: a fixed set of workgroups pulls tiles from a global
-// counter until every tile of every item is done; each item carries its own
-// tile kind, so one launch serves sinograms of uneven width.
-template <int MODE, int ST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void bp_persistent_kernel(const Item *__restrict__ items, int n_items,
-                                                            int total_tiles, unsigned *counter) {
-  __shared__ __align__(16) unsigned char lds[persist_lds<MODE, ST>()];
+// Persistent launches: a fixed set of workgroups, as many as stay resident,
+// pull tiles from a global counter until every tile of every item is done.
+// One launch serves items of uneven width that share a tile kind and pass
+// length; each variant keeps the register budget of its grid kernel. The
+// items come costliest first, so the launch ends with the small tiles.
+template <class Tile>
+__device__ __forceinline__ void persistent_loop(const Item *__restrict__ items, int n_items, int total_tiles,
+                                                unsigned *counter, Tile tile) {
   __shared__ int s_job;
   for (;;) {
-    __syncthreads();
+    __syncthreads();  // the previous tile is done with the LDS and s_job
     if (threadIdx.x == 0) s_job = static_cast<int>(atomicAdd(counter, 1u));
     __syncthreads();
     const int job = s_job;
     if (job >= total_tiles) break;
     const Item &it = items[find_item_by_tile(items, n_items, job)];
-    const int tile = job - it.tile_base;
-    switch (it.kind) {
-      case TK_64x64: bp_tile<TK_64x64, kPersistAC, MODE, ST>(it, tile, lds); break;
-      case TK_128x64: bp_tile<TK_128x64, kPersistAC, MODE, ST>(it, tile, lds); break;
-      default: bp_tile<TK_128x128, kPersistAC, MODE, ST>(it, tile, lds); break;
-    }
+    tile(it, job - it.tile_base);
   }
+}
+
+template <int KIND, int AC, int MODE, int ST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(grid_waves<KIND, AC, MODE, ST>(), grid_waves<KIND, AC, MODE, ST>()))) void bp_persistent_kernel(const Item *__restrict__ items, int n_items,
+                                                                 int total_tiles, unsigned *counter) {
+  constexpr int A = eff_ac<KIND, AC, MODE, ST>();
+  __shared__ __align__(16) unsigned char lds[stage_bytes<KIND, A, MODE, ST>()];
+  persistent_loop(items, n_items, total_tiles, counter, [&](const Item &it, int tile) {
+    if constexpr (use_fx<KIND, A, MODE, ST>())
+      bp_tile_px2<KIND, A, ST>(it, tile, lds);
+    else
+      bp_tile<KIND, A, MODE, ST>(it, tile, lds);
+  });
+}
+
+template <int KIND, int AC, int ST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KindGeo<KIND>::WAVES, KindGeo<KIND>::WAVES))) void bp_persistent_mirror_kernel(const Item *__restrict__ items, int n_items,
+                                                                        int total_tiles, unsigned *counter) {
+  __shared__ __align__(16) unsigned char lds[StageMirror<KIND, AC, ST>::BYTES];
+  persistent_loop(items, n_items, total_tiles, counter,
+                  [&](const Item &it, int tile) { bp_tile_mirror<KIND, AC, ST>(it, tile, lds); });
+}
+
+// Workgroups of a persistent launch: as many as the CUs hold at once, by the
+// variant's waves per SIMD and its LDS.
+int resident_workgroups(int cus, int waves, size_t lds_bytes) {
+  const int by_lds = static_cast<int>((160 * 1024) / (lds_bytes + 64));
+  return cus * std::max(1, std::min(waves, by_lds));
+}
+
+template <int KIND, int AC, int MODE, int ST>
+hipError_t launch_persist(const Item *d_items, int n_items, int total_tiles, unsigned *counter, int cus, hipStream_t s) {
+  constexpr int A = eff_ac<KIND, AC, MODE, ST>();
+  const int g = resident_workgroups(cus, grid_waves<KIND, AC, MODE, ST>(), stage_bytes<KIND, A, MODE, ST>());
+  hipLaunchKernelGGL((bp_persistent_kernel<KIND, AC, MODE, ST>), dim3(g), dim3(256), 0, s, d_items, n_items,
+                     total_tiles, counter);
+  return hipGetLastError();
+}
+
+template <int KIND, int AC, int MODE>
+hipError_t launch_persist_st(const Item *d_items, int n_items, int total_tiles, unsigned *counter, int cus, int store,
+                             hipStream_t s) {
+  switch (store) {
+    case ST_F32: return launch_persist<KIND, AC, MODE, ST_F32>(d_items, n_items, total_tiles, counter, cus, s);
+    case ST_F16: return launch_persist<KIND, AC, MODE, ST_F16>(d_items, n_items, total_tiles, counter, cus, s);
+    default: return launch_persist<KIND, AC, MODE, ST_BF16>(d_items, n_items, total_tiles, counter, cus, s);
+  }
+}
+
+template <int KIND>
+hipError_t launch_persist_k(const Item *d_items, int n_items, int total_tiles, int ac, int mode, int store, bool mirror,
+                            unsigned *counter, int cus, hipStream_t s) {
+  if (mirror) {
+    const int g = resident_workgroups(cus, KindGeo<KIND>::WAVES, StageMirror<KIND, 8, ST_F16>::BYTES);
+    if (store == ST_F16)
+      hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 8, ST_F16>), dim3(g), dim3(256), 0, s, d_items, n_items,
+                         total_tiles, counter);
+    else
+      hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 8, ST_BF16>), dim3(g), dim3(256), 0, s, d_items, n_items,
+                         total_tiles, counter);
+    return hipGetLastError();
+  }
+  if (mode == MODE_LINEAR) {
+    if (ac == 16) return launch_persist_st<KIND, 16, MODE_LINEAR>(d_items, n_items, total_tiles, counter, cus, store, s);
+    return launch_persist_st<KIND, 32, MODE_LINEAR>(d_items, n_items, total_tiles, counter, cus, store, s);
+  }
+  if (ac == 16) return launch_persist_st<KIND, 16, MODE_CTAS>(d_items, n_items, total_tiles, counter, cus, store, s);
+  return launch_persist_st<KIND, 32, MODE_CTAS>(d_items, n_items, total_tiles, counter, cus, store, s);
 }
 
 template <int KIND, int AC, int MODE>
@@ -1061,26 +1114,23 @@ hipError_t launch_backproject(const Item *d_items, int n_items, int total_tiles,
   }
 }
 
-hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int total_tiles,
-                                         int mode, int store, unsigned *d_counter,
-                                         int n_workgroups, hipStream_t s) {
+hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int total_tiles, int kind,
+                                         int angles_per_pass, int mode, int store, bool mirror,
+                                         unsigned *d_counter, int cus, hipStream_t s) {
   if (total_tiles <= 0) return hipSuccess;
+  if (mirror && (mode != MODE_CTAS || store == ST_F32)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, d_counter);
-  const dim3 g(n_workgroups), b(256);
-  if (mode == MODE_LINEAR) {
-    switch (store) {
-      case ST_F32: hipLaunchKernelGGL((bp_persistent_kernel<MODE_LINEAR, ST_F32>), g, b, 0, s, d_items, n_items, total_tiles, d_counter); break;
-      case ST_F16: hipLaunchKernelGGL((bp_persistent_kernel<MODE_LINEAR, ST_F16>), g, b, 0, s, d_items, n_items, total_tiles, d_counter); break;
-      default: hipLaunchKernelGGL((bp_persistent_kernel<MODE_LINEAR, ST_BF16>), g, b, 0, s, d_items, n_items, total_tiles, d_counter); break;
-    }
-  } else {
-    switch (store) {
-      case ST_F32: hipLaunchKernelGGL((bp_persistent_kernel<MODE_CTAS, ST_F32>), g, b, 0, s, d_items, n_items, total_tiles, d_counter); break;
-      case ST_F16: hipLaunchKernelGGL((bp_persistent_kernel<MODE_CTAS, ST_F16>), g, b, 0, s, d_items, n_items, total_tiles, d_counter); break;
-      default: hipLaunchKernelGGL((bp_persistent_kernel<MODE_CTAS, ST_BF16>), g, b, 0, s, d_items, n_items, total_tiles, d_counter); break;
-    }
+  switch (kind) {
+    case TK_64x64:
+      return launch_persist_k<TK_64x64>(d_items, n_items, total_tiles, angles_per_pass, mode, store, mirror, d_counter,
+                                        cus, s);
+    case TK_128x64:
+      return launch_persist_k<TK_128x64>(d_items, n_items, total_tiles, angles_per_pass, mode, store, mirror,
+                                         d_counter, cus, s);
+    default:
+      return launch_persist_k<TK_128x128>(d_items, n_items, total_tiles, angles_per_pass, mode, store, mirror,
+                                          d_counter, cus, s);
   }
-  return hipGetLastError();
 }
 
 }  // namespace ctas

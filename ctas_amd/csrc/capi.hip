@@ -32,13 +32,6 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
-int persistent_workgroups() {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 1024;
-  hipDeviceProp_t p;
-  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 1024;
-  return std::max(1, p.multiProcessorCount) * 4;
-}
 
 namespace {
 
@@ -129,14 +122,16 @@ void drop_extra(const ctas_fbp *h) {
   g_extra.erase(h);
 }
 
-// Mirror slots of the handle's FP64 table (one slot per angle at most).
-int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
+// Mirror slots of an FP64 table of N angles into d_slots [N] and their row
+// and column terms into d_slot_tab [N][kSlotTab] (allocated when null).
+int build_slots(const std::vector<double> &sc64, int N, MirrorSlot *&d_slots, int *&d_slot_tab, int *n_slots,
+                int *n_paired) {
   std::vector<MirrorSlot> slots;
-  h->n_paired = mirror_slots(sc64, slots);
-  h->n_slots = static_cast<int>(slots.size());
-  if (!h->d_slots && hipMalloc(&h->d_slots, sizeof(MirrorSlot) * std::max(h->N, 1)) != hipSuccess)
+  *n_paired = mirror_slots(sc64, slots);
+  *n_slots = static_cast<int>(slots.size());
+  if (!d_slots && hipMalloc(&d_slots, sizeof(MirrorSlot) * std::max(N, 1)) != hipSuccess)
     return fail(CTAS_E_NOMEM, "device allocation of the mirror slots failed");
-  if (!h->d_slot_tab && hipMalloc(&h->d_slot_tab, sizeof(int) * kSlotTab * std::max(h->N, 1)) != hipSuccess)
+  if (!d_slot_tab && hipMalloc(&d_slot_tab, sizeof(int) * kSlotTab * std::max(N, 1)) != hipSuccess)
     return fail(CTAS_E_NOMEM, "device allocation of the mirror slot tables failed");
   std::vector<int> tab(static_cast<size_t>(kSlotTab) * slots.size());
   for (size_t i = 0; i < slots.size(); ++i) {
@@ -157,9 +152,22 @@ int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
     }
     slots[i].integral = frac == 0;
   }
-  CTAS_HIP_TRY(hipMemcpy(h->d_slots, slots.data(), sizeof(MirrorSlot) * slots.size(), hipMemcpyHostToDevice));
-  CTAS_HIP_TRY(hipMemcpy(h->d_slot_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(d_slots, slots.data(), sizeof(MirrorSlot) * slots.size(), hipMemcpyHostToDevice));
+  CTAS_HIP_TRY(hipMemcpy(d_slot_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
   return CTAS_OK;
+}
+
+// Mirror slots of the handle's FP64 table (one slot per angle at most).
+int upload_slots(ctas_fbp *h, const std::vector<double> &sc64) {
+  return build_slots(sc64, h->N, h->d_slots, h->d_slot_tab, &h->n_slots, &h->n_paired);
+}
+
+// The mirrored backprojection serves FP16/BF16 storage with CTAS sampling
+// when at least half of the angles pair up as theta and pi - theta (its
+// staging indexes a sinogram with 32-bit offsets).
+bool mirror_applies(int flags, int N, int w, int n_paired) {
+  return mode_of(flags) == MODE_CTAS && store_of(flags) != ST_F32 && 4 * n_paired >= N &&
+         static_cast<long long>(N) * w + 2LL * filt_pad(w) < (1LL << 31);
 }
 
 // Filtered sinograms of 16-bit storage carry zero pads (read without
@@ -168,11 +176,7 @@ int filt_pad_of(const ctas_fbp *h) { return store_of(h->flags) == ST_F32 ? 0 : f
 
 // FP16/BF16 storage with CTAS sampling takes the mirrored path when at
 // least half of the angles pair up as theta and pi - theta.
-bool use_mirror(const ctas_fbp *h) {
-  // its staging indexes a sinogram with 32-bit offsets
-  return mode_of(h->flags) == MODE_CTAS && store_of(h->flags) != ST_F32 && 4 * h->n_paired >= h->N &&
-         static_cast<long long>(h->N) * h->w + 2LL * filt_pad(h->w) < (1LL << 31);
-}
+bool use_mirror(const ctas_fbp *h) { return mirror_applies(h->flags, h->N, h->w, h->n_paired); }
 
 Item base_item(const ctas_fbp *h, int kind, bool mirror) {
   Item it{};
@@ -349,6 +353,9 @@ struct GeomRes {
   double2 *d_sincos64 = nullptr;
   int neg_sin = 0;
   float2 *d_tw = nullptr;
+  MirrorSlot *d_slots = nullptr;  // mirror slots of the FP64 table and their row/column terms
+  int *d_slot_tab = nullptr;
+  int n_slots = 0, n_paired = 0;
 };
 
 // 1 when some {sin, cos} entry has a negative sine.
@@ -398,6 +405,7 @@ int geometry_resources(int w, int N, int filter, float param, float anarc, int c
   CTAS_HIP_TRY(hipMemcpy(r.d_W, W.data(), sizeof(float) * W.size(), hipMemcpyHostToDevice));
   CTAS_HIP_TRY(hipMemcpy(r.d_sincos, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
   if (int rc = make_twiddles(zidth(w), &r.d_tw)) return rc;
+  if (int rc = build_slots(sc64, N, r.d_slots, r.d_slot_tab, &r.n_slots, &r.n_paired)) return rc;
   g_geo[key] = r;
   *out = r;
   return CTAS_OK;
@@ -856,62 +864,113 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   if (!items || n <= 0) return fail(CTAS_E_ARG, "bad arguments");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
-  int dev = 0;
+  int dev = 0, cus = 0;
   CTAS_HIP_TRY(hipGetDevice(&dev));
+  CTAS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int st = store_of(flags), mode = mode_of(flags);
-  std::vector<Item> its(n);
-  size_t filt_bytes = 0;
-  int max_logZ = 0, min_logZ = 99, pairs = 0, tiles = 0;
-  // Work order: costliest sinograms first, so that the persistent launch
-  // ends with the small tiles.
-  std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return static_cast<double>(items[a].w) * items[a].w * items[a].n_angles >
-           static_cast<double>(items[b].w) * items[b].w * items[b].n_angles;
-  });
-  std::vector<size_t> filt_off(n);
-  for (int oi = 0; oi < n; ++oi) {
-    const ctas_batch_item &bi = items[order[oi]];
+  for (int i = 0; i < n; ++i) {
+    const ctas_batch_item &bi = items[i];
     if (bi.w < 2 || bi.n_angles <= 0 || !bi.d_sino || !bi.d_out)
-      return fail(CTAS_E_ARG, "bad batch item " + std::to_string(order[oi]));
+      return fail(CTAS_E_ARG, "bad batch item " + std::to_string(i));
     if (int rc = check_center(bi.w, bi.center)) return rc;
+  }
+  // Tile kind and angles per pass of each (width class, angle-count class)
+  // from the autotuner: classes not tuned yet in this process are timed now
+  // on a handle of the first item of the class.
+  std::map<std::pair<int, int>, std::pair<int, int>> pick;
+  for (int i = 0; i < n; ++i) {
+    const auto cls = std::make_pair(width_class(items[i].w), width_class(items[i].n_angles));
+    if (pick.count(cls)) continue;
+    const TuneKey key{cls.first, cls.second, mode, st, dev};
+    bool tuned;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      tuned = g_tune.count(key) != 0;
+    }
+    if (!tuned) {
+      ctas_fbp *t = nullptr;
+      int rc = ctas_fbp_create(&t, items[i].w, items[i].n_angles, contrast, filter, filter_param, anarc, flags);
+      if (rc == CTAS_OK) rc = ctas_fbp_autotune(t, 2);
+      ctas_fbp_destroy(t);
+      if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    pick[cls] = g_tune[key];
+  }
+  struct Entry {
+    int src, kind, ac;
+    bool mirror;
+    double cost;
     GeomRes g;
-    if (int rc = geometry_resources(bi.w, bi.n_angles, filter, filter_param, anarc, contrast, &g)) return rc;
+  };
+  std::vector<Entry> ents(n);
+  for (int i = 0; i < n; ++i) {
+    const ctas_batch_item &bi = items[i];
+    Entry &e = ents[i];
+    e.src = i;
+    if (int rc = geometry_resources(bi.w, bi.n_angles, filter, filter_param, anarc, contrast, &e.g)) return rc;
+    const auto &pk = pick[std::make_pair(width_class(bi.w), width_class(bi.n_angles))];
+    e.kind = pk.first;
+    e.ac = pk.second;
+    e.mirror = mirror_applies(flags, bi.n_angles, bi.w, e.g.n_paired);
+    e.cost = static_cast<double>(bi.w) * bi.w * bi.n_angles;
+  }
+  // One persistent launch per (tile kind, pass length, mirrored) group, its
+  // items costliest first, so that each launch ends with the small tiles.
+  std::stable_sort(ents.begin(), ents.end(), [](const Entry &a, const Entry &b) {
+    if (a.mirror != b.mirror) return a.mirror < b.mirror;
+    if (a.kind != b.kind) return a.kind < b.kind;
+    if (a.ac != b.ac) return a.ac < b.ac;
+    return a.cost > b.cost;
+  });
+  std::vector<Item> its(n);
+  std::vector<size_t> filt_off(n);
+  size_t filt_bytes = 0;
+  std::vector<std::tuple<int, int, int, int, int, bool>> launches;  // first, items, tiles, kind, ac, mirror
+  int tiles = 0, group_first = 0;
+  for (int oi = 0; oi < n; ++oi) {
+    const Entry &e = ents[oi];
+    const ctas_batch_item &bi = items[e.src];
+    if (oi > 0 && (e.kind != ents[oi - 1].kind || e.ac != ents[oi - 1].ac || e.mirror != ents[oi - 1].mirror)) {
+      const Entry &p = ents[oi - 1];
+      launches.emplace_back(group_first, oi - group_first, tiles, p.kind, p.ac, p.mirror);
+      group_first = oi;
+      tiles = 0;
+    }
     Item it{};
     it.sino = bi.d_sino;
     it.out = bi.d_out;
-    it.sincos = g.d_sincos;
-    it.sincos64 = use_fp64_angles(flags) ? g.d_sincos64 : nullptr;
-    it.neg_sin = g.neg_sin;
-    it.W = g.d_W;
-    it.tw = g.d_tw;
+    it.sincos = e.g.d_sincos;
+    it.sincos64 = use_fp64_angles(flags) ? e.g.d_sincos64 : nullptr;
+    it.neg_sin = e.g.neg_sin;
+    it.W = e.g.d_W;
+    it.tw = e.g.d_tw;
     it.center = bi.center;
     it.recCof = 1.0f;
     it.w = bi.w;
     it.N = bi.n_angles;
     it.Z = zidth(bi.w);
     while ((1 << it.logZ) < it.Z) ++it.logZ;
-    max_logZ = std::max(max_logZ, it.logZ);
-    min_logZ = std::min(min_logZ, it.logZ);
-    int kind;
-    {
-      std::lock_guard<std::mutex> lk(g_tune_mu);
-      auto f = g_tune.find(TuneKey{width_class(bi.w), width_class(bi.n_angles), mode, st, dev});
-      kind = f != g_tune.end() ? f->second.first : (bi.w >= 1024 ? TK_128x128 : TK_64x64);
+    it.kind = e.kind;
+    const int cols = e.mirror ? bi.w / 2 + 1 : bi.w;
+    it.tiles_x = (cols + tile_cols(e.kind) - 1) / tile_cols(e.kind);
+    it.tiles = it.tiles_x * ((bi.w + tile_rows(e.kind) - 1) / tile_rows(e.kind));
+    if (e.mirror) {
+      it.slots = e.g.d_slots;
+      it.slot_tab = e.g.d_slot_tab;
+      it.n_slots = e.g.n_slots;
     }
-    it.kind = kind;
-    it.tiles_x = (bi.w + tile_cols(kind) - 1) / tile_cols(kind);
-    it.tiles = it.tiles_x * ((bi.w + tile_rows(kind) - 1) / tile_rows(kind));
+    // 16-bit filtered sinograms carry zero pads (read without bounds checks)
+    it.pad = st == ST_F32 ? 0 : filt_pad(bi.w);
     it.tile_base = tiles;
-    it.pair_base = pairs;
     tiles += it.tiles;
-    pairs += (bi.n_angles + 1) / 2;
-    filt_off[oi] = filt_bytes;
-    filt_bytes += (static_cast<size_t>(bi.n_angles) * bi.w * store_bytes(st) + 255) & ~static_cast<size_t>(255);
+    filt_off[oi] = filt_bytes + static_cast<size_t>(it.pad) * store_bytes(st);
+    filt_bytes += (store_bytes(st) * (static_cast<size_t>(bi.n_angles) * bi.w + 2 * static_cast<size_t>(it.pad)) +
+                   255) & ~static_cast<size_t>(255);
     its[oi] = it;
   }
+  launches.emplace_back(group_first, n - group_first, tiles, ents[n - 1].kind, ents[n - 1].ac, ents[n - 1].mirror);
   std::lock_guard<std::mutex> lk(g_batch_mu);
   BatchWork &bw = g_batch[dev];
   CTAS_HIP_TRY(hipStreamSynchronize(s));
@@ -943,15 +1002,17 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   }
   CTAS_HIP_TRY(hipMemcpy(bw.d_items, its.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
   CTAS_HIP_TRY(hipMemcpy(bw.d_items_f, fits.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
+  if (st != ST_F32) CTAS_HIP_TRY(launch_pad_zero(bw.d_items, n, s));
   if (contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(bw.d_items, n, s));
   for (const auto &g : groups)
     CTAS_HIP_TRY(launch_filter(bw.d_items_f + std::get<0>(g), std::get<1>(g), std::get<2>(g), std::get<3>(g),
                                std::get<3>(g), contrast, st, s));
-  (void)pairs;
-  (void)min_logZ;
-  (void)max_logZ;
-  CTAS_HIP_TRY(launch_backproject_persistent(bw.d_items, n, tiles, mode, st, bw.d_counter,
-                                             persistent_workgroups(), s));
+  for (size_t l = 0; l < launches.size(); ++l) {
+    const auto &L = launches[l];
+    CTAS_HIP_TRY(launch_backproject_persistent(bw.d_items + std::get<0>(L), std::get<1>(L), std::get<2>(L),
+                                               std::get<3>(L), std::get<4>(L), mode, st, std::get<5>(L),
+                                               bw.d_counter + l, cus, s));
+  }
   CTAS_HIP_TRY(hipStreamSynchronize(s));
   if (contrast == CTAS_CONTRAST_ABS) {
     std::vector<int> stv(n);

@@ -146,10 +146,12 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
 // their tiles cover the columns up to w/2 together with the mirrored ones.
 hipError_t launch_backproject(const Item *d_items, int n_items, int total_tiles, int kind,
                               int angles_per_pass, int mode, int store, bool mirror, hipStream_t s);
-// Same, one persistent launch with a work counter; items may mix tile kinds.
-hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int total_tiles,
-                                         int mode, int store, unsigned *d_counter,
-                                         int n_workgroups, hipStream_t s);
+// Same, one persistent launch with a work counter over the tiles of items of
+// one tile kind and pass length (widths and angle counts may differ); mirror:
+// FP16/BF16 items whose angles pair up (Item::slots). cus: compute units.
+hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int total_tiles, int kind,
+                                         int angles_per_pass, int mode, int store, bool mirror,
+                                         unsigned *d_counter, int cus, hipStream_t s);
 // Ring filter (RingFilter::apply) on S sinograms in place.
 hipError_t launch_ring(float *d_sino, float *d_avg, int S, int N, int w, int box, hipStream_t s);
 
@@ -179,8 +181,6 @@ constexpr int kScoreBlocks = 128;
 // Phantom projector (phantom.hip).
 hipError_t launch_phantom(int N, int w, double c, int n_ell, const double *d_ell, double N0,
                           uint64_t seed, const float *d_gain, float *d_out, hipStream_t s);
-
-int persistent_workgroups();
 
 // Makes work queued next on `waiter` start after the work already queued on
 // `producer` (nullptr: the null stream, whose work is ordered with every

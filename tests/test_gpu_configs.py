@@ -140,3 +140,50 @@ def test_sweep_81_centres_at_2048x1800():
     for k, r in zip(picks, ref):
         print(f"axis {axes[k]:.1f}: gpu {got[k]:.6e} oracle {r:.6e}")
         assert abs(got[k] - r) <= 2e-4 * abs(r)
+
+
+def test_autotune_picks_a_candidate_and_caches_it_per_class():
+    """ctas_fbp_autotune times the candidate tiles (64x64, 128x64, 128x128;
+    16 or 32 angles per pass) and keeps the fastest for the (width class,
+    angle-count class): a new handle of the same classes starts from it, and
+    the tuned kernel gives the oracle's slice bit for bit (FLT)."""
+    N, w, c = 360, 768, 2.5
+    rec = ca.CTrec(w, N, ca.FLT, "RAMP")
+    rec.autotune(2)
+    kind, ac = rec.tiles()
+    assert kind in (ca.TILE_64x64, ca.TILE_128x64, ca.TILE_128x128) and ac in (16, 32)
+    other = ca.CTrec(700, 300, ca.FLT, "RAMP")  # same classes (1024, 512)
+    assert other.tiles() == (kind, ac)
+    rng = np.random.default_rng(9)
+    P = rng.standard_normal((N, w)).astype(np.float32)
+    _, out = rec.reconstruct(P, c)
+    assert np.array_equal(out[0], oracle.bp_ctas(P, oracle.angles(N), c))
+
+
+@pytest.mark.parametrize("store", [ca.STORE_FP16, ca.STORE_BF16])
+def test_uneven_batch_reduced_precision_mirrored(torch_cuda, store):
+    """FP16/BF16 storage in the uneven batch: items whose angles pair up over
+    180 degrees take the mirrored persistent launch, the others (an odd angle
+    count over 150 degrees) the row-pair one; each against the FP64 oracle
+    in the same CTAS sampling, 2e-3 relative RMS on 8 rows."""
+    torch = torch_cuda
+    shapes = [(360, 512, 180.0), (900, 1024, 180.0), (181, 768, 150.0), (1440, 2048, 180.0), (360, 3072, 180.0)]
+    items, refs, rows_of, arcs = [], [], [], []
+    for k, (N, w, arc) in enumerate(shapes):
+        c = [1.5, -3.25, 2.0, 3.37, -6.5][k]
+        P = oracle.phantom_sino(N, w, c, oracle.random_ellipses(16, w, 2.0 * w / 2048, 900 + k), 1e5, 900 + k)
+        Pf = oracle.prepare(P, oracle.filter_window(5, 0.0, w, ca.ABS), ca.ABS)
+        rows = np.linspace(w // 8, w - w // 8, 8).astype(np.int32)
+        refs.append(oracle.bp_f64_rows(Pf, c, rows, np.deg2rad(arc)))
+        rows_of.append(rows)
+        items.append((torch.from_numpy(P).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), c))
+        arcs.append(arc)
+    # one arc per call: the 150-degree item goes in a batch of its own
+    for arc in sorted(set(arcs)):
+        ca.batch_uneven([it for it, a in zip(items, arcs) if a == arc], ca.ABS, 5, 0.0, arc, store)
+    torch.cuda.synchronize()
+    for k, (N, w, arc) in enumerate(shapes):
+        got = items[k][1].cpu().numpy()[rows_of[k]]
+        err = rel_rms(got, refs[k])
+        print(f"store={store:#x} w={w} N={N} arc={arc}: rel_rms {err:.2e}")
+        assert err <= 2e-3
