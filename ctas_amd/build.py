@@ -86,7 +86,7 @@ def build_cpp_api(force: bool = True) -> Path:
     rpath = "-Wl,-rpath,$ORIGIN"
     if force or _needs(lib, deps):
         _run([cxx, *CXX_FLAGS, "-shared", *inc, *map(str, api_src), "-o", str(lib),
-              "-L", str(PKG), "-lctas_hip", rpath, "-pthread"])
+              "-L", str(PKG), "-lctas_hip", rpath, "-pthread", "-ldl"])
     bindir = PKG / "bin"
     bindir.mkdir(exist_ok=True)
     for tool in sorted((CSRC / "tools").glob("*.cpp")):

@@ -5,8 +5,10 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <sstream>
 
 namespace ctas {
@@ -166,6 +168,13 @@ std::vector<ImagePlane> index_images(const std::vector<std::string> &paths, Shap
   for (const std::string &p : paths) {
     Shape2 sh;
     long n = 1;
+    if (is_hdf5_spec(p)) {
+      const std::vector<long> idx = hdf5_planes(p, sh);
+      if (planes.empty()) shape = sh;
+      else if (sh != shape) throw_error("image", "All images must have the same shape (\"" + p + "\" differs).");
+      for (long k : idx) planes.push_back({p, k, true});
+      continue;
+    }
     if (ends_with(p, ".npy")) {
       const NpyInfo info = npy_info(p);
       n = info.dims[0];
@@ -182,6 +191,13 @@ std::vector<ImagePlane> index_images(const std::vector<std::string> &paths, Shap
 
 void read_plane(const ImagePlane &p, const Shape2 &shape, float *dst) {
   const size_t elems = static_cast<size_t>(shape[0]) * shape[1];
+  if (p.h5) {
+    Shape2 face;
+    hdf5_planes(p.path, face);
+    if (face != shape) throw_error("image", "\"" + p.path + "\" changed shape.");
+    hdf5_read_plane(p.path, p.plane, dst);
+    return;
+  }
   if (ends_with(p.path, ".npy")) {
     const NpyInfo info = npy_info(p.path);
     if (info.dims[1] != shape[0] || info.dims[2] != shape[1] || p.plane >= info.dims[0])
@@ -197,12 +213,71 @@ void read_plane(const ImagePlane &p, const Shape2 &shape, float *dst) {
   std::memcpy(dst, m.data(), elems * 4);
 }
 
-NpyWriter::NpyWriter(const std::string &path, long count, long rows, long cols)
-    : plane_bytes_(static_cast<size_t>(rows) * cols * 4), path_(path) {
+namespace {
+
+int sample_bytes(int bpp) { return bpp ? std::abs(bpp) / 8 : 4; }
+
+std::string npy_descr(int bpp) {
+  if (!bpp) return "<f4";
+  return std::string(bpp < 0 ? "<i" : "<u") + std::to_string(std::abs(bpp) / 8);
+}
+
+template <typename T>
+void limit_to(const float *data, size_t n, float mincon, float maxcon, T *out) {
+  if (mincon == maxcon && n) {
+    mincon = *std::min_element(data, data + n);
+    maxcon = *std::max_element(data, data + n);
+  }
+  if (mincon == maxcon) {
+    std::fill(out, out + n, T(0));
+    return;
+  }
+  const float lo = static_cast<float>(std::numeric_limits<T>::min());
+  const float type_width = static_cast<float>(std::numeric_limits<T>::max()) - lo;
+  const float lims_width = maxcon - mincon;
+  for (size_t i = 0; i < n; ++i) {
+    const float interim = (data[i] - mincon) / lims_width;
+    if (interim < 0) {
+      out[i] = std::numeric_limits<T>::min();
+    } else if (interim > 1) {
+      out[i] = std::numeric_limits<T>::max();
+    } else {
+      // the float value can round up past the type's maximum (32 and 64 bits)
+      const double v = static_cast<double>(lo + interim * type_width);
+      out[i] = v >= static_cast<double>(std::numeric_limits<T>::max()) ? std::numeric_limits<T>::max()
+                                                                        : static_cast<T>(v);
+    }
+  }
+}
+
+}  // namespace
+
+std::vector<unsigned char> to_integer(const float *data, size_t n, int bpp, float mincon, float maxcon) {
+  std::vector<unsigned char> out(n * static_cast<size_t>(sample_bytes(bpp)));
+  void *o = out.data();
+  switch (bpp) {
+    case -8: limit_to(data, n, mincon, maxcon, static_cast<int8_t *>(o)); break;
+    case 8: limit_to(data, n, mincon, maxcon, static_cast<uint8_t *>(o)); break;
+    case -16: limit_to(data, n, mincon, maxcon, static_cast<int16_t *>(o)); break;
+    case 16: limit_to(data, n, mincon, maxcon, static_cast<uint16_t *>(o)); break;
+    case -32: limit_to(data, n, mincon, maxcon, static_cast<int32_t *>(o)); break;
+    case 32: limit_to(data, n, mincon, maxcon, static_cast<uint32_t *>(o)); break;
+    case -64: limit_to(data, n, mincon, maxcon, static_cast<int64_t *>(o)); break;
+    case 64: limit_to(data, n, mincon, maxcon, static_cast<uint64_t *>(o)); break;
+    default: throw_error("save image", "unsupported bits per sample " + std::to_string(bpp));
+  }
+  return out;
+}
+
+NpyWriter::NpyWriter(const std::string &path, long count, long rows, long cols, int bpp, float mincon,
+                     float maxcon)
+    : plane_bytes_(static_cast<size_t>(rows) * cols * sample_bytes(bpp)), path_(path), bpp_(bpp), mincon_(mincon),
+      maxcon_(maxcon), elems_(rows * cols) {
+  if (bpp) (void)to_integer(nullptr, 0, bpp, 0, 0);  // rejects an unsupported bpp
   fd_ = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
   if (fd_ < 0) throw_error("image", "Could not write file \"" + path + "\".");
-  std::string h = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(count) + ", " +
-                  std::to_string(rows) + ", " + std::to_string(cols) + "), }";
+  std::string h = "{'descr': '" + npy_descr(bpp) + "', 'fortran_order': False, 'shape': (" +
+                  std::to_string(count) + ", " + std::to_string(rows) + ", " + std::to_string(cols) + "), }";
   while ((10 + h.size() + 1) % 64) h += ' ';
   h += '\n';
   std::string head("\x93NUMPY\x01\x00", 8);
@@ -219,7 +294,9 @@ NpyWriter::~NpyWriter() {
 }
 
 void NpyWriter::write(long index, const float *plane) {
-  const char *p = reinterpret_cast<const char *>(plane);
+  std::vector<unsigned char> conv;
+  if (bpp_) conv = to_integer(plane, static_cast<size_t>(elems_), bpp_, mincon_, maxcon_);
+  const char *p = bpp_ ? reinterpret_cast<const char *>(conv.data()) : reinterpret_cast<const char *>(plane);
   size_t done = 0;
   const off_t at = static_cast<off_t>(data_ + plane_bytes_ * static_cast<size_t>(index));
   while (done < plane_bytes_) {
@@ -237,7 +314,16 @@ std::vector<Map> read_images(const std::string &path) {
 void write_tiff(const std::string &path, const Map &img) { write_tiff(path, img.data(), img.rows(), img.columns()); }
 
 void write_tiff(const std::string &path, const float *data, long rows, long cols) {
+  write_tiff(path, data, rows, cols, 0, 0.0f, 0.0f);
+}
+
+void write_tiff(const std::string &path, const float *data, long rows, long cols, int bpp, float mincon,
+                float maxcon) {
   const uint32_t w = static_cast<uint32_t>(cols), h = static_cast<uint32_t>(rows);
+  const uint32_t sb = static_cast<uint32_t>(sample_bytes(bpp));
+  std::vector<unsigned char> conv;
+  if (bpp) conv = to_integer(data, static_cast<size_t>(w) * h, bpp, mincon, maxcon);
+  const void *pixels = bpp ? static_cast<const void *>(conv.data()) : static_cast<const void *>(data);
   std::vector<unsigned char> v;
   v.insert(v.end(), {'I', 'I'});
   put16(v, 42);
@@ -258,18 +344,18 @@ void write_tiff(const std::string &path, const float *data, long rows, long cols
   };
   entry(256, 4, 1, w);
   entry(257, 4, 1, h);
-  entry(258, 3, 1, 32);
+  entry(258, 3, 1, 8 * sb);
   entry(259, 3, 1, 1);
   entry(262, 3, 1, 1);
   entry(273, 4, 1, data_off);
   entry(277, 3, 1, 1);
   entry(278, 4, 1, h);
-  entry(279, 4, 1, w * h * 4);
-  entry(339, 3, 1, 3);
+  entry(279, 4, 1, w * h * sb);
+  entry(339, 3, 1, bpp == 0 ? 3 : bpp < 0 ? 2 : 1);  // IEEE float, signed or unsigned integer
   put32(v, 0);
   const size_t at = v.size();
-  v.resize(at + static_cast<size_t>(w) * h * 4);
-  std::memcpy(v.data() + at, data, static_cast<size_t>(w) * h * 4);
+  v.resize(at + static_cast<size_t>(w) * h * sb);
+  std::memcpy(v.data() + at, pixels, static_cast<size_t>(w) * h * sb);
   std::ofstream f(path, std::ios::binary);
   if (!f) throw_error("image", "Could not write file \"" + path + "\".");
   f.write(reinterpret_cast<const char *>(v.data()), static_cast<std::streamsize>(v.size()));
@@ -307,3 +393,15 @@ std::string mask_name(const std::string &mask, long index, long count) {
 }
 
 }  // namespace ctas
+
+// Test and scripting helper of the tools' integer output (not part of the
+// engine's C ABI): to_integer into `out` (n samples of |bpp|/8 bytes).
+extern "C" int ctas_io_to_integer(const float *data, size_t n, int bpp, float mincon, float maxcon, void *out) {
+  try {
+    const std::vector<unsigned char> v = ctas::to_integer(data, n, bpp, mincon, maxcon);
+    std::memcpy(out, v.data(), v.size());
+    return 0;
+  } catch (const ctas::CtasErr &) {
+    return -1;
+  }
+}

@@ -1,5 +1,6 @@
 // ctas-ct: CT reconstruction of sinogram(s), the drop-in for the reference's
-// `ctas ct` (src/executables/ct.cpp:43-244): same options and defaults; the
+// `ctas ct` (src/executables/ct.cpp:43-244): same options and defaults
+// (inputs: TIFF, .npy stacks, HDF5 "file:dataset[:[xyz]slices]"); the
 // slices go to the GPU in batches instead of one per CPU thread, with the
 // reading of the next batch and the writing of the previous one overlapping
 // the device work (ctas_fbp_pipeline).
@@ -31,6 +32,10 @@ void usage(const char *cmd) {
             << "  -r, --resolution FLOAT  pixel size (micron)\n"
             << "  -w, --wavelength FLOAT  wavelength (Angstrom)\n"
             << "  -R, --ring INT          half size of the ring filter (default 0)\n"
+            << "  -m, --min FLOAT         pixel value corresponding to black\n"
+            << "  -M, --max FLOAT         pixel value corresponding to white\n"
+            << "  -i, --int BPP           bits per pixel of integer output (8, 16, 32, 64; negative: signed;\n"
+            << "                          default 0, float32, unless -m or -M is given: then 8)\n"
             << "      --cpu               accepted for compatibility (the GPU is always used)\n"
             << "      --interp ctas|linear  detector sampling (default ctas, the reference's)\n"
             << "      --store fp32|fp16|bf16  filtered-sinogram storage (default fp32)\n"
@@ -50,8 +55,9 @@ std::string upgrade(const std::string &path, const std::string &prefix) {
 int main(int argc, char **argv) {
   std::string out, contrast = "ABS", filter = "RAMP", interp = "ctas", store = "fp32";
   float center = 0, arc = 180, dd = 0, lambda = 0;
-  int ring = 0, batch = 16, io_threads = 8;
-  bool verbose = false, have_out = false;
+  int ring = 0, batch = 16, io_threads = 8, bpp = 0;
+  float mincon = 0, maxcon = 0;
+  bool verbose = false, have_out = false, have_limits = false, have_bpp = false;
   static option opts[] = {{"output", required_argument, nullptr, 'o'},   {"contrast", required_argument, nullptr, 'k'},
                           {"center", required_argument, nullptr, 'c'},   {"arcan", required_argument, nullptr, 'a'},
                           {"filter", required_argument, nullptr, 'f'},   {"resolution", required_argument, nullptr, 'r'},
@@ -81,9 +87,9 @@ int main(int argc, char **argv) {
         lambda /= 1.0E10f;
         break;
       case 'R': ring = std::atoi(optarg); break;
-      case 'm': case 'M': case 'i':
-        std::cerr << "Integer output is not supported: slices are written as float32.\n";
-        return 1;
+      case 'm': mincon = std::strtof(optarg, nullptr); have_limits = true; break;
+      case 'M': maxcon = std::strtof(optarg, nullptr); have_limits = true; break;
+      case 'i': bpp = std::atoi(optarg); have_bpp = true; break;
       case 1: break;
       case 2: interp = optarg; break;
       case 3: store = optarg; break;
@@ -98,6 +104,13 @@ int main(int argc, char **argv) {
     return 0;
   }
   if (arc <= 0.0f) { std::cerr << "CT arc (given by -a) must be strictly positive.\n"; return 1; }
+  // ct.cpp:148: limits without -i mean 8-bit output
+  if (have_limits && !have_bpp) bpp = 8;
+  if (bpp != 0 && bpp != 8 && bpp != -8 && bpp != 16 && bpp != -16 && bpp != 32 && bpp != -32 && bpp != 64 &&
+      bpp != -64) {
+    std::cerr << "Unsupported bits per sample " << bpp << ".\n";
+    return 1;
+  }
   const auto t_start = std::chrono::steady_clock::now();
   auto secs = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(); };
   try {
@@ -128,9 +141,12 @@ int main(int argc, char **argv) {
       std::unique_ptr<NpyWriter> npy;
       bool verbose;
       long batch;
+      int bpp;
+      float mincon, maxcon;
       std::atomic<long> done{0};
-    } ctx{planes, ish, osh, out, S, npy ? std::make_unique<NpyWriter>(out, S, osh[0], osh[1]) : nullptr, verbose,
-          batch};
+    } ctx{planes, ish, osh, out, S,
+          npy ? std::make_unique<NpyWriter>(out, S, osh[0], osh[1], bpp, mincon, maxcon) : nullptr, verbose, batch,
+          bpp, mincon, maxcon};
     auto read = [](void *u, long i, float *sino) -> int {
       Ctx &c = *static_cast<Ctx *>(u);
       try {
@@ -144,7 +160,7 @@ int main(int argc, char **argv) {
       Ctx &c = *static_cast<Ctx *>(u);
       try {
         if (c.npy) c.npy->write(i, slice);
-        else write_tiff(mask_name(c.out, i, c.S), slice, c.osh[0], c.osh[1]);
+        else write_tiff(mask_name(c.out, i, c.S), slice, c.osh[0], c.osh[1], c.bpp, c.mincon, c.maxcon);
         const long done = ++c.done;
         if (c.verbose && (done % c.batch == 0 || done == c.S)) std::cerr << "CT'ing: " << done << "/" << c.S << "\n";
         return 0;

@@ -192,3 +192,45 @@ def test_ct_tool_npy_stack_pipeline(tmp_path):
     _, ref = ca.CTrec(w, N, ca.ABS, "HANN").reconstruct(ref_in, [c] * S)
     assert got.shape == (S, w, w)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.gpu
+def test_ct_tool_hdf5_input_and_integer_output(tmp_path):
+    """ctas-ct on a HDF5 sinogram volume with a slice string
+    ("vol.h5:/sino:z1:4", external.world.cpp:325-376) gives the slices of
+    the same sinograms reconstructed from a .npy stack; with -m/-M/-i the
+    slices are written as integers by the reference's limit<T> (ct.cpp:112-118,
+    148: limits without -i mean 8 bits), checked against the float output."""
+    if ca.device_count() < 1:
+        pytest.skip("no HIP device")
+    import ctypes as C
+    ca.lib()
+    io = C.CDLL(str(Path(ca.__file__).resolve().parent / "libctas.so"))
+    fp = C.POINTER(C.c_float)
+    io.ctas_io_h5_write.argtypes = [C.c_char_p, fp, C.c_long, C.c_long, C.c_long]
+    io.ctas_io_to_integer.argtypes = [fp, C.c_size_t, C.c_int, C.c_float, C.c_float, C.c_void_p]
+    S, N, w, c = 5, 180, 256, 1.25
+    sinos = np.stack([oracle.phantom_sino(N, w, c, oracle.random_ellipses(6, w, 1.0, 80 + s), 1e5, 80 + s)
+                      for s in range(S)]).astype(np.float32)
+    if io.ctas_io_h5_write(f"{tmp_path / 'vol.h5'}:/sino".encode(), sinos.ctypes.data_as(fp), S, N, w) != 0:
+        pytest.skip("the HDF5 library is not available")
+    np.save(tmp_path / "ref.npy", sinos[1:4])
+    r = run("ctas-ct", f"{tmp_path / 'vol.h5'}:/sino:z1:4", "-o", tmp_path / "h5.npy", "-c", c, "-f", "HANN")
+    assert r.returncode == 0, r.stderr
+    r = run("ctas-ct", tmp_path / "ref.npy", "-o", tmp_path / "ref_rec.npy", "-c", c, "-f", "HANN")
+    assert r.returncode == 0, r.stderr
+    got, ref = np.load(tmp_path / "h5.npy"), np.load(tmp_path / "ref_rec.npy")
+    assert got.shape == (3, w, w) and np.array_equal(got, ref)
+    # integer output: 16-bit signed with limits, 8-bit from limits alone
+    for args, bpp, lo, hi in ((("-i", -16, "-m", -0.01, "-M", 0.03), -16, -0.01, 0.03),
+                              (("-m", 0.0, "-M", 0.02), 8, 0.0, 0.02)):
+        r = run("ctas-ct", tmp_path / "ref.npy", "-o", tmp_path / "int.npy", "-c", c, "-f", "HANN", *args)
+        assert r.returncode == 0, r.stderr
+        got = np.load(tmp_path / "int.npy")
+        want = np.zeros(ref.shape, got.dtype)
+        assert got.dtype == np.dtype(("i" if bpp < 0 else "u") + str(abs(bpp) // 8))
+        assert io.ctas_io_to_integer(np.ascontiguousarray(ref).ctypes.data_as(fp), ref.size, bpp, lo, hi,
+                                     want.ctypes.data) == 0
+        assert np.array_equal(got, want)
+    r = run("ctas-ct", tmp_path / "ref.npy", "-o", tmp_path / "bad.tif", "-i", 12)
+    assert r.returncode != 0 and "Unsupported bits per sample" in r.stderr
