@@ -7,7 +7,7 @@
 // 2x180x256, then a 11264->16 fully connected layer with leaky ReLU and a
 // 16->2 output (offset in
 // downsampled detector bins, confidence logit). Activations and weights are
-// bfloat16; every product runs on the matrix cores (v_mfma_f32_16x16x16_bf16,
+// bfloat16; every product runs on the matrix cores (v_mfma_f32_16x16x32_bf16,
 // float32 accumulation) through one implicit-GEMM kernel whose operands are
 // gathered by small functors: convolution forward (W x im2col(x)), backward
 // data (W^T x col(g) with the mirrored stencil), backward weights
@@ -45,7 +45,6 @@
 namespace ctas {
 namespace cnn {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t bf16_rne(float v) {
@@ -65,54 +64,60 @@ __device__ __forceinline__ float leaky(float v) { return v > 0.0f ? v : kLeak * 
 constexpr int kBT = 64;                    // GEMM block tile (M and N)
 
 // ---- generic GEMM: C[m][n] = sum_k A(m, k) B(k, n) -------------------------
-// Block: 256 threads = 4 waves, 64x64 C tile, each wave 32x32 as 2x2 MFMA
-// 16x16x16 tiles. K advances by 16 through LDS. Operand layout of
-// v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)+j] and
-// B[4(l/16)+j][l%16] (j = 0..3); the result D[4(l/16)+r][l%16] (r = 0..3).
+// Block: 256 threads = 4 waves, 64x64 C tile, each wave 32x32 as 2x2 tiles
+// of gfx950's v_mfma_f32_16x16x32_bf16. K advances by 32 through LDS. Operand
+// layout of that instruction: lane l holds A[l%16][8(l/16)+j] and
+// B[8(l/16)+j][l%16] (j = 0..7, eight bf16 in four registers); the result
+// D[4(l/16)+r][l%16] (r = 0..3). Both tiles sit in LDS with K contiguous
+// ([row][k] for A, [col][k] for B, rows padded to 40 elements), so that a
+// lane's eight operands are one 16-byte read; the operand functors gather
+// eight consecutive k of one row or column per thread, written as one
+// 16-byte store.
 // gridDim.z splits K into ranges of k_chunk; each split writes its own
 // partial (the store functor gets the split index).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+constexpr int kKS = 32, kKP = kKS + 8;  // K per step, padded LDS row
+
 template <class FA, class FB, class FC>
 __global__ __launch_bounds__(256) void gemm_kernel(FA fa, FB fb, FC fc, int M, int N, int K, int k_chunk) {
-  __shared__ uint16_t As[kBT][16 + 2];
-  __shared__ uint16_t Bs[16][kBT + 2];
+  __shared__ __align__(16) uint16_t As[kBT][kKP];
+  __shared__ __align__(16) uint16_t Bs[kBT][kKP];
   const int tid = threadIdx.x;
   const int m0 = blockIdx.y * kBT, n0 = blockIdx.x * kBT;
   const int kb = blockIdx.z * k_chunk, ke = min(K, kb + k_chunk);
   const int wave = tid >> 6, lane = tid & 63;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int si = tid >> 2, sk = (tid & 3) * 8;  // staging: row / column si, k offset sk
   f4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = kb; k0 < ke; k0 += 16) {
+  for (int k0 = kb; k0 < ke; k0 += kKS) {
+    u16x8 va, vb;
+    const int m = m0 + si, n = n0 + si;
 #pragma unroll
-    for (int e = tid; e < kBT * 16; e += 256) {
-      const int i = e >> 4, kk = e & 15;
-      const int m = m0 + i, k = k0 + kk;
-      As[i][kk] = (m < M && k < ke) ? fa(m, k) : static_cast<uint16_t>(0);
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + sk + j;
+      va[j] = (m < M && k < ke) ? fa(m, k) : static_cast<uint16_t>(0);
+      vb[j] = (n < N && k < ke) ? fb(k, n) : static_cast<uint16_t>(0);
     }
-#pragma unroll
-    for (int e = tid; e < kBT * 16; e += 256) {
-      const int kk = e >> 6, j = e & 63;
-      const int n = n0 + j, k = k0 + kk;
-      Bs[kk][j] = (n < N && k < ke) ? fb(k, n) : static_cast<uint16_t>(0);
-    }
+    *reinterpret_cast<u16x8 *>(&As[si][sk]) = va;
+    *reinterpret_cast<u16x8 *>(&Bs[si][sk]) = vb;
     __syncthreads();
-    const int r16 = lane & 15, k4 = (lane >> 4) * 4;
+    const int r16 = lane & 15, k8 = (lane >> 4) * 8;
+    bf16x8 a[2], b[2];
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi) {
-      s4 a;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = static_cast<short>(As[wm + 16 * mi + r16][k4 + j]);
-#pragma unroll
-      for (int ni = 0; ni < 2; ++ni) {
-        s4 b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = static_cast<short>(Bs[k4 + j][wn + 16 * ni + r16]);
-        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mi][ni], 0, 0, 0);
-      }
+    for (int t = 0; t < 2; ++t) {
+      a[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&As[wm + 16 * t + r16][k8]));
+      b[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8 *>(&Bs[wn + 16 * t + r16][k8]));
     }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
     __syncthreads();
   }
   const int col = lane & 15, row4 = (lane >> 4) * 4;
@@ -130,7 +135,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(FA fa, FB fb, FC fc, int M, i
 template <class FA, class FB, class FC>
 hipError_t gemm(FA fa, FB fb, FC fc, int M, int N, int K, hipStream_t s, int splits = 1) {
   int chunk = (K + splits - 1) / splits;
-  chunk = (chunk + 15) & ~15;
+  chunk = (chunk + kKS - 1) & ~(kKS - 1);
   splits = (K + chunk - 1) / chunk;
   const dim3 grid((N + kBT - 1) / kBT, (M + kBT - 1) / kBT, splits);
   hipLaunchKernelGGL((gemm_kernel<FA, FB, FC>), grid, dim3(256), 0, s, fa, fb, fc, M, N, K, chunk);
@@ -310,12 +315,24 @@ __global__ void maxpool_bwd(const uint16_t *__restrict__ x, const float *__restr
 }
 
 // Sum of the split-K partials (fixed order) into the weight gradient.
-__global__ void reduce_splits(const float *__restrict__ ws, float *__restrict__ out, int splits, int MN) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= MN) return;
+// Sum over the splits of a K-split product, in a fixed order: 16 outputs per
+// block, each summed by 16 threads over interleaved splits (z = g, g + 16,
+// ...) and their 16 partials added in order.
+__global__ __launch_bounds__(256) void reduce_splits(const float *__restrict__ ws, float *__restrict__ out,
+                                                     int splits, int MN) {
+  const int o = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int i = blockIdx.x * 16 + o;
   float s = 0.0f;
-  for (int z = 0; z < splits; ++z) s += ws[static_cast<size_t>(z) * MN + i];
-  out[i] = s;
+  if (i < MN)
+    for (int z = g; z < splits; z += 16) s += ws[static_cast<size_t>(z) * MN + i];
+  __shared__ float part[16][17];
+  part[g][o] = s;
+  __syncthreads();
+  if (g == 0 && i < MN) {
+    float t = part[0][o];
+    for (int k = 1; k < 16; ++k) t += part[k][o];
+    out[i] = t;
+  }
 }
 
 // Epilogue of a split forward product: y[b][m][p] = bf16(leaky(sum over
@@ -350,6 +367,37 @@ __global__ void bias_grad(const float *__restrict__ d, const uint16_t *__restric
     __syncthreads();
   }
   if (threadIdx.x == 0) db[co] = red[0];
+}
+
+// Bias gradient of a convolution layer in two fixed-order steps that fill
+// the chip: block (co, b) sums the slope-scaled gradient over the pixels of
+// one channel of one sample (contiguous, coalesced) into part[co][b], then
+// bias_grad_sum adds the samples in order.
+__global__ __launch_bounds__(256) void bias_grad_part(const float *__restrict__ d, const uint16_t *__restrict__ post,
+                                                      float *__restrict__ part, int B, int C, int HW) {
+  const int co = blockIdx.x, b = blockIdx.y;
+  const size_t base = (static_cast<size_t>(b) * C + co) * HW;
+  float s = 0.0f;
+  for (int p = threadIdx.x; p < HW; p += blockDim.x) {
+    const size_t i = base + p;
+    s += bf16_f(post[i]) > 0.0f ? d[i] : kLeak * d[i];
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[static_cast<size_t>(co) * B + b] = red[0];
+}
+
+__global__ void bias_grad_sum(const float *__restrict__ part, float *__restrict__ db, int B, int C) {
+  const int co = blockIdx.x * blockDim.x + threadIdx.x;
+  if (co >= C) return;
+  float s = 0.0f;
+  for (int b = 0; b < B; ++b) s += part[static_cast<size_t>(co) * B + b];
+  db[co] = s;
 }
 
 // Loss over targets t[b] = {offset, tolerance} (output units):
@@ -612,7 +660,7 @@ int product_act(ctas_cnn *n, FA fa, FB fb, int M, int Ncols, int K, const float 
   }
   if (int rc = ensure_ws(n, static_cast<size_t>(splits) * M * Ncols)) return rc;
   int chunk = (K + splits - 1) / splits;
-  chunk = (chunk + 15) & ~15;
+  chunk = (chunk + kKS - 1) & ~(kKS - 1);
   const int real = (K + chunk - 1) / chunk;
   CTAS_HIP_TRY(gemm(fa, fb, StorePartial{n->ws, M, Ncols}, M, Ncols, K, s, splits));
   hipLaunchKernelGGL(finish_act, dim3((M * Ncols + 255) / 256), dim3(256), 0, s, n->ws, real, M, Ncols, bias, y, HW);
@@ -643,13 +691,15 @@ int forward(ctas_cnn *n, const uint16_t *x, int B) {
 // fixed order)
 template <class FA, class FB>
 int weight_grad(ctas_cnn *n, FA fa, FB fb, int M, int Ncols, int K, float *dst) {
-  const int splits = std::max(1, std::min(256, K / 4096));
+  // enough K splits that the few output tiles of a weight gradient still
+  // put several workgroups on every CU
+  const int splits = std::max(1, std::min(2048, K / 1024));
   if (int rc = ensure_ws(n, static_cast<size_t>(splits) * M * Ncols)) return rc;
   int chunk = (K + splits - 1) / splits;
-  chunk = (chunk + 15) & ~15;
+  chunk = (chunk + kKS - 1) & ~(kKS - 1);
   const int real = (K + chunk - 1) / chunk;
   CTAS_HIP_TRY(gemm(fa, fb, StorePartial{n->ws, M, Ncols}, M, Ncols, K, n->stream, splits));
-  hipLaunchKernelGGL(reduce_splits, dim3((M * Ncols + 255) / 256), dim3(256), 0, n->stream, n->ws, dst, real,
+  hipLaunchKernelGGL(reduce_splits, dim3((M * Ncols + 15) / 16), dim3(256), 0, n->stream, n->ws, dst, real,
                      M * Ncols);
   return CTAS_OK;
 }
@@ -682,7 +732,10 @@ int backward(ctas_cnn *n, const uint16_t *x, int B) {
     if (int rc = weight_grad(n, GradA{gact, n->act[l], L.cout, L.H, L.W}, Im2colT{Im2col{in, L.cin, L.H, L.W}},
                              L.cout, L.cin * 9, B * HW, G32(n, 2 * l)))
       return rc;
-    hipLaunchKernelGGL(bias_grad, dim3(L.cout), dim3(256), 0, s, gact, n->act[l], G32(n, 2 * l + 1), B, L.cout, HW);
+    // the workspace's split partials were reduced by weight_grad on this stream
+    if (int rc = ensure_ws(n, static_cast<size_t>(L.cout) * B)) return rc;
+    hipLaunchKernelGGL(bias_grad_part, dim3(L.cout, B), dim3(256), 0, s, gact, n->act[l], n->ws, B, L.cout, HW);
+    hipLaunchKernelGGL(bias_grad_sum, dim3(1), dim3(64), 0, s, n->ws, G32(n, 2 * l + 1), B, L.cout);
     if (l > 0) {
       CTAS_HIP_TRY(gemm(WT{W16(n, 2 * l), L.cin}, ColG{gact, n->act[l], L.cout, L.H, L.W},
                         StoreDX{n->gbig2, L.cin, L.H, L.W}, L.cin, B * HW, L.cout * 9, s));
