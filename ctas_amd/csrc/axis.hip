@@ -194,28 +194,32 @@ constexpr int kScoreChunk = 64;  // centres per batch (bounds the scratch slices
 }  // namespace
 
 // improveMe (lct.cpp:1108-1129): bisection towards the lower neighbour.
-float improve_me(float eps, const std::vector<float> &vax, const std::vector<float> &err, int idx, int cdir,
-                 const std::function<float(float)> &f, int *rc) {
-  float aa = vax[idx - cdir];
-  float bb = vax[idx + cdir];
-  float rf = vax[idx];
-  float vl = err[idx];
-  while (std::fabs(aa - rf) > eps || std::fabs(rf - bb) > eps) {
-    const bool afar = std::fabs(aa - rf) > std::fabs(rf - bb);
-    float &lo = afar ? aa : bb;
-    float &hi = afar ? bb : aa;
-    const float x = (rf + lo) / 2;
-    const float vx = f(x);
-    if (*rc) return rf;
-    if (vl <= vx) {
-      lo = x;
+// The bracket around offsets[at] is halved on its wider side until both
+// sides are within eps; a probe that scores no better than the current best
+// becomes that side's edge, a better one the new best (the old best then
+// closes the other side).
+float improve_me(float eps, const std::vector<float> &offsets, const std::vector<float> &scores, int at, int dir,
+                 const std::function<float(float)> &score_at, int *rc) {
+  float left = offsets[at - dir];
+  float right = offsets[at + dir];
+  float best = offsets[at];
+  float best_score = scores[at];
+  while (std::fabs(left - best) > eps || std::fabs(best - right) > eps) {
+    const bool left_wider = std::fabs(left - best) > std::fabs(best - right);
+    float &wide = left_wider ? left : right;
+    float &narrow = left_wider ? right : left;
+    const float probe = (best + wide) / 2;
+    const float probe_score = score_at(probe);
+    if (*rc) return best;
+    if (best_score <= probe_score) {
+      wide = probe;
     } else {
-      vl = vx;
-      hi = rf;
-      rf = x;
+      best_score = probe_score;
+      narrow = best;
+      best = probe;
     }
   }
-  return rf;
+  return best;
 }
 
 namespace {
@@ -347,12 +351,12 @@ extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int
   if (w > 16384) return fail(CTAS_E_ARG, "projection width above 16384");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
-  const float cent = static_cast<float>(0.5 * (w - 1));
+  const float mid = static_cast<float>(0.5 * (w - 1));
   const size_t n = static_cast<size_t>(rows) * w;
-  const int stpsz = 16;
-  const int steps = static_cast<int>(1 + 2 * (maxDev == 0.0f ? 1.0f : maxDev * static_cast<float>(w) / stpsz));
+  const int kStep = 16;
+  const int n_trials = static_cast<int>(1 + 2 * (maxDev == 0.0f ? 1.0f : maxDev * static_cast<float>(w) / kStep));
   // device: p0, p180, profile results, trial shifts, their sums
-  const int max_trials = std::max(steps, 8) + w / stpsz + 4;
+  const int max_trials = std::max(n_trials, 8) + w / kStep + 4;
   struct Buf {
     void *p = nullptr;
     ~Buf() {
@@ -374,7 +378,7 @@ extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int
   float res[3];
   CTAS_HIP_TRY(hipMemcpy(res, d_res, sizeof res, hipMemcpyDeviceToHost));
   if (res[0] == 0.0f) return fail(CTAS_E_AXIS, "Impossible input projection averaging to zero.");
-  const float fcr = res[1], aver = res[2];
+  const float seed = res[1], mean_level = res[2];
   int dev_rc = CTAS_OK;
   // errors of trial shifts xs[0..k), one workgroup each
   auto errs = [&](const float *xs, int k, float *e) {
@@ -384,7 +388,7 @@ extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int
       dev_rc = fail(CTAS_E_DEVICE, "registration: upload failed");
       return;
     }
-    hipLaunchKernelGGL(pair_err_kernel, dim3(k), dim3(256), 0, nullptr, d_p0, d_p180, rows, w, d_sh, aver, d_acc);
+    hipLaunchKernelGGL(pair_err_kernel, dim3(k), dim3(256), 0, nullptr, d_p0, d_p180, rows, w, d_sh, mean_level, d_acc);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpy(acc.data(), d_acc, sizeof(double) * k, hipMemcpyDeviceToHost) != hipSuccess) {
       dev_rc = fail(CTAS_E_DEVICE, "registration: error sums failed on the device");
@@ -397,43 +401,43 @@ extern "C" int ctas_raxis_pair(const float *p0, const float *p180, int rows, int
     errs(&x, 1, &e);
     return e;
   };
-  std::vector<float> vax(steps), err(steps);
-  for (int ii = 0; ii < steps; ++ii) vax[ii] = 2 * (fcr - cent) + static_cast<float>(stpsz * (ii - (steps - 1) / 2));
-  errs(vax.data(), steps, err.data());
+  std::vector<float> offsets(n_trials), errors(n_trials);
+  for (int ii = 0; ii < n_trials; ++ii) offsets[ii] = 2 * (seed - mid) + static_cast<float>(kStep * (ii - (n_trials - 1) / 2));
+  errs(offsets.data(), n_trials, errors.data());
   if (dev_rc != CTAS_OK) return dev_rc;
   int idx = -1;
-  float minval = FLT_MAX;
-  for (int i = 1; i < steps - 1; ++i)
-    if (err[i] < err[i - 1] && err[i] < err[i + 1] && err[i] < minval) minval = err[idx = i];
+  float best_error = FLT_MAX;
+  for (int i = 1; i < n_trials - 1; ++i)
+    if (errors[i] < errors[i - 1] && errors[i] < errors[i + 1] && errors[i] < best_error) best_error = errors[idx = i];
   int rc = CTAS_OK;
   if (idx >= 0) {
-    *result = cent + 0.5f * improve_me(1.0f, vax, err, idx, 1, calc_err, &rc);
+    *result = mid + 0.5f * improve_me(1.0f, offsets, errors, idx, 1, calc_err, &rc);
     return dev_rc != CTAS_OK ? dev_rc : rc;
   }
-  idx = err[0] < err[steps - 1] ? 0 : steps - 1;
-  const int cdir = idx == 0 ? -1 : 1;
-  minval = err[idx];
-  const int maxSteps = static_cast<int>((static_cast<float>(w / 2) - static_cast<float>(cdir) * vax[idx]) / stpsz);
-  if (maxSteps <= 0) return fail(CTAS_E_AXIS, "Failed to calculate rotation centre.");
-  std::vector<float> v2(maxSteps + 2), e2(maxSteps + 2);
-  v2[0] = vax[idx - cdir];
-  v2[1] = vax[idx];
-  e2[0] = err[idx - cdir];
-  e2[1] = err[idx];
-  int idx2 = 1;
-  for (int i = 0; i < maxSteps; ++i) {
-    v2[i + 2] = vax[idx] + static_cast<float>(cdir * (i + 1) * stpsz);
-    e2[i + 2] = calc_err(v2[i + 2]);
+  idx = errors[0] < errors[n_trials - 1] ? 0 : n_trials - 1;
+  const int dir = idx == 0 ? -1 : 1;
+  best_error = errors[idx];
+  const int walk_limit = static_cast<int>((static_cast<float>(w / 2) - static_cast<float>(dir) * offsets[idx]) / kStep);
+  if (walk_limit <= 0) return fail(CTAS_E_AXIS, "Failed to calculate rotation centre.");
+  std::vector<float> walk_offsets(walk_limit + 2), walk_errors(walk_limit + 2);
+  walk_offsets[0] = offsets[idx - dir];
+  walk_offsets[1] = offsets[idx];
+  walk_errors[0] = errors[idx - dir];
+  walk_errors[1] = errors[idx];
+  int walk_best = 1;
+  for (int i = 0; i < walk_limit; ++i) {
+    walk_offsets[i + 2] = offsets[idx] + static_cast<float>(dir * (i + 1) * kStep);
+    walk_errors[i + 2] = calc_err(walk_offsets[i + 2]);
     if (dev_rc != CTAS_OK) return dev_rc;
-    if (e2[i + 2] < minval) {
-      minval = e2[i + 2];
-      idx2 = i + 2;
+    if (walk_errors[i + 2] < best_error) {
+      best_error = walk_errors[i + 2];
+      walk_best = i + 2;
     } else {
       break;
     }
   }
-  if (idx2 == maxSteps + 1) return fail(CTAS_E_AXIS, "Failed to calculate rotation centre.");
-  *result = cent + 0.5f * improve_me(1.0f, v2, e2, idx2, cdir, calc_err, &rc);
+  if (walk_best == walk_limit + 1) return fail(CTAS_E_AXIS, "Failed to calculate rotation centre.");
+  *result = mid + 0.5f * improve_me(1.0f, walk_offsets, walk_errors, walk_best, dir, calc_err, &rc);
   return dev_rc != CTAS_OK ? dev_rc : rc;
 }
 
@@ -451,53 +455,53 @@ extern "C" int ctas_raxis(const float *sino, int N, int w, float anarc, float ma
   if (arc == 0.0f) return fail(CTAS_E_AXIS, "Zero sinogram arc.");
   if (std::fabs(arc) < static_cast<float>(M_PI))
     return fail(CTAS_E_AXIS, "Sinogram arc " + std::to_string(std::fabs(arc) * 180 / M_PI) + " is less than 180 degree.");
-  const float maxDevLM = 0.25f;
-  if (maxDev < 0 || maxDev > maxDevLM)
+  const float kShiftLimit = 0.25f;
+  if (maxDev < 0 || maxDev > kShiftLimit)
     return fail(CTAS_E_AXIS, "Invalid maximum relative shift of the rotation axis " + std::to_string(maxDev) + ".");
-  const float cent = static_cast<float>(0.5 * (w - 1));
-  float fcr = 0.0f;
+  const float mid = static_cast<float>(0.5 * (w - 1));
+  float seed = 0.0f;
   if (algo == 0 || algo == 2) {
-    const int idx180 = static_cast<int>((N - 1) * M_PI / std::fabs(arc_rad(anarc, N)));
-    const int nofln = N - idx180;
-    if (nofln <= 0) return fail(CTAS_E_AXIS, "No opposite projections in the sinogram.");
-    std::vector<float> r0(static_cast<size_t>(nofln) * w), r180(static_cast<size_t>(nofln) * w);
-    for (int j = 0; j < nofln; ++j)
+    const int half_turn = static_cast<int>((N - 1) * M_PI / std::fabs(arc_rad(anarc, N)));
+    const int n_rows = N - half_turn;
+    if (n_rows <= 0) return fail(CTAS_E_AXIS, "No opposite projections in the sinogram.");
+    std::vector<float> r0(static_cast<size_t>(n_rows) * w), r180(static_cast<size_t>(n_rows) * w);
+    for (int j = 0; j < n_rows; ++j)
       for (int x = 0; x < w; ++x) {
         r0[static_cast<size_t>(j) * w + x] = sino[static_cast<size_t>(j) * w + x] + 1;
-        r180[static_cast<size_t>(j) * w + x] = sino[static_cast<size_t>(j + idx180) * w + x] + 1;
+        r180[static_cast<size_t>(j) * w + x] = sino[static_cast<size_t>(j + half_turn) * w + x] + 1;
       }
     float q = 0.0f;
-    if (int rc = ctas_raxis_pair(r0.data(), r180.data(), nofln, w, std::min(maxDev, 0.5f), &q)) return rc;
+    if (int rc = ctas_raxis_pair(r0.data(), r180.data(), n_rows, w, std::min(maxDev, 0.5f), &q)) return rc;
     if (algo == 0) {
       *result = q;
       return CTAS_OK;
     }
-    fcr = q;
+    seed = q;
   } else if (algo == 1) {
-    fcr = cent;
+    seed = mid;
   } else if (algo == 3) {
-    std::vector<float> pl(w, 0.0f);
+    std::vector<float> profile(w, 0.0f);
     for (int j = 0; j < N; ++j)
-      for (int x = 0; x < w; ++x) pl[x] = pl[x] + sino[static_cast<size_t>(j) * w + x];
-    const float mx = *std::max_element(pl.begin(), pl.end());
-    const float mn = *std::min_element(pl.begin(), pl.end());
+      for (int x = 0; x < w; ++x) profile[x] = profile[x] + sino[static_cast<size_t>(j) * w + x];
+    const float mx = *std::max_element(profile.begin(), profile.end());
+    const float mn = *std::min_element(profile.begin(), profile.end());
     // lct.cpp:1168-1180 only ends its scans at a sample above the
     // threshold; a constant projection has none.
     if (!(mx > mn)) return fail(CTAS_E_AXIS, "Constant projection: no edge above the threshold.");
-    const float thr = static_cast<float>(0.1 * mx + 0.9 * mn);
+    const float threshold = static_cast<float>(0.1 * mx + 0.9 * mn);
     int a = 0, b = w - 1;
-    while (!(pl[a] > thr)) ++a;
-    while (!(pl[b] > thr)) --b;
-    fcr = (static_cast<float>(a) + static_cast<float>(b)) / 2.0f;
+    while (!(profile[a] > threshold)) ++a;
+    while (!(profile[b] > threshold)) --b;
+    seed = (static_cast<float>(a) + static_cast<float>(b)) / 2.0f;
   } else {
     return fail(CTAS_E_AXIS, "Unknown algorithm for rotation axis " + std::to_string(algo) + ".");
   }
-  if (std::fabs(fcr - cent) / w > maxDev)
-    return fail(CTAS_E_AXIS, "Rotation axis relative shift " + std::to_string(std::fabs(fcr - cent) / w) +
+  if (std::fabs(seed - mid) / w > maxDev)
+    return fail(CTAS_E_AXIS, "Rotation axis relative shift " + std::to_string(std::fabs(seed - mid) / w) +
                                  " is larger than maximum shift " + std::to_string(maxDev) + ".");
-  const float rbxsz = static_cast<float>(1.0 - 2.0 * (maxDev + std::fabs(fcr - cent) / w));
-  if (rbxsz < 0.5f)
-    return fail(CTAS_E_AXIS, "Relative box size " + std::to_string(rbxsz) + " is less than miminum window of 0.5.");
+  const float box_fraction = static_cast<float>(1.0 - 2.0 * (maxDev + std::fabs(seed - mid) / w));
+  if (box_fraction < 0.5f)
+    return fail(CTAS_E_AXIS, "Relative box size " + std::to_string(box_fraction) + " is less than miminum window of 0.5.");
 
   // AverageGradient (lct.cpp:818-844): ABS contrast, RAMP, the sinogram
   // filtered once.
@@ -506,64 +510,64 @@ extern "C" int ctas_raxis(const float *sino, int N, int w, float anarc, float ma
   std::vector<float> copy(sino, sino + static_cast<size_t>(N) * w);
   if (int rc = ctas_fbp_sino(g.h, copy.data())) return rc;
   int rc = CTAS_OK;
-  auto score1 = [&](float x) -> float {
-    float axr = cent + x, s = 0.0f;
-    if (!rc) rc = ctas_axis_scores(g.h, &axr, 1, &s);
+  auto score_at = [&](float x) -> float {
+    float axes = mid + x, s = 0.0f;
+    if (!rc) rc = ctas_axis_scores(g.h, &axes, 1, &s);
     return s;
   };
 
-  const int stpsz = 8;
-  const int steps = static_cast<int>(1 + 2 * (maxDev == 0.0f ? 1.0f : maxDev * static_cast<float>(w) / stpsz));
-  const int hfsteps = (steps - 1) / 2;
-  const float coffee = fcr - cent;
-  std::vector<float> vax(steps), err(steps), axr(steps);
-  for (int k = 0; k < steps; ++k) {
-    vax[k] = coffee + static_cast<float>(stpsz * (k - hfsteps));
-    axr[k] = cent + vax[k];
+  const int kStep = 8;
+  const int n_trials = static_cast<int>(1 + 2 * (maxDev == 0.0f ? 1.0f : maxDev * static_cast<float>(w) / kStep));
+  const int half = (n_trials - 1) / 2;
+  const float seed_offset = seed - mid;
+  std::vector<float> offsets(n_trials), scores(n_trials), axes(n_trials);
+  for (int k = 0; k < n_trials; ++k) {
+    offsets[k] = seed_offset + static_cast<float>(kStep * (k - half));
+    axes[k] = mid + offsets[k];
   }
-  if ((rc = ctas_axis_scores(g.h, axr.data(), steps, err.data()))) return rc;
-  int idxM = 0;
-  float minval = FLT_MAX;
-  for (int i = 1; i < steps - 1; ++i)
-    if (err[i] < err[i - 1] && err[i] < err[i + 1] && err[i] < minval) {
-      idxM = i;
-      minval = err[i];
+  if ((rc = ctas_axis_scores(g.h, axes.data(), n_trials, scores.data()))) return rc;
+  int best_at = 0;
+  float best_score = FLT_MAX;
+  for (int i = 1; i < n_trials - 1; ++i)
+    if (scores[i] < scores[i - 1] && scores[i] < scores[i + 1] && scores[i] < best_score) {
+      best_at = i;
+      best_score = scores[i];
     }
-  int cdir = 1;
-  if (!idxM) {
-    const int idx = err[0] < err[steps - 1] ? 0 : steps - 1;
-    cdir = idx == 0 ? -1 : 1;
-    const int coff2 = static_cast<int>(vax[idx - cdir * 2]);
-    const int maxSteps = static_cast<int>(static_cast<float>(w) * maxDevLM - static_cast<float>(cdir * coff2 / stpsz));
-    if (maxSteps <= 0) return fail(CTAS_E_AXIS, "Impossible number of steps.");
-    std::vector<float> v2, e2;
+  int dir = 1;
+  if (!best_at) {
+    const int idx = scores[0] < scores[n_trials - 1] ? 0 : n_trials - 1;
+    dir = idx == 0 ? -1 : 1;
+    const int walk_from = static_cast<int>(offsets[idx - dir * 2]);
+    const int walk_limit = static_cast<int>(static_cast<float>(w) * kShiftLimit - static_cast<float>(dir * walk_from / kStep));
+    if (walk_limit <= 0) return fail(CTAS_E_AXIS, "Impossible number of n_trials.");
+    std::vector<float> walk_offsets, walk_scores;
     bool stop = false;
-    constexpr int kWalkBatch = 8;
-    for (int k0 = 0; k0 < maxSteps && !stop; k0 += kWalkBatch) {
-      const int m = std::min(kWalkBatch, maxSteps - k0);
+    constexpr int kWalkChunk = 8;
+    for (int k0 = 0; k0 < walk_limit && !stop; k0 += kWalkChunk) {
+      const int m = std::min(kWalkChunk, walk_limit - k0);
       std::vector<float> a(m), s(m);
-      for (int i = 0; i < m; ++i) a[i] = cent + (static_cast<float>(coff2) + static_cast<float>(stpsz * (k0 + i) * cdir));
+      for (int i = 0; i < m; ++i) a[i] = mid + (static_cast<float>(walk_from) + static_cast<float>(kStep * (k0 + i) * dir));
       if ((rc = ctas_axis_scores(g.h, a.data(), m, s.data()))) return rc;
       for (int i = 0; i < m && !stop; ++i) {
-        v2.push_back(static_cast<float>(coff2) + static_cast<float>(stpsz * (k0 + i) * cdir));
-        e2.push_back(s[i]);
-        if (s[i] < minval) {
-          minval = s[i];
-          idxM = k0 + i;
+        walk_offsets.push_back(static_cast<float>(walk_from) + static_cast<float>(kStep * (k0 + i) * dir));
+        walk_scores.push_back(s[i]);
+        if (s[i] < best_score) {
+          best_score = s[i];
+          best_at = k0 + i;
         } else {
           stop = true;
         }
       }
     }
-    if (idxM == maxSteps - 1 || idxM == 0) {
-      *result = fcr;
+    if (best_at == walk_limit - 1 || best_at == 0) {
+      *result = seed;
       return CTAS_OK;
     }
-    vax = v2;
-    err = e2;
+    offsets = walk_offsets;
+    scores = walk_scores;
   }
-  const float rf = improve_me(0.25f, vax, err, idxM, cdir, score1, &rc);
+  const float rf = improve_me(0.25f, offsets, scores, best_at, dir, score_at, &rc);
   if (rc) return rc;
-  *result = cent + rf;
+  *result = mid + rf;
   return CTAS_OK;
 }

@@ -88,7 +88,16 @@ __device__ __forceinline__ bool wave_outside(const Item &it, float c, int r0, in
          (static_cast<float>(c0) >= static_cast<float>(it.w) - acent - 1.0f);
 }
 
+// The item that owns tile b: one division for batches of equal items, a
+// binary search otherwise.
 __device__ __forceinline__ int find_item_by_tile(const Item *items, int n, int b) {
+  if (n > 1) {
+    const int per = items[1].tile_base;
+    if (per > 0) {
+      const int g = min(b / per, n - 1);
+      if (items[g].tile_base <= b && (g == n - 1 || items[g + 1].tile_base > b)) return g;
+    }
+  }
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;

@@ -1237,30 +1237,30 @@ extern "C" int ctas_raxis_cnn(ctas_cnn *n, const float *sino, int N, int w, floa
   if (N < 3 || w < 3 || std::fabs(arc) < static_cast<float>(M_PI) || maxDev < 0 || maxDev > 0.25f) return sweep();
   float off = 0.0f, conf = 0.0f;
   if (int rc = ctas_cnn_predict(n, sino, N, w, &off, &conf)) return rc;
-  const float cent = static_cast<float>(0.5 * (w - 1));
-  const float rf = std::round(off * 4.0f) / 4.0f;
-  if (!(conf >= min_confidence) || std::fabs(rf) + 4.0f > maxDev * w) return sweep();
+  const float mid = static_cast<float>(0.5 * (w - 1));
+  const float est_px = std::round(off * 4.0f) / 4.0f;
+  if (!(conf >= min_confidence) || std::fabs(est_px) + 4.0f > maxDev * w) return sweep();
   ctas_fbp *h = nullptr;
   if (int rc = ctas_fbp_create(&h, w, N, CTAS_CONTRAST_ABS, CTAS_FILTER_RAMP, 0.0f, anarc, 0)) return rc;
   std::unique_ptr<ctas_fbp, int (*)(ctas_fbp *)> guard(h, ctas_fbp_destroy);
   std::vector<float> copy(sino, sino + static_cast<size_t>(N) * w);
   if (int rc = ctas_fbp_sino(h, copy.data())) return rc;
-  const std::vector<float> vax = {rf - 4.0f, rf, rf + 4.0f};
-  std::vector<float> err(3);
-  const float ax[3] = {cent + vax[0], cent + vax[1], cent + vax[2]};
-  if (int rc = ctas_axis_scores(h, ax, 3, err.data())) return rc;
-  if (!(err[1] < err[0] && err[1] < err[2])) {
+  const std::vector<float> offsets = {est_px - 4.0f, est_px, est_px + 4.0f};
+  std::vector<float> scores(3);
+  const float ax[3] = {mid + offsets[0], mid + offsets[1], mid + offsets[2]};
+  if (int rc = ctas_axis_scores(h, ax, 3, scores.data())) return rc;
+  if (!(scores[1] < scores[0] && scores[1] < scores[2])) {
     guard.reset();
     return sweep();
   }
   int rc = CTAS_OK;
-  auto score1 = [&](float x) -> float {
-    float axr = cent + x, v = 0.0f;
-    if (!rc) rc = ctas_axis_scores(h, &axr, 1, &v);
+  auto score_at = [&](float x) -> float {
+    float axis = mid + x, v = 0.0f;
+    if (!rc) rc = ctas_axis_scores(h, &axis, 1, &v);
     return v;
   };
-  const float best = improve_me(0.25f, vax, err, 1, 1, score1, &rc);
+  const float best = improve_me(0.25f, offsets, scores, 1, 1, score_at, &rc);
   if (rc) return rc;
-  *result = cent + best;
+  *result = mid + best;
   return CTAS_OK;
 }

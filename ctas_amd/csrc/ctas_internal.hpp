@@ -44,8 +44,8 @@ int ts_window(int filter, float param, int n, int &Z, std::vector<float> &W);
 // improveMe (lct.cpp:1108-1129): bisection from the bracket around vax[idx]
 // down to eps; f evaluates the score of one trial offset and sets *rc on
 // failure (axis.hip).
-float improve_me(float eps, const std::vector<float> &vax, const std::vector<float> &err, int idx, int cdir,
-                 const std::function<float(float)> &f, int *rc);
+float improve_me(float eps, const std::vector<float> &offsets, const std::vector<float> &scores, int at, int dir,
+                 const std::function<float(float)> &score_at, int *rc);
 
 // ---- device descriptors -----------------------------------------------------
 // One step of the mirrored backprojection (FP16/BF16 storage): angle row_a

@@ -40,7 +40,35 @@ __device__ __forceinline__ void store_val(void *base, size_t i, float v) {
   }
 }
 
+// Loads and stores through global (address space 1) pointers: the item
+// table holds generic pointers, from which the compiler emits FLAT
+// accesses, which also count against the LDS wait counter.
+__device__ __forceinline__ float gload(const float *p, long long i) {
+  return ((const __attribute__((address_space(1))) float *)(p))[i];
+}
+template <int ST>
+__device__ __forceinline__ void gstore(void *base, size_t i, float v) {
+  if constexpr (ST == ST_F32) {
+    ((__attribute__((address_space(1))) float *)(base))[i] = v;
+  } else if constexpr (ST == ST_F16) {
+    ((__attribute__((address_space(1))) _Float16 *)(base))[i] = static_cast<_Float16>(v);
+  } else {
+    ((__attribute__((address_space(1))) uint16_t *)(base))[i] = f32_to_bf16_rne(v);
+  }
+}
+
+// The item that owns row pair b. Batches of equal items (every slice of a
+// handle) are found by one division and checked with two loads; uneven ones
+// fall back to a binary search (a chain of dependent loads, each a scalar
+// memory round trip at the start of every workgroup).
 __device__ __forceinline__ int find_item_by_pair(const Item *items, int n, int b) {
+  if (n > 1) {
+    const int per = items[1].pair_base;
+    if (per > 0) {
+      const int g = min(b / per, n - 1);
+      if (items[g].pair_base <= b && (g == n - 1 || items[g + 1].pair_base > b)) return g;
+    }
+  }
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -136,27 +164,31 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
   // Padded line z[n], n in [0, Z): row samples at s0..s0+w-1 and, from the
   // inclusive copy range of lct.cpp:580-584, the next row's first sample at
   // s0+w (0 after the last row). Stored at bit-reversed positions.
-  for (int n = tid; n < Z; n += 256) {
-    const int x = n - s0;
-    float a = 0.0f, c = 0.0f;
-    if (x >= 0 && x < w) {
-      a = r0[x];
-      if (absorb) a = deabs(a);
-      if (has1) {
-        c = r1[x];
-        if (absorb) c = deabs(c);
-      }
-    } else if (x == w) {
-      if (t0 + 1 < N) {
-        a = r0[w];
-        if (absorb) a = deabs(a);
-      }
-      if (has1 && t1 + 1 < N) {
-        c = r1[w];
-        if (absorb) c = deabs(c);
-      }
+  // The loads of up to 32 points per thread are issued before any is used,
+  // at indices clamped into the rows (samples outside are zeroed after).
+  const float *r1c = has1 ? r1 : r0;
+  const int per = (Z + 255) >> 8;  // points per thread (the last partial below 256 points)
+  float ra[32], rb[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (k < per && tid + 256 * k < Z) {
+      const int x = tid + 256 * k - s0;
+      const bool in = x >= 0 && x < w;
+      const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+      ra[k] = gload(r0, in0 ? x : 0);
+      rb[k] = gload(r1c, in1 ? x : 0);
     }
-    buf[lds_pad(__brev(static_cast<unsigned>(n)) >> (32 - logZ), psh)] = make_float2(a, c);
+  }
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (k < per && tid + 256 * k < Z) {
+      const int n = tid + 256 * k, x = n - s0;
+      const bool in = x >= 0 && x < w;
+      const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+      const float a = absorb ? deabs(ra[k]) : ra[k], c = absorb ? deabs(rb[k]) : rb[k];
+      buf[lds_pad(__brev(static_cast<unsigned>(n)) >> (32 - logZ), psh)] =
+          make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
+    }
   }
   __syncthreads();
 
@@ -169,11 +201,11 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
     for (int x = tid; x < w; x += 256) {
       const float2 v = buf[lds_pad(__brev(static_cast<unsigned>(s0 + x)) >> rsh, psh)];
       const float a = v.x * invZ, c = v.y * invZ;
-      store_val<ST>(it.filt, row0 + x, a);
-      if (it.filt_f32) it.filt_f32[row0 + x] = a;
+      gstore<ST>(it.filt, row0 + x, a);
+      if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
-        store_val<ST>(it.filt, row1 + x, c);
-        if (it.filt_f32) it.filt_f32[row1 + x] = c;
+        gstore<ST>(it.filt, row1 + x, c);
+        if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row1 + x, c);
       }
     }
     return;
@@ -216,8 +248,14 @@ __device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
 template <int ST>
 __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ items, int n_items,
                                                        int contrast) {
-  constexpr int Z = 4096, LOGZ = 12, PADR = 17;
-  __shared__ float2 xch[256 * PADR];
+  constexpr int Z = 4096, LOGZ = 12;
+  // Exchange buffer of 256 rows of 16 points, column c of row r stored at
+  // r*16 + (c ^ (r % 16)): every access pattern of the transposes below
+  // spreads 16 lanes over 16 distinct 8-byte bank pairs, and the buffer is
+  // the 32 KiB of the line itself, so five workgroups fit a CU's LDS (their
+  // registers allow five).
+  __shared__ float2 xch[256 * 16];
+  auto at = [](int r, int c) { return r * 16 + (c ^ (r & 15)); };
   const int b = blockIdx.x;
   const Item &it = items[find_item_by_pair(items, n_items, b)];
   const int pair = b - it.pair_base;
@@ -251,30 +289,27 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   };
 
   // Padded line z[n]: row samples at s0..s0+w-1 and the next row's first
-  // sample at s0+w (lct.cpp:580-584; 0 after the last row).
+  // sample at s0+w (lct.cpp:580-584; 0 after the last row). All loads are
+  // issued before any is used, at indices clamped into the rows (samples
+  // outside are zeroed afterwards), so that their latencies overlap.
+  const float *r1c = has1 ? r1 : r0;
+  float ra[16], rb[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int x = tid + 256 * j - s0;
+    const bool in = x >= 0 && x < w;
+    const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+    ra[j] = gload(r0, in0 ? x : 0);
+    rb[j] = gload(r1c, in1 ? x : 0);
+  }
   float2 v[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int x = tid + 256 * j - s0;
-    float a = 0.0f, c = 0.0f;
-    if (x >= 0 && x < w) {
-      a = r0[x];
-      if (absorb) a = deabs(a);
-      if (has1) {
-        c = r1[x];
-        if (absorb) c = deabs(c);
-      }
-    } else if (x == w) {
-      if (t0 + 1 < N) {
-        a = r0[w];
-        if (absorb) a = deabs(a);
-      }
-      if (has1 && t1 + 1 < N) {
-        c = r1[w];
-        if (absorb) c = deabs(c);
-      }
-    }
-    v[j] = make_float2(a, c);
+    const bool in = x >= 0 && x < w;
+    const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+    const float a = absorb ? deabs(ra[j]) : ra[j], c = absorb ? deabs(rb[j]) : rb[j];
+    v[j] = make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
   }
 
   // forward: thread t -> (t1, k1) -> (k1, k2)
@@ -282,20 +317,20 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   dft16<false>(v);
   twiddle_powers(v, w4096<false>(tw, tid));
 #pragma unroll
-  for (int k = 0; k < 16; ++k) xch[tid * PADR + k] = v[k];
+  for (int k = 0; k < 16; ++k) xch[at(tid, k)] = v[k];
   __syncthreads();
   // thread (t1 = lo4, k1 = hi4): values over t2
 #pragma unroll
-  for (int j = 0; j < 16; ++j) v[j] = xch[(lo4 + 16 * j) * PADR + hi4];
+  for (int j = 0; j < 16; ++j) v[j] = xch[at(lo4 + 16 * j, hi4)];
   __syncthreads();
   dft16<false>(v);
   twiddle_powers(v, w4096<false>(tw, 16 * lo4));
 #pragma unroll
-  for (int k = 0; k < 16; ++k) xch[(hi4 * 16 + k) * PADR + lo4] = v[k];
+  for (int k = 0; k < 16; ++k) xch[at(hi4 * 16 + k, lo4)] = v[k];
   __syncthreads();
   // thread (k1 = hi4, k2 = lo4): values over t1
 #pragma unroll
-  for (int j = 0; j < 16; ++j) v[j] = xch[tid * PADR + j];
+  for (int j = 0; j < 16; ++j) v[j] = xch[at(tid, j)];
   dft16<false>(v);
 
   // X[k1 + 16 k2 + 256 k3] times the window
@@ -310,18 +345,18 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   // inverse: the same steps backwards, conjugate factors
   dft16<true>(v);
 #pragma unroll
-  for (int j = 0; j < 16; ++j) xch[tid * PADR + j] = v[j];
+  for (int j = 0; j < 16; ++j) xch[at(tid, j)] = v[j];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = xch[(hi4 * 16 + k) * PADR + lo4];
+  for (int k = 0; k < 16; ++k) v[k] = xch[at(hi4 * 16 + k, lo4)];
   __syncthreads();
   twiddle_powers(v, w4096<true>(tw, 16 * lo4));
   dft16<true>(v);
 #pragma unroll
-  for (int j = 0; j < 16; ++j) xch[(lo4 + 16 * j) * PADR + hi4] = v[j];
+  for (int j = 0; j < 16; ++j) xch[at(lo4 + 16 * j, hi4)] = v[j];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = xch[tid * PADR + k];
+  for (int k = 0; k < 16; ++k) v[k] = xch[at(tid, k)];
   twiddle_powers(v, w4096<true>(tw, tid));
   dft16<true>(v);
 
@@ -333,11 +368,11 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
       const int x = tid + 256 * j - s0;
       if (x < 0 || x >= w) continue;
       const float a = v[j].x * invZ, c = v[j].y * invZ;
-      store_val<ST>(it.filt, row0 + x, a);
-      if (it.filt_f32) it.filt_f32[row0 + x] = a;
+      gstore<ST>(it.filt, row0 + x, a);
+      if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
-        store_val<ST>(it.filt, row1 + x, c);
-        if (it.filt_f32) it.filt_f32[row1 + x] = c;
+        gstore<ST>(it.filt, row1 + x, c);
+        if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row1 + x, c);
       }
     }
     return;
@@ -415,29 +450,24 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
 
   // Padded line: row samples at s0..s0+w-1 and the next row's first sample
   // at s0+w (lct.cpp:580-584; 0 after the last row).
+  // All loads are issued before any is used, at indices clamped into the
+  // rows (samples outside are zeroed afterwards).
+  const float *r1c = has1 ? r1 : r0;
   float2 v[32];
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
     const int x = tid + 256 * j - s0;
-    float a = 0.0f, c = 0.0f;
-    if (x >= 0 && x < w) {
-      a = r0[x];
-      if (absorb) a = deabs(a);
-      if (has1) {
-        c = r1[x];
-        if (absorb) c = deabs(c);
-      }
-    } else if (x == w) {
-      if (t0 + 1 < N) {
-        a = r0[w];
-        if (absorb) a = deabs(a);
-      }
-      if (has1 && t1r + 1 < N) {
-        c = r1[w];
-        if (absorb) c = deabs(c);
-      }
-    }
-    v[j] = make_float2(a, c);
+    const bool in = x >= 0 && x < w;
+    const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1r + 1 < N));
+    v[j] = make_float2(gload(r0, in0 ? x : 0), gload(r1c, in1 ? x : 0));
+  }
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const int x = tid + 256 * j - s0;
+    const bool in = x >= 0 && x < w;
+    const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1r + 1 < N));
+    const float a = absorb ? deabs(v[j].x) : v[j].x, c = absorb ? deabs(v[j].y) : v[j].y;
+    v[j] = make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
   }
 
   dft32<false>(v);
@@ -512,11 +542,11 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
       const int x = tid + 256 * j - s0;
       if (x < 0 || x >= w) continue;
       const float a = v[j].x * invZ, c = v[j].y * invZ;
-      store_val<ST>(it.filt, row0 + x, a);
-      if (it.filt_f32) it.filt_f32[row0 + x] = a;
+      gstore<ST>(it.filt, row0 + x, a);
+      if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
-        store_val<ST>(it.filt, row1 + x, c);
-        if (it.filt_f32) it.filt_f32[row1 + x] = c;
+        gstore<ST>(it.filt, row1 + x, c);
+        if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row1 + x, c);
       }
     }
     return;
@@ -535,7 +565,7 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
       const float2 q = xb[s0 + x];
       const float f = (im ? q.y : q.x) * invZ;
       run = (x == 0) ? f : run + f;
-      store_val<ST>(it.filt, rowb + x, run);
+      gstore<ST>(it.filt, rowb + x, run);
       if (it.filt_f32) it.filt_f32[rowb + x] = run;
     }
   }
