@@ -914,14 +914,27 @@ __device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigne
         }
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
-          // the pixel's two samples (lo WA, hi WB) and the mirror pixel's
-          // (lo WB, hi WA), merged by bit-field inserts (the asm keeps them
-          // 32-bit merges) and each added by one dot2
-          uint32_t X = (WA[q] & 0xFFFFu) | (WB[q] & 0xFFFF0000u);
-          uint32_t Y = (WB[q] & 0xFFFFu) | (WA[q] & 0xFFFF0000u);
-          asm("" : "+v"(X), "+v"(Y));
-          acc[r][q] = add_pair<ST>(acc[r][q], X);
-          accm[r][q] = add_pair<ST>(accm[r][q], Y);
+          if constexpr (ST == ST_F16) {
+            // FP16: one packed add forms both pixels' pair sums, (lo WA +
+            // hi WB, hi WA + lo WB) (the swap is the add's operand select),
+            // each rounded once to FP16 and added by a dot2 that picks its
+            // half: three instructions for the four updates
+            const half2v pa = __builtin_bit_cast(half2v, WA[q]), pb = __builtin_bit_cast(half2v, WB[q]);
+            const half2v sum = pa + __builtin_shufflevector(pb, pb, 1, 0);
+            const half2v lo = {static_cast<_Float16>(1.0f), static_cast<_Float16>(0.0f)};
+            const half2v hi = {static_cast<_Float16>(0.0f), static_cast<_Float16>(1.0f)};
+            acc[r][q] = __builtin_amdgcn_fdot2(sum, lo, acc[r][q], false);
+            accm[r][q] = __builtin_amdgcn_fdot2(sum, hi, accm[r][q], false);
+          } else {
+            // the pixel's two samples (lo WA, hi WB) and the mirror pixel's
+            // (lo WB, hi WA), merged by bit-field inserts (the asm keeps them
+            // 32-bit merges) and each added by one dot2
+            uint32_t X = (WA[q] & 0xFFFFu) | (WB[q] & 0xFFFF0000u);
+            uint32_t Y = (WB[q] & 0xFFFFu) | (WA[q] & 0xFFFF0000u);
+            asm("" : "+v"(X), "+v"(Y));
+            acc[r][q] = add_pair<ST>(acc[r][q], X);
+            accm[r][q] = add_pair<ST>(accm[r][q], Y);
+          }
         }
       }
     }
