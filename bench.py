@@ -151,7 +151,27 @@ def pmc_measure(args, kernel_name, kind, ac, store, interp, updates, bp_ms, peak
                      "frac": round(achieved / peak, 4),
                      "note": "VALU wave instructions of the launch (x64 lanes) per full-grid voxel-update; "
                              "peak = measured v_add_f32 issue rate"},
-            "lds_instr": bp_a.get("SQ_INSTS_LDS"), "waves": bp_b.get("SQ_WAVES")}
+            "lds_issue": lds_issue(bp_a.get("SQ_INSTS_LDS"), bp_ms, clk_cu, updates, peaks),
+            "waves": bp_b.get("SQ_WAVES")}
+
+
+def binding_of(pmc):
+    """The issue roof a launch is nearer to: VALU or LDS instructions."""
+    v = (pmc.get("valu") or {}).get("frac", 0.0)
+    l_ = (pmc.get("lds_issue") or {}).get("frac", 0.0)
+    return "valu_issue" if v >= l_ else "lds_issue"
+
+
+def lds_issue(insts, bp_ms, clk_cu, updates, peaks):
+    """LDS wave instructions of the launch against the measured ds_read_b32
+    issue rate (the kernels' sample reads are b32)."""
+    if not insts:
+        return None
+    achieved = insts / (bp_ms * 1e-3) / clk_cu
+    peak = peaks["lds_instr_per_clk_cu"]["b32"]
+    return {"sq_insts_lds": insts, "instr_per_update": round(insts * 64 / updates, 3),
+            "achieved": round(achieved, 3), "peak_b32": peak, "unit": "wave instructions/clk/CU",
+            "frac": round(achieved / peak, 4)}
 
 
 def pmc_child(args) -> int:
@@ -529,24 +549,36 @@ def main() -> int:
                      "kernel": "bp_grid_kernel", "kernel_ms": round(bp_ms, 3),
                      "bytes_per_update": 4,
                      "peak_b32": lds_peak32, "frac_b32": round(achieved / lds_peak32, 4),
-                     "binding": "valu_issue",
+                     "binding": binding_of(pmc) if pmc else "valu_issue",
                      "valu": (pmc["valu"] if pmc and "valu" in pmc else None),
+                     "lds_issue": (pmc.get("lds_issue") if pmc else None),
                      "note": "LDS sample bytes per full-grid voxel-update (one ds_read_b32 each) over the "
                              "kernel's HIP-event time; peak = ds_read_b128 bytes/clk/CU x CUs x clock "
-                             "measured in this run (peak_b32: the same for ds_read_b32); the kernel is "
-                             "bound by VALU issue (valu.frac)"},
+                             "measured in this run (peak_b32: the same for ds_read_b32); binding: the "
+                             "instruction-issue roof nearest to full (valu.frac, lds_issue.frac)"},
         "stages_ms": {"prepare": round(prep_ms, 3), "backproject": round(bp_ms, 3)},
         "peaks": peaks,
     }
     if pmc:
-        result["pmc"] = {k: v for k, v in pmc.items() if k not in ("valu",)}
+        result["pmc"] = {k: v for k, v in pmc.items() if k not in ("valu", "lds_issue")}
     if not args.no_variants:
         var = {}
         for name, store in (("fp16", ca.STORE_FP16), ("bf16", ca.STORE_BF16)):
             print(f"[bench] {name} CTAS", file=sys.stderr, flush=True)
-            _, vms, vbp, _ = run_variant(store, ca.INTERP_CTAS, max(2, args.steps // 2), 1)
+            vrec, vms, vbp, _ = run_variant(store, ca.INTERP_CTAS, max(2, args.steps // 2), 1)
             var[name] = {"value": round(updates / (vms / 1e3) / 1e9, 2), "ms_per_step": round(vms, 3),
                          "bp_kernel_ms": round(vbp, 3), "speedup_vs_fp32": round(ms / vms, 3)}
+            if name == "fp16" and not args.no_pmc:
+                # the mirrored kernel: 2 B of LDS sample words per voxel-update
+                vk, vac = vrec.tiles()
+                vp = pmc_measure(args, "bp_mirror_kernel", vk, vac, store, ca.INTERP_CTAS, updates, vbp, peaks)
+                a16 = updates * 2.0 / (vbp / 1e3) / 1e9
+                var[name]["roofline"] = {"bound": "lds", "achieved": round(a16, 1), "peak": lds_peak, "unit": "GB/s",
+                                         "frac": round(a16 / lds_peak, 4),
+                                         "traffic": round(vp["traffic"]) if "traffic" in vp else None,
+                                         "binding": binding_of(vp), "valu": vp.get("valu"),
+                                         "lds_issue": vp.get("lds_issue"),
+                                         "kernel": "bp_mirror_kernel", "bytes_per_update": 2}
         print("[bench] fp32 LINEAR", file=sys.stderr, flush=True)
         _, lms, lbp, _ = run_variant(ca.STORE_FP32, ca.INTERP_LINEAR, max(2, args.steps // 2), 1)
         var["fp32_linear"] = {"value": round(updates / (lms / 1e3) / 1e9, 2), "ms_per_step": round(lms, 3),

@@ -245,28 +245,47 @@ __device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
   return INV ? make_float2(t.x, -t.y) : t;
 }
 
+// Raw samples of one row pair of a 4096-point line, circularly shifted
+// (lct.cpp:580-584 puts the row at s0 = (Z - w)/2, the next row's first
+// sample at s0 + w, zeros elsewhere; here the row is at n = 0..w-1 and that
+// sample at n = w; a circular shift of the line shifts the filtered line
+// alike, the window multiplying the spectrum, so the row is read back from
+// n = 0..w-1). With w <= 2048 thread t holds the points t + 256 j, j < 8, of
+// both rows; the upper half is zero but for n = 2048 when w = 2048 (ea, ec),
+// added into thread 0's first DFT. Indices are clamped into the rows; the
+// samples outside the line are zeroed when the pair is filtered.
+struct Raw16 {
+  float a[8], c[8], ea, ec;
+};
+
+__device__ __forceinline__ void load16(const Item &it, int pair, int tid, Raw16 &r) {
+  const int w = it.w, N = it.N, t0 = 2 * pair, t1 = t0 + 1;
+  const bool has1 = t1 < N;
+  const float *r0 = it.sino + static_cast<size_t>(t0) * w;
+  const float *r1c = has1 ? r0 + w : r0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int x = tid + 256 * j;
+    const bool in = x < w;
+    const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+    r.a[j] = gload(r0, in0 ? x : 0);
+    r.c[j] = gload(r1c, in1 ? x : 0);
+  }
+  const bool e0 = w == 2048 && t0 + 1 < N, e1 = w == 2048 && has1 && t1 + 1 < N;
+  r.ea = gload(r0, e0 ? w : 0);
+  r.ec = gload(r1c, e1 ? w : 0);
+}
+
+// One row pair: deAbs, the 4096-point transform, window, inverse, stores.
 template <int ST>
-__global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ items, int n_items,
-                                                       int contrast) {
-  constexpr int Z = 4096, LOGZ = 12;
-  // Exchange buffer of 256 rows of 16 points, column c of row r stored at
-  // r*16 + (c ^ (r % 16)): every access pattern of the transposes below
-  // spreads 16 lanes over 16 distinct 8-byte bank pairs, and the buffer is
-  // the 32 KiB of the line itself, so five workgroups fit a CU's LDS (their
-  // registers allow five).
-  __shared__ float2 xch[256 * 16];
-  auto at = [](int r, int c) { return r * 16 + (c ^ (r & 15)); };
-  const int b = blockIdx.x;
-  const Item &it = items[find_item_by_pair(items, n_items, b)];
-  const int pair = b - it.pair_base;
-  const int w = it.w, N = it.N;
-  const int s0 = (Z - w) >> 1;
+__device__ __forceinline__ void filter16_pair(const Item &it, int pair, const Raw16 &r, float2 *xch, int contrast,
+                                              int tid) {
+  constexpr int Z = 4096;
+  auto at = [](int row, int col) { return row * 16 + (col ^ (row & 15)); };
+  const int w = it.w, N = it.N;  // w <= 2048: zidth(w) = 4096
   const int t0 = 2 * pair, t1 = t0 + 1;
   const bool has1 = t1 < N;
-  const float *__restrict__ r0 = it.sino + static_cast<size_t>(t0) * w;
-  const float *__restrict__ r1 = r0 + w;
   const float2 *__restrict__ tw = it.tw;
-  const int tid = threadIdx.x;
 
   // deAbs (physical.world.cpp:88-100), as in filter_kernel.
   const bool absorb = (contrast == CTAS_CONTRAST_ABS);
@@ -288,33 +307,30 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
     return -__logf(v);
   };
 
-  // Padded line z[n]: row samples at s0..s0+w-1 and the next row's first
-  // sample at s0+w (lct.cpp:580-584; 0 after the last row). All loads are
-  // issued before any is used, at indices clamped into the rows (samples
-  // outside are zeroed afterwards), so that their latencies overlap.
-  const float *r1c = has1 ? r1 : r0;
-  float ra[16], rb[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int x = tid + 256 * j - s0;
-    const bool in = x >= 0 && x < w;
-    const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
-    ra[j] = gload(r0, in0 ? x : 0);
-    rb[j] = gload(r1c, in1 ? x : 0);
-  }
   float2 v[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int x = tid + 256 * j - s0;
-    const bool in = x >= 0 && x < w;
+  for (int j = 0; j < 8; ++j) {
+    const int x = tid + 256 * j;
+    const bool in = x < w;
     const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
-    const float a = absorb ? deabs(ra[j]) : ra[j], c = absorb ? deabs(rb[j]) : rb[j];
+    const float a = absorb ? deabs(r.a[j]) : r.a[j], c = absorb ? deabs(r.c[j]) : r.c[j];
     v[j] = make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
   }
 
   // forward: thread t -> (t1, k1) -> (k1, k2)
   const int lo4 = tid & 15, hi4 = tid >> 4;
-  dft16<false>(v);
+  dft16_lo8<false>(v);
+  if (w == 2048 && tid == 0) {
+    // z[2048] (point j = 8 of thread 0): factor exp(-2 pi i 8 k / 16) = (-1)^k
+    const bool e0 = t0 + 1 < N, e1 = has1 && t1 + 1 < N;
+    const float a = e0 ? (absorb ? deabs(r.ea) : r.ea) : 0.0f;
+    const float c = e1 ? (absorb ? deabs(r.ec) : r.ec) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      v[k].x += (k & 1) ? -a : a;
+      v[k].y += (k & 1) ? -c : c;
+    }
+  }
   twiddle_powers(v, w4096<false>(tw, tid));
 #pragma unroll
   for (int k = 0; k < 16; ++k) xch[at(tid, k)] = v[k];
@@ -333,11 +349,13 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   for (int j = 0; j < 16; ++j) v[j] = xch[at(tid, j)];
   dft16<false>(v);
 
-  // X[k1 + 16 k2 + 256 k3] times the window
+  // X[k1 + 16 k2 + 256 k3] times the window and 1/Z (a power of two: the
+  // scaling is exact wherever it is applied)
   const int kbase = hi4 + 16 * lo4;
+  const float invZ = 1.0f / static_cast<float>(Z);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const float wk = it.W[kbase + 256 * k];
+    const float wk = it.W[kbase + 256 * k] * invZ;
     v[k].x *= wk;
     v[k].y *= wk;
   }
@@ -358,16 +376,15 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = xch[at(tid, k)];
   twiddle_powers(v, w4096<true>(tw, tid));
-  dft16<true>(v);
+  dft16_out8<true>(v);  // the points t + 256 j, j < 8: n < 2048
 
-  const float invZ = 1.0f / static_cast<float>(Z);
   const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
   if (contrast != CTAS_CONTRAST_REF) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int x = tid + 256 * j - s0;
-      if (x < 0 || x >= w) continue;
-      const float a = v[j].x * invZ, c = v[j].y * invZ;
+    for (int j = 0; j < 8; ++j) {
+      const int x = tid + 256 * j;
+      if (x >= w) continue;
+      const float a = v[j].x, c = v[j].y;
       gstore<ST>(it.filt, row0 + x, a);
       if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
@@ -381,21 +398,38 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   // prefix sum along each row, one lane per row, in sequential order.
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 16; ++j) xch[tid + 256 * j] = v[j];
+  for (int j = 0; j < 8; ++j) xch[tid + 256 * j] = v[j];
   __syncthreads();
   if (tid == 0 || (tid == 64 && has1)) {
     const bool im = tid == 64;
     const size_t rowb = im ? row1 : row0;
     float run = 0.0f;
     for (int x = 0; x < w; ++x) {
-      const float2 q = xch[s0 + x];
-      const float f = (im ? q.y : q.x) * invZ;
+      const float2 q = xch[x];
+      const float f = im ? q.y : q.x;
       run = (x == 0) ? f : run + f;
       store_val<ST>(it.filt, rowb + x, run);
       if (it.filt_f32) it.filt_f32[rowb + x] = run;
     }
   }
-  (void)LOGZ;
+}
+
+// The 4096-point filter: one row pair per workgroup, its loads issued
+// before any is used.
+template <int ST>
+__global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ items, int n_items,
+                                                       int contrast) {
+  // Exchange buffer of 256 rows of 16 points, column c of row r stored at
+  // r*16 + (c ^ (r % 16)): every access pattern of the transposes spreads
+  // 16 lanes over 16 distinct 8-byte bank pairs; it is the 32 KiB of the
+  // line itself.
+  __shared__ float2 xch[256 * 16];
+  const int tid = threadIdx.x;
+  const Item &it = items[find_item_by_pair(items, n_items, blockIdx.x)];
+  const int pair = blockIdx.x - it.pair_base;
+  Raw16 r;
+  load16(it, pair, tid, r);
+  filter16_pair<ST>(it, pair, r, xch, contrast, tid);
 }
 
 // ---- Z = 8192: 32 x 16 x 16 in registers --------------------------------------
@@ -419,8 +453,7 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
   const int b = blockIdx.x;
   const Item &it = items[find_item_by_pair(items, n_items, b)];
   const int pair = b - it.pair_base;
-  const int w = it.w, N = it.N;
-  const int s0 = (Z - w) >> 1;
+  const int w = it.w, N = it.N;  // w <= 4096: zidth(w) = 8192
   const int t0 = 2 * pair, t1r = t0 + 1;
   const bool has1 = t1r < N;
   const float *__restrict__ r0 = it.sino + static_cast<size_t>(t0) * w;
@@ -448,29 +481,42 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
     return -__logf(v);
   };
 
-  // Padded line: row samples at s0..s0+w-1 and the next row's first sample
-  // at s0+w (lct.cpp:580-584; 0 after the last row).
-  // All loads are issued before any is used, at indices clamped into the
-  // rows (samples outside are zeroed afterwards).
+  // Padded line, circularly shifted as in filter16_kernel: the row at
+  // n = 0..w-1 and the next row's first sample at n = w (lct.cpp:580-584;
+  // 0 after the last row). With w <= 4096 thread t holds the points
+  // t + 256 j, j < 16, and n = 4096 (w = 4096) is added into thread 0's
+  // first DFT. All loads are issued before any is used, at indices clamped
+  // into the rows (samples outside are zeroed afterwards).
   const float *r1c = has1 ? r1 : r0;
   float2 v[32];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const int x = tid + 256 * j - s0;
-    const bool in = x >= 0 && x < w;
+  for (int j = 0; j < 16; ++j) {
+    const int x = tid + 256 * j;
+    const bool in = x < w;
     const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1r + 1 < N));
     v[j] = make_float2(gload(r0, in0 ? x : 0), gload(r1c, in1 ? x : 0));
   }
 #pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const int x = tid + 256 * j - s0;
-    const bool in = x >= 0 && x < w;
+  for (int j = 0; j < 16; ++j) {
+    const int x = tid + 256 * j;
+    const bool in = x < w;
     const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1r + 1 < N));
     const float a = absorb ? deabs(v[j].x) : v[j].x, c = absorb ? deabs(v[j].y) : v[j].y;
     v[j] = make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
   }
 
-  dft32<false>(v);
+  dft32_lo16<false>(v);
+  if (w == 4096 && tid == 0) {
+    // z[4096] (point j = 16 of thread 0): factor exp(-2 pi i 16 k / 32) = (-1)^k
+    const bool e0 = t0 + 1 < N, e1 = has1 && t1r + 1 < N;
+    const float a = e0 ? (absorb ? deabs(r0[w]) : r0[w]) : 0.0f;
+    const float c = e1 ? (absorb ? deabs(r1[w]) : r1[w]) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      v[k].x += (k & 1) ? -a : a;
+      v[k].y += (k & 1) ? -c : c;
+    }
+  }
 #pragma unroll
   for (int k = 1; k < 32; ++k) v[k] = cmul(v[k], w8192(tw, tid * k));
 #pragma unroll
@@ -502,7 +548,8 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
     dft16<false>(u[s]);
 #pragma unroll
     for (int m1 = 0; m1 < 16; ++m1) {
-      const float wk = it.W[k + 32 * q + 512 * m1];
+      // the window and 1/Z (a power of two: exact wherever applied)
+      const float wk = it.W[k + 32 * q + 512 * m1] * (1.0f / static_cast<float>(Z));
       u[s][m1].x *= wk;
       u[s][m1].y *= wk;
     }
@@ -532,16 +579,15 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
   for (int k = 0; k < 32; ++k) v[k] = xb[tid * kPA8 + k];
 #pragma unroll
   for (int k = 1; k < 32; ++k) v[k] = cmulc(v[k], w8192(tw, tid * k));
-  dft32<true>(v);
+  dft32_out16<true>(v);  // the points t + 256 j, j < 16: n < 4096
 
-  const float invZ = 1.0f / static_cast<float>(Z);
   const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
   if (contrast != CTAS_CONTRAST_REF) {
 #pragma unroll
-    for (int j = 0; j < 32; ++j) {
-      const int x = tid + 256 * j - s0;
-      if (x < 0 || x >= w) continue;
-      const float a = v[j].x * invZ, c = v[j].y * invZ;
+    for (int j = 0; j < 16; ++j) {
+      const int x = tid + 256 * j;
+      if (x >= w) continue;
+      const float a = v[j].x, c = v[j].y;
       gstore<ST>(it.filt, row0 + x, a);
       if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
@@ -555,15 +601,15 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
   // prefix sum along each row, one lane per row, in sequential order.
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 32; ++j) xb[tid + 256 * j] = v[j];
+  for (int j = 0; j < 16; ++j) xb[tid + 256 * j] = v[j];
   __syncthreads();
   if (tid == 0 || (tid == 64 && has1)) {
     const bool im = tid == 64;
     const size_t rowb = im ? row1 : row0;
     float run = 0.0f;
     for (int x = 0; x < w; ++x) {
-      const float2 q = xb[s0 + x];
-      const float f = (im ? q.y : q.x) * invZ;
+      const float2 q = xb[x];
+      const float f = im ? q.y : q.x;
       run = (x == 0) ? f : run + f;
       gstore<ST>(it.filt, rowb + x, run);
       if (it.filt_f32) it.filt_f32[rowb + x] = run;

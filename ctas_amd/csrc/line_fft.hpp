@@ -170,13 +170,9 @@ __device__ __forceinline__ void dft4(float2 &x0, float2 &x1, float2 &x2, float2 
   }
 }
 
-// 16-point DFT in registers, natural order in and out: j = 4a + b,
-// k = c + 4d; inner DFT4 over a, factor exp(-+2 pi i b c / 16), outer DFT4
-// over b.
+// The factors exp(-+2 pi i b c / 16) between the two radix-4 stages of dft16.
 template <bool INV>
-__device__ __forceinline__ void dft16(float2 (&v)[16]) {
-#pragma unroll
-  for (int b = 0; b < 4; ++b) dft4<INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);  // v[b + 4c] = u_b[c]
+__device__ __forceinline__ void dft16_factors(float2 (&v)[16]) {
 #pragma unroll
   for (int b = 1; b < 4; ++b)
 #pragma unroll
@@ -184,6 +180,16 @@ __device__ __forceinline__ void dft16(float2 (&v)[16]) {
       const float co = kCos16[b * c], si = INV ? kSin16[b * c] : -kSin16[b * c];
       v[b + 4 * c] = cmul(v[b + 4 * c], make_float2(co, si));
     }
+}
+
+// 16-point DFT in registers, natural order in and out: j = 4a + b,
+// k = c + 4d; inner DFT4 over a, factor exp(-+2 pi i b c / 16), outer DFT4
+// over b.
+template <bool INV>
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) dft4<INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);  // v[b + 4c] = u_b[c]
+  dft16_factors<INV>(v);
   float2 r[16];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -196,6 +202,53 @@ __device__ __forceinline__ void dft16(float2 (&v)[16]) {
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = r[k];
+}
+
+// dft16 of points whose upper half v[8..15] is zero (not read): the inner
+// 4-point DFTs over (v[b], v[4 + b], 0, 0) are sums and differences.
+template <bool INV>
+__device__ __forceinline__ void dft16_lo8(float2 (&v)[16]) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const float2 x0 = v[b], x1 = v[4 + b];
+    v[b] = cadd(x0, x1);
+    v[8 + b] = csub(x0, x1);
+    const float2 p = make_float2(x0.x + x1.y, x0.y - x1.x), m = make_float2(x0.x - x1.y, x0.y + x1.x);
+    v[4 + b] = INV ? m : p;
+    v[12 + b] = INV ? p : m;
+  }
+  dft16_factors<INV>(v);
+  float2 r[16];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float2 y0 = v[4 * c], y1 = v[4 * c + 1], y2 = v[4 * c + 2], y3 = v[4 * c + 3];
+    dft4<INV>(y0, y1, y2, y3);
+    r[c] = y0;
+    r[c + 4] = y1;
+    r[c + 8] = y2;
+    r[c + 12] = y3;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = r[k];
+}
+
+// dft16 of which only the outputs v[0..7] are formed (v[8..15] are left
+// undefined): the outer 4-point DFTs give their first two outputs.
+template <bool INV>
+__device__ __forceinline__ void dft16_out8(float2 (&v)[16]) {
+#pragma unroll
+  for (int b = 0; b < 4; ++b) dft4<INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);
+  dft16_factors<INV>(v);
+  float2 r[8];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float2 y0 = v[4 * c], y1 = v[4 * c + 1], y2 = v[4 * c + 2], y3 = v[4 * c + 3];
+    const float2 a0 = cadd(y0, y2), a1 = csub(y0, y2), a2 = cadd(y1, y3), a3 = csub(y1, y3);
+    r[c] = cadd(a0, a2);
+    r[c + 4] = INV ? make_float2(a1.x - a3.y, a1.y + a3.x) : make_float2(a1.x + a3.y, a1.y - a3.x);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = r[k];
 }
 
 // v[k] *= r^k, k = 1..15, with r^2, r^4, r^8 from squarings and the other
@@ -283,6 +336,42 @@ __device__ __forceinline__ void dft32(float2 (&v)[32]) {
     v[k] = cadd(e[k], t);
     v[k + 16] = csub(e[k], t);
   }
+}
+
+// dft32 of points whose upper half v[16..31] is zero (not read): the even
+// and the odd points each have their upper half zero.
+template <bool INV>
+__device__ __forceinline__ void dft32_lo16(float2 (&v)[32]) {
+  float2 e[16], o[16];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    e[j] = v[2 * j];
+    o[j] = v[2 * j + 1];
+  }
+  dft16_lo8<INV>(e);
+  dft16_lo8<INV>(o);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const float2 t = cmul(o[k], make_float2(kCos32[k], INV ? kSin32[k] : -kSin32[k]));
+    v[k] = cadd(e[k], t);
+    v[k + 16] = csub(e[k], t);
+  }
+}
+
+// dft32 of which only the outputs v[0..15] are formed.
+template <bool INV>
+__device__ __forceinline__ void dft32_out16(float2 (&v)[32]) {
+  float2 e[16], o[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    e[j] = v[2 * j];
+    o[j] = v[2 * j + 1];
+  }
+  dft16<INV>(e);
+  dft16<INV>(o);
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    v[k] = cadd(e[k], cmul(o[k], make_float2(kCos32[k], INV ? kSin32[k] : -kSin32[k])));
 }
 
 // exp(-2 pi i m / 8192) from the half-turn table tw[m] (m < 4096).
