@@ -128,7 +128,8 @@ def pmc_measure(args, kernel_name, kind, ac, store, interp, updates, bp_ms, peak
     """HBM traffic and VALU instructions of one backprojection launch from
     two rocprofv3 --pmc passes over a child run of the same launch."""
     print("[bench] rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, SQ_INSTS_VALU)", file=sys.stderr, flush=True)
-    a = _pmc_pass(args, ["FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS"], kernel_name, kind, ac, store, interp)
+    a = _pmc_pass(args, ["FETCH_SIZE", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_ACTIVE_INST_VALU2", "SQ_LDS_IDX_ACTIVE"],
+                  kernel_name, kind, ac, store, interp)
     b = _pmc_pass(args, ["WRITE_SIZE", "SQ_WAVES"], kernel_name, kind, ac, store, interp)
     bp_a, bp_b, cp_a, cp_b = a.get("bp", {}), b.get("bp", {}), a.get("copy", {}), b.get("copy", {})
     need = [(bp_a, "FETCH_SIZE"), (bp_a, "SQ_INSTS_VALU"), (bp_b, "WRITE_SIZE"), (cp_a, "FETCH_SIZE"),
@@ -149,9 +150,13 @@ def pmc_measure(args, kernel_name, kind, ac, store, interp, updates, bp_ms, peak
                      "instr_per_update": round(valu * 64 / updates, 3),
                      "achieved": round(achieved, 3), "peak": peak, "unit": "wave instructions/clk/CU",
                      "frac": round(achieved / peak, 4),
+                     "dual_issued": bp_a.get("SQ_ACTIVE_INST_VALU2"),
+                     "dual_issued_frac": (round(bp_a["SQ_ACTIVE_INST_VALU2"] / valu, 4)
+                                          if bp_a.get("SQ_ACTIVE_INST_VALU2") else None),
                      "note": "VALU wave instructions of the launch (x64 lanes) per full-grid voxel-update; "
-                             "peak = measured v_add_f32 issue rate"},
-            "lds_issue": lds_issue(bp_a.get("SQ_INSTS_LDS"), bp_ms, clk_cu, updates, peaks),
+                             "peak = measured v_add_f32 issue rate; dual_issued = SQ_ACTIVE_INST_VALU2"},
+            "lds_issue": lds_issue(bp_a.get("SQ_INSTS_LDS"), bp_ms, clk_cu, updates, peaks,
+                                   bp_a.get("SQ_LDS_IDX_ACTIVE")),
             "waves": bp_b.get("SQ_WAVES")}
 
 
@@ -162,16 +167,18 @@ def binding_of(pmc):
     return "valu_issue" if v >= l_ else "lds_issue"
 
 
-def lds_issue(insts, bp_ms, clk_cu, updates, peaks):
+def lds_issue(insts, bp_ms, clk_cu, updates, peaks, idx_active=None):
     """LDS wave instructions of the launch against the measured ds_read_b32
-    issue rate (the kernels' sample reads are b32)."""
+    issue rate (the kernels' sample reads are b32), and the fraction of the
+    launch's CU cycles in which the LDS was busy (SQ_LDS_IDX_ACTIVE)."""
     if not insts:
         return None
     achieved = insts / (bp_ms * 1e-3) / clk_cu
     peak = peaks["lds_instr_per_clk_cu"]["b32"]
     return {"sq_insts_lds": insts, "instr_per_update": round(insts * 64 / updates, 3),
             "achieved": round(achieved, 3), "peak_b32": peak, "unit": "wave instructions/clk/CU",
-            "frac": round(achieved / peak, 4)}
+            "frac": round(achieved / peak, 4),
+            "lds_busy_frac": round(idx_active / (bp_ms * 1e-3) / clk_cu, 4) if idx_active else None}
 
 
 def pmc_child(args) -> int:

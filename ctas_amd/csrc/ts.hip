@@ -285,7 +285,7 @@ constexpr int kLineA = 256 * kA;             // [t][k], k < 8
 constexpr int kLineB = 128 * kB;             // [k * 16 + m2][t1]
 constexpr size_t kTs2048Lds = sizeof(float2) * (2 * kLineA + 2 * kLineB) + sizeof(float) * 32;  // + odd sums
 
-template <int J>
+template <int J, bool ODD>
 __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long long proj_stride, int n_proj,
                                                      TsConsts consts, float *filtered,
                                                      float *__restrict__ result, const float *__restrict__ W,
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long lon
 #pragma unroll
       for (int j = 0; j < 8; ++j) cur[l][j] = raw[l][j];
     if (p + 1 < n_proj) fetch(p + 1);
-    if (odd) {
+    if (ODD) {
       const float *base = proj + static_cast<size_t>(p) * proj_stride;
 #pragma unroll
       for (int l = 0; l < 2; ++l)
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void ts2048_kernel(const float *proj, long lon
       for (int k = 1; k < 8; ++k) y[l][k] = cmulc(y[l][k], tw1[k]);
       dft8<true>(y[l]);
     }
-    if (odd) {
+    if (ODD) {
 #pragma unroll
       for (int l = 0; l < 2; ++l) {
         const float4 t = odd_total(ostash + 16 * l);
@@ -530,7 +530,7 @@ __device__ __forceinline__ float2 w4096(const float2 *__restrict__ tw, int m) {
 constexpr int kP16 = 17;  // padded row length of the exchanges
 constexpr size_t kTs4096Lds = sizeof(float2) * 256 * kP16 + sizeof(float) * 16;  // + odd sums
 
-template <int J>
+template <int J, bool ODD>
 __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long long proj_stride, int n_proj,
                                                      TsConsts consts, float *filtered,
                                                      float *__restrict__ result, const float *__restrict__ W,
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long lon
 
   for (int p = 0; p < n_proj; ++p) {
     if (!kPrefetch) fetch(p);
-    if (odd)
+    if (ODD)
       odd_sums(proj + static_cast<size_t>(p) * proj_stride + row0, proj + static_cast<size_t>(p) * proj_stride + row1c,
                true, has1, n, absorb, odd, tid, ostash);
     float2 v[16];
@@ -641,7 +641,7 @@ __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long lon
 #pragma unroll
     for (int k = 1; k < 16; ++k) v[k] = cmulc(v[k], tw1[k]);
     dft16<true>(v);
-    if (odd) {
+    if (ODD) {
       const float4 t = odd_total(ostash);
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(256) void ts4096_kernel(const float *proj, long lon
 constexpr int kTsPA8 = 33, kTsPB8 = 17;
 constexpr size_t kTs8192Lds = sizeof(float2) * 512 * kTsPB8 + sizeof(float) * 16;  // + odd sums
 
-template <int J>
+template <int J, bool ODD>
 __global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long long proj_stride, int n_proj,
                                                      TsConsts consts, float *filtered,
                                                      float *__restrict__ result, const float *__restrict__ W,
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long lon
   for (int p = 0; p < n_proj; ++p) {
     const float *r0 = proj + static_cast<size_t>(p) * proj_stride + row0;
     const float *r1 = proj + static_cast<size_t>(p) * proj_stride + row1c;
-    if (odd) odd_sums(r0, r1, true, has1, n, absorb, odd, tid, ostash);
+    if (ODD) odd_sums(r0, r1, true, has1, n, absorb, odd, tid, ostash);
     float2 v[32];
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
@@ -817,7 +817,7 @@ __global__ __launch_bounds__(256) void ts8192_kernel(const float *proj, long lon
 #pragma unroll
     for (int k = 1; k < 32; ++k) v[k] = cmulc(v[k], w8192(tw, tid * k));
     dft32<true>(v);
-    if (odd) {
+    if (ODD) {
       const float4 t = odd_total(ostash);
 #pragma unroll
       for (int j = 0; j < 32; ++j) {
@@ -891,16 +891,16 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
     const dim3 grid((rows + 3) / 4), block(256);
     const int J = (n + 255) / 256;  // n <= 2048: J <= 8
     if (J <= 4)
-      hipLaunchKernelGGL(ts2048_kernel<4>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
+      hipLaunchKernelGGL((odd ? ts2048_kernel<4, true> : ts2048_kernel<4, false>), grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
                          result, W, tw, rows, n, contrast, odd, odd_g);
     else
-      hipLaunchKernelGGL(ts2048_kernel<8>, grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
+      hipLaunchKernelGGL((odd ? ts2048_kernel<8, true> : ts2048_kernel<8, false>), grid, block, kTs2048Lds, s, proj, proj_stride, n_proj, consts, filtered,
                          result, W, tw, rows, n, contrast, odd, odd_g);
     return hipGetLastError();
   }
   if (Z == 8192 && n <= 4096 && kTs8192Lds <= static_cast<size_t>(lds_max)) {
     const dim3 grid((rows + 1) / 2), block(256);
-    hipLaunchKernelGGL(ts8192_kernel<16>, grid, block, kTs8192Lds, s, proj, proj_stride, n_proj, consts, filtered,
+    hipLaunchKernelGGL((odd ? ts8192_kernel<16, true> : ts8192_kernel<16, false>), grid, block, kTs8192Lds, s, proj, proj_stride, n_proj, consts, filtered,
                        result, W, tw, rows, n, contrast, odd, odd_g);
     return hipGetLastError();
   }
@@ -908,10 +908,10 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
     const dim3 grid((rows + 1) / 2), block(256);
     const int J = (n + 255) / 256;  // n <= 4096: J <= 16
     if (J <= 8)
-      hipLaunchKernelGGL(ts4096_kernel<8>, grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
+      hipLaunchKernelGGL((odd ? ts4096_kernel<8, true> : ts4096_kernel<8, false>), grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
                          result, W, tw, rows, n, contrast, odd, odd_g);
     else
-      hipLaunchKernelGGL(ts4096_kernel<16>, grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
+      hipLaunchKernelGGL((odd ? ts4096_kernel<16, true> : ts4096_kernel<16, false>), grid, block, kTs4096Lds, s, proj, proj_stride, n_proj, consts, filtered,
                          result, W, tw, rows, n, contrast, odd, odd_g);
     return hipGetLastError();
   }
