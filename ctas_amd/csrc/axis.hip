@@ -48,6 +48,13 @@ Taps make_taps(float sigma) {
   return t;
 }
 
+// i mod w for i >= -kMaxTaps: one addition unless the taps outnumber w.
+__device__ __forceinline__ int wrap_index(int i, int w) {
+  if (i >= 0) return i;
+  i += w;
+  return i >= 0 ? i : ((i % w) + w) % w;
+}
+
 // Row pass: dst[u][v] = sum_n g[n] src[u][(v - n) mod w]. Grid (w, n).
 __global__ __launch_bounds__(256) void smooth_rows_kernel(const float *__restrict__ src, float *__restrict__ dst,
                                                           int w, Taps taps) {
@@ -58,9 +65,7 @@ __global__ __launch_bounds__(256) void smooth_rows_kernel(const float *__restric
   for (int v = threadIdx.x; v < w; v += blockDim.x) {
     float a = 0.0f;
     for (int n = 0; n < taps.nt; ++n) {
-      int idx = (v - n) % w;
-      if (idx < 0) idx += w;
-      a = a + taps.g[n] * row[idx];
+      a = a + taps.g[n] * row[wrap_index(v - n, w)];
     }
     dst[base + v] = a;
   }
@@ -74,9 +79,7 @@ __global__ __launch_bounds__(256) void smooth_cols_kernel(const float *__restric
   for (int v = threadIdx.x; v < w; v += blockDim.x) {
     float a = 0.0f;
     for (int m = 0; m < taps.nt; ++m) {
-      int r = (u - m) % w;
-      if (r < 0) r += w;
-      a = a + taps.g[m] * src[sl + static_cast<size_t>(r) * w + v];
+      a = a + taps.g[m] * src[sl + static_cast<size_t>(wrap_index(u - m, w)) * w + v];
     }
     dst[sl + static_cast<size_t>(u) * w + v] = a;
   }
@@ -91,24 +94,24 @@ __device__ __forceinline__ float sobel_term(const float *cs, int h, int w, int y
   return acc;
 }
 
+// Partial sums of the score of slice k (blockIdx.y) over the derivative rows
+// y = 1 + blockIdx.x, 1 + blockIdx.x + nb, ... (nb = gridDim.x), lanes over
+// the columns; partial[(k nb + b) 2 + {0, 1}] = {sum of cS, #(cS > 0)}.
 __global__ __launch_bounds__(256) void sobel_score_kernel(const float *__restrict__ slices,
                                                           const float *__restrict__ centres, int w,
                                                           int smoothed, double *__restrict__ partial) {
-  const int k = blockIdx.y;
+  const int k = blockIdx.y, nb = gridDim.x;
   const float c = centres[k];
   const int r = static_cast<int>(ceilf(fabsf(c)));
   const int h = w - 2 * r;
   const float *cs = slices + static_cast<size_t>(k) * w * w + static_cast<size_t>(r) * w;
   double sm = 0.0, cnt = 0.0;
-  if (h >= 3) {
-    const long long total = static_cast<long long>(h - 2) * (w - 2);
-    for (long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
-         e += static_cast<long long>(gridDim.x) * blockDim.x) {
-      const int y = 1 + static_cast<int>(e / (w - 2));
-      const int x = 1 + static_cast<int>(e % (w - 2));
-      // full-array row y holds cropped-derivative row y - r (zero above r)
-      const int yy = y - r;
-      if (yy < 0) continue;
+  // full-array row y holds cropped-derivative row y - r (zero above r):
+  // rows y = max(1, r) .. h - 2, columns 1 .. w - 2
+  for (int y = 1 + blockIdx.x; y <= h - 2; y += nb) {
+    const int yy = y - r;
+    if (yy < 0) continue;
+    for (int x = 1 + threadIdx.x; x <= w - 2; x += blockDim.x) {
       float dH = 0.0f, dV = 0.0f;
       dH = sobel_term(cs, h, w, yy, x, -1, -1, -1.0f, dH);
       dH = sobel_term(cs, h, w, yy, x, -1, 0, -2.0f, dH);
@@ -141,21 +144,35 @@ __global__ __launch_bounds__(256) void sobel_score_kernel(const float *__restric
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partial[(static_cast<size_t>(k) * kScoreBlocks + blockIdx.x) * 2] = ps[0];
-    partial[(static_cast<size_t>(k) * kScoreBlocks + blockIdx.x) * 2 + 1] = pc[0];
+    partial[(static_cast<size_t>(k) * nb + blockIdx.x) * 2] = ps[0];
+    partial[(static_cast<size_t>(k) * nb + blockIdx.x) * 2 + 1] = pc[0];
   }
 }
 
-__global__ void score_finish_kernel(const double *__restrict__ partial, float *__restrict__ scores, int n) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+// Score of slice k (blockIdx.x): the nb partials summed in a fixed order.
+__global__ __launch_bounds__(256) void score_finish_kernel(const double *__restrict__ partial,
+                                                           float *__restrict__ scores, int nb) {
+  const int k = blockIdx.x;
   double s = 0.0, c = 0.0;
-  for (int b = 0; b < kScoreBlocks; ++b) {
-    s += partial[(static_cast<size_t>(k) * kScoreBlocks + b) * 2];
-    c += partial[(static_cast<size_t>(k) * kScoreBlocks + b) * 2 + 1];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    s += partial[(static_cast<size_t>(k) * nb + b) * 2];
+    c += partial[(static_cast<size_t>(k) * nb + b) * 2 + 1];
   }
-  const float sm = static_cast<float>(s);
-  scores[k] = c > 0.0 ? sm / static_cast<float>(c) : 0.0f;
+  __shared__ double ps[256], pc[256];
+  ps[threadIdx.x] = s;
+  pc[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      ps[threadIdx.x] += ps[threadIdx.x + o];
+      pc[threadIdx.x] += pc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float sm = static_cast<float>(ps[0]);
+    scores[k] = pc[0] > 0.0 ? sm / static_cast<float>(pc[0]) : 0.0f;
+  }
 }
 
 }  // namespace
@@ -167,9 +184,12 @@ hipError_t launch_ag_scores(float *d_slices, int n, int w, const float *d_centre
     hipLaunchKernelGGL(smooth_rows_kernel, dim3(w, n), dim3(256), sizeof(float) * w, s, d_slices, d_tmp, w, taps);
     hipLaunchKernelGGL(smooth_cols_kernel, dim3(w, n), dim3(256), 0, s, d_tmp, d_slices, w, taps);
   }
-  hipLaunchKernelGGL(sobel_score_kernel, dim3(kScoreBlocks, n), dim3(256), 0, s, d_slices, d_centres, w,
+  // as many row blocks per slice as fill the chip, within the partial
+  // buffer's kScoreBlocks x kScoreChunk pairs
+  const int nb = std::max(1, std::min(1024, kScoreBlocks * kScoreChunk / std::max(n, 1)));
+  hipLaunchKernelGGL(sobel_score_kernel, dim3(nb, n), dim3(256), 0, s, d_slices, d_centres, w,
                      sigma > 1.0f ? 1 : 0, d_partial);
-  hipLaunchKernelGGL(score_finish_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d_partial, d_scores, n);
+  hipLaunchKernelGGL(score_finish_kernel, dim3(n), dim3(256), 0, s, d_partial, d_scores, nb);
   return hipGetLastError();
 }
 
@@ -189,7 +209,6 @@ int grow_buf(T *&ptr, size_t &cap, size_t bytes) {
   return CTAS_OK;
 }
 
-constexpr int kScoreChunk = 64;  // centres per batch (bounds the scratch slices)
 
 }  // namespace
 

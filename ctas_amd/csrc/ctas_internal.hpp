@@ -174,10 +174,12 @@ hipError_t launch_ts(const float *proj, long long proj_stride, int n_proj, const
 bool ts_odd_basis(int filter, float param, int n, int Z, std::vector<float2> &uv, float &gain);
 // Axis-score kernels (axis.hip).
 // AverageGradient::apply scores of n slices [n][w][w] reconstructed at the
-// given centres; d_tmp holds n*w*w floats, d_partial n*kScoreBlocks*2.
+// given centres (n <= kScoreChunk); d_tmp holds n*w*w floats, d_partial
+// kScoreBlocks*kScoreChunk*2 doubles.
 hipError_t launch_ag_scores(float *d_slices, int n, int w, const float *d_centres, float sigma,
                             float *d_tmp, double *d_partial, float *d_scores, hipStream_t s);
 constexpr int kScoreBlocks = 128;
+constexpr int kScoreChunk = 64;  // centres per scoring batch (bounds the scratch slices)
 // Phantom projector (phantom.hip).
 hipError_t launch_phantom(int N, int w, double c, int n_ell, const double *d_ell, double N0,
                           uint64_t seed, const float *d_gain, float *d_out, hipStream_t s);
