@@ -413,18 +413,21 @@ def test_raxis_reference_rejections():
         oracle.raxis(P, 180.0 / 240, 0.25, 2)
 
 
-def _ag_score_fft(slice_, centre, sigma):
+def _ag_score_fft(slice_, centre, sigma, dtype=np.float64):
     """AverageGradient::apply as the reference computes it: FFT-based
     circular convolution with the one-sided Gaussian exp(-(u^2+v^2)/sig22)
     (lct.cpp:837-843, 878-884), crop, Sobel, magnitude read from the top
     rows of the full derivative arrays (lct.cpp:905), borders zeroed,
-    sum / #(value > 0)."""
+    sum / #(value > 0). dtype float32 runs the FFTs in single precision
+    (scipy.fft keeps complex64), as the reference's FFTW float plans do."""
+    import scipy.fft as sfft
     w = slice_.shape[0]
-    S = slice_.astype(np.float64)
+    S = slice_.astype(dtype)
     if sigma > 1:
         i0, i1 = np.meshgrid(np.arange(w), np.arange(w), indexing="ij")
         sig22 = float(np.float32(2) * np.float32(sigma) * np.float32(sigma))
-        S = np.real(np.fft.ifft2(np.fft.fft2(S) * np.fft.fft2(np.exp(-(i0 * i0 + i1 * i1) / sig22))))
+        G = np.exp(-(i0 * i0 + i1 * i1) / sig22).astype(dtype)
+        S = np.real(sfft.ifft2(sfft.fft2(S) * sfft.fft2(G))).astype(dtype)
     r = int(np.ceil(abs(np.float32(centre))))
     h = w - 2 * r
     cs = S[r:w - r]
@@ -447,20 +450,25 @@ def _ag_score_fft(slice_, centre, sigma):
     return M.sum() / (M > 0).sum()
 
 
-@pytest.mark.parametrize("N,w,c0", [(361, 160, 0.0), (241, 192, 1.0)])
-def test_ag_score_matches_fft_restatement(N, w, c0):
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("N,w,c0", [(361, 160, 0.0), (241, 320, 1.0)])
+def test_ag_score_matches_fft_restatement(N, w, c0, dtype):
     """The separable direct smoothing + counting rule of the oracle equals
-    the reference's FFT formulation (w/(N-1) > 1 in the second case)."""
+    the reference's FFT formulation, with the FFTs in double and in single
+    precision. The second case has sigma = (0.75 deg in rad) x 320 / pi =
+    1.33 > 1, so the smoothing is on and its round-off decides #(cS > 0);
+    in the first, sigma = 0.44 and the smoothing is off."""
     P = oracle.phantom_sino(N, w, c0, oracle.random_ellipses(8, w, 1.0, 72), 1e5, 72)
     step = 180.0 / (N - 1)
     Pf = oracle.prepare(P, oracle.filter_window(RAMP, 0.0, w, oracle.ABS), oracle.ABS)
     sc = oracle.angles(N, step)
     sigma = float(np.float32(oracle.angO(step, N) * np.float32(w) / np.pi))
+    assert (sigma > 1) == (w == 320)
     for d in (-6.0, 0.0, 2.25):
         a = 0.5 * (w - 1) + d
         c = np.float32(np.float64(a) - 0.5 * (w - 1))
-        ref = _ag_score_fft(oracle.bp_ctas(Pf, sc, float(c)), c, sigma)
-        assert oracle.ag_score(Pf, sc, sigma, a) == pytest.approx(ref, rel=1e-3)
+        ref = _ag_score_fft(oracle.bp_ctas(Pf, sc, float(c)), c, sigma, dtype)
+        assert oracle.ag_score(Pf, sc, sigma, a) == pytest.approx(ref, rel=2e-4)
 
 
 def test_bp_f64_rows_is_bp_f64_restricted():
