@@ -68,12 +68,14 @@ def test_prepare_matches_oracle(contrast, filt):
     assert np.abs(got - ref).max() <= 2e-5 * scale
 
 
-@pytest.mark.parametrize("w", [1100, 2048])
+@pytest.mark.parametrize("w", [1025, 1100, 2047, 2048])
 @pytest.mark.parametrize("contrast,filt", [(ca.ABS, "HANN"), (ca.PHS, "RAMP"), (ca.REF, "BLACKMAN")])
 @pytest.mark.parametrize("store", [ca.STORE_FP32, ca.STORE_BF16])
 def test_prepare_matches_oracle_4096_point_lines(w, contrast, filt, store):
-    """Widths 1025-2048 pad to 4096 points, the radix-16 filter kernel; an
-    odd number of rows leaves the last line with one row."""
+    """Widths 1025-2048 pad to 4096 points, the radix-16 filter kernel, which
+    places the row at the start of the line: the next row's first sample
+    lands inside the first half for w < 2048 and at point 2048 for w = 2048;
+    an odd number of rows leaves the last line with one row."""
     N = 9
     P = _phantom(N, w, n_ell=6, seed=w)
     if contrast != ca.ABS:
@@ -106,6 +108,24 @@ def test_hann_offset_centre_batch():
     _, ref = oracle.fbp_batch(P, [3.37] * S, ca.ABS, 5, 0.0, 180.0, threads=3)
     for s in range(S):
         assert rel_rms(out[s], ref[s]) <= 1e-4
+
+
+@pytest.mark.parametrize("w", [2049, 3001, 4095])
+@pytest.mark.parametrize("contrast,filt", [(ca.ABS, "HANN"), (ca.REF, "RAMP")])
+def test_prepare_matches_oracle_8192_point_lines(w, contrast, filt):
+    """Widths 2049-4095 pad to 8192 points, the 32 x 16 x 16 filter kernel
+    with the row at the start of the line (the next row's first sample at
+    point w < 4096); 5 rows, the last line with one row."""
+    N = 5
+    P = _phantom(N, w, n_ell=6, seed=w)
+    if contrast != ca.ABS:
+        P = -np.log(np.maximum(P, 1e-3)).astype(np.float32)
+    rec = ca.CTrec(w, N, contrast, filt)
+    got = rec.prepare(P)
+    f, p = ca.parse_filter(filt)
+    ref = oracle.prepare(P, oracle.filter_window(f, p, w, contrast), contrast)
+    assert rel_rms(got, ref) <= 2e-5
+    assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
 @pytest.mark.parametrize("contrast", [ca.ABS, ca.REF])
