@@ -886,10 +886,9 @@ __device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigne
 
   run_passes_mirror<S>(it, lds, AC, ty0, tx0, wskip, [&](const unsigned char *buf, int na) {
     const unsigned char *lds0 = lds;  // word addresses are relative to it
-#pragma unroll 2
-    for (int a = 0; a < na; ++a) {
+    // slot a's row terms (window A, window B) and column terms of this lane
+    auto terms = [&](int a, int (&A)[TR], int (&B)[TR], int (&Cq)[TC]) {
       const int *tab = reinterpret_cast<const int *>(buf + S::WIN) + a * S::TAB;
-      int A[TR], B[TR], Cq[TC];
 #pragma unroll
       for (int r = 0; r < TR; r += 4) {
         const int4 va = *reinterpret_cast<const int4 *>(tab + rbase + r);
@@ -902,30 +901,74 @@ __device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigne
         const int4 vc = *reinterpret_cast<const int4 *>(tab + cbase + q);
         Cq[q] = vc.x; Cq[q + 1] = vc.y; Cq[q + 2] = vc.z; Cq[q + 3] = vc.w;
       }
+    };
+    // the words of both windows for the pixel pairs of row r
+    auto words = [&](const int (&A)[TR], const int (&B)[TR], const int (&Cq)[TC], int r, uint32_t (&WA)[TC],
+                     uint32_t (&WB)[TC]) {
 #pragma unroll
-      for (int r = 0; r < TR; ++r) {
-        uint32_t WA[TC], WB[TC];
+      for (int q = 0; q < TC; ++q) {
+        const unsigned ua = static_cast<unsigned>(A[r] + Cq[q]);
+        const unsigned ub = static_cast<unsigned>(B[r] - Cq[q]);
+        WA[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ua >> 16) << 2));
+        WB[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ub >> 16) << 2));
+      }
+    };
+    if constexpr (ST == ST_F16) {
+      // FP16: one packed add forms both pixels' pair sums of a slot, (lo WA
+      // + hi WB, hi WA + lo WB) (the swap is the add's operand select); the
+      // sums of two slots are added packed, each step rounded once to FP16,
+      // and a dot2 that picks its half adds each pixel's four samples to its
+      // FP32 accumulator: five instructions for eight updates
+      const half2v lo = {static_cast<_Float16>(1.0f), static_cast<_Float16>(0.0f)};
+      const half2v hi = {static_cast<_Float16>(0.0f), static_cast<_Float16>(1.0f)};
+      auto pair_sum = [](uint32_t wa, uint32_t wb) {
+        const half2v pa = __builtin_bit_cast(half2v, wa), pb = __builtin_bit_cast(half2v, wb);
+        return pa + __builtin_shufflevector(pb, pb, 1, 0);
+      };
+      int a = 0;
+      for (; a + 1 < na; a += 2) {
+        int A0[TR], B0[TR], C0[TC], A1[TR], B1[TR], C1[TC];
+        terms(a, A0, B0, C0);
+        terms(a + 1, A1, B1, C1);
 #pragma unroll
-        for (int q = 0; q < TC; ++q) {
-          const unsigned ua = static_cast<unsigned>(A[r] + Cq[q]);
-          const unsigned ub = static_cast<unsigned>(B[r] - Cq[q]);
-          WA[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ua >> 16) << 2));
-          WB[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ub >> 16) << 2));
-        }
+        for (int r = 0; r < TR; ++r) {
+          uint32_t WA0[TC], WB0[TC], WA1[TC], WB1[TC];
+          words(A0, B0, C0, r, WA0, WB0);
+          words(A1, B1, C1, r, WA1, WB1);
 #pragma unroll
-        for (int q = 0; q < TC; ++q) {
-          if constexpr (ST == ST_F16) {
-            // FP16: one packed add forms both pixels' pair sums, (lo WA +
-            // hi WB, hi WA + lo WB) (the swap is the add's operand select),
-            // each rounded once to FP16 and added by a dot2 that picks its
-            // half: three instructions for the four updates
-            const half2v pa = __builtin_bit_cast(half2v, WA[q]), pb = __builtin_bit_cast(half2v, WB[q]);
-            const half2v sum = pa + __builtin_shufflevector(pb, pb, 1, 0);
-            const half2v lo = {static_cast<_Float16>(1.0f), static_cast<_Float16>(0.0f)};
-            const half2v hi = {static_cast<_Float16>(0.0f), static_cast<_Float16>(1.0f)};
+          for (int q = 0; q < TC; ++q) {
+            const half2v sum = pair_sum(WA0[q], WB0[q]) + pair_sum(WA1[q], WB1[q]);
             acc[r][q] = __builtin_amdgcn_fdot2(sum, lo, acc[r][q], false);
             accm[r][q] = __builtin_amdgcn_fdot2(sum, hi, accm[r][q], false);
-          } else {
+          }
+        }
+      }
+      if (a < na) {  // an odd slot count: the last slot alone
+        int A[TR], B[TR], Cq[TC];
+        terms(a, A, B, Cq);
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+          uint32_t WA[TC], WB[TC];
+          words(A, B, Cq, r, WA, WB);
+#pragma unroll
+          for (int q = 0; q < TC; ++q) {
+            const half2v sum = pair_sum(WA[q], WB[q]);
+            acc[r][q] = __builtin_amdgcn_fdot2(sum, lo, acc[r][q], false);
+            accm[r][q] = __builtin_amdgcn_fdot2(sum, hi, accm[r][q], false);
+          }
+        }
+      }
+    } else {
+#pragma unroll 2
+      for (int a = 0; a < na; ++a) {
+        int A[TR], B[TR], Cq[TC];
+        terms(a, A, B, Cq);
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+          uint32_t WA[TC], WB[TC];
+          words(A, B, Cq, r, WA, WB);
+#pragma unroll
+          for (int q = 0; q < TC; ++q) {
             // the pixel's two samples (lo WA, hi WB) and the mirror pixel's
             // (lo WB, hi WA), merged by bit-field inserts (the asm keeps them
             // 32-bit merges) and each added by one dot2
