@@ -34,29 +34,29 @@ def _need_gpu():
         pytest.skip("no HIP device")
 
 
-_CFG2 = {}
+_W2048 = {}
 
 
-def _config2():
+def _inputs_2048x1800():
     """Two slices of the batched FP32 workload: 20 seeded ellipses (semi-axes
     x4), Poisson 1e5, HANN padded to 4096, c = +3.37."""
-    if not _CFG2:
+    if not _W2048:
         N, w, c, S = 1800, 2048, 3.37, 2
         P = np.stack([oracle.phantom_sino(N, w, c, oracle.random_ellipses(20, w, 4.0, 200 + s), 1e5, 200 + s)
                       for s in range(S)])
         W = oracle.filter_window(5, 0.0, w, ca.ABS)
         Pf = np.stack([oracle.prepare(P[s], W, ca.ABS) for s in range(S)])
         ref = np.stack([oracle.bp_ctas_rows(Pf[s], oracle.angles(N), c, ROWS) for s in range(S)])
-        _CFG2.update(P=P, Pf=Pf, ref=ref, N=N, w=w, c=c, S=S)
-    return _CFG2
+        _W2048.update(P=P, Pf=Pf, ref=ref, N=N, w=w, c=c, S=S)
+    return _W2048
 
 
-def test_config2_shape_graph_autotuned(torch_cuda):
+def test_2048x1800_graph_autotuned(torch_cuda):
     """2048 x 1800 x 2 slices through graph_capture / graph_launch with the
     tile the autotuner picks; then the same tile on the oracle's filtered
     sinograms (FLT) must give the oracle's rows bit for bit."""
     torch = torch_cuda
-    d = _config2()
+    d = _inputs_2048x1800()
     N, w, c, S = d["N"], d["w"], d["c"], d["S"]
     d_in = torch.from_numpy(d["P"]).cuda()
     d_out = torch.zeros((S, w, w), dtype=torch.float32, device="cuda")
@@ -87,7 +87,7 @@ def test_config2_shape_graph_autotuned(torch_cuda):
         assert np.array_equal(out[s][ROWS], d["ref"][s]), f"slice {s}: {(out[s][ROWS] != d['ref'][s]).sum()} pixels"
 
 
-def test_config4_uneven_batch_against_oracle(torch_cuda):
+def test_uneven_widths_batch_against_oracle(torch_cuda):
     """One item of each width class (512 .. 4096: Z from 1024 to 8192, gain
     4/3 for 768, 1536 and 3072) with angle counts from {360, 900, 1440,
     1800}, HANN, ABS, in one persistent launch; 8 rows of each slice against

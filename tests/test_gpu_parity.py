@@ -88,8 +88,8 @@ def test_prepare_matches_oracle_4096_point_lines(w, contrast, filt, store):
     assert np.abs(got - ref).max() <= 2e-5 * scale
 
 
-def test_config1_end_to_end():
-    """512 px x 720 angles, 10 ellipses, Poisson 1e5, RAMP, c = 0 (config 1)."""
+def test_512x720_ramp_end_to_end():
+    """512 px x 720 angles, 10 ellipses, Poisson 1e5, RAMP, c = 0."""
     N, w = 720, 512
     P = _phantom(N, w, 0.0, 10, 1e5, seed=11)
     rec = ca.CTrec(w, N, ca.ABS, "RAMP")
@@ -364,25 +364,25 @@ def test_raxis_matches_oracle(c0):
     assert abs(ca.raxis(P, step, 0.25, 1) - 0.5 * (w - 1) - c0) <= 0.25
 
 
-_CFG3 = {}
+_W4096 = {}
 
 
-def _config3_inputs():
+def _inputs_4096x3000():
     """One slice at the reduced-precision workload's shape: 4096 px x 3000
     angles over 180 degrees, 40 seeded ellipses (semi-axes scaled x8),
     Poisson 1e5, 1% multiplicative per-detector gain errors (a ring source);
     the FP64 oracle and the reference's float CTAS oracle on 64 evenly
     spaced slice rows."""
-    if not _CFG3:
+    if not _W4096:
         N, w, c = 3000, 4096, 3.37
         P = oracle.phantom_sino(N, w, c, oracle.random_ellipses(40, w, 8.0, 33), 1e5, 33)
         gain = (1.0 + 0.01 * np.random.default_rng(33).standard_normal(w)).astype(np.float32)
         P = (P * gain[None, :]).astype(np.float32)
         rows = np.arange(32, w, 64)
         Pf = oracle.prepare(P, oracle.filter_window(5, 0.0, w, ca.ABS), ca.ABS)
-        _CFG3.update(P=P, c=c, rows=rows, ref64=oracle.bp_f64_rows(Pf, c, rows),
+        _W4096.update(P=P, c=c, rows=rows, ref64=oracle.bp_f64_rows(Pf, c, rows),
                      ref_ctas=oracle.bp_ctas_rows(Pf, oracle.angles(N), c, rows))
-    return _CFG3
+    return _W4096
 
 
 @pytest.mark.parametrize("store", [ca.STORE_FP16, ca.STORE_BF16, ca.STORE_FP32])
@@ -391,7 +391,7 @@ def test_reduced_precision_at_4096x3000(store):
     FP64 geometry): relative RMS <= 2e-3 against the FP64 oracle. FP32: the
     reference's float geometry, <= 1e-4 against the float CTAS oracle (the
     reference's CPU path; that geometry is itself ~2.5e-3 from FP64 here)."""
-    d = _config3_inputs()
+    d = _inputs_4096x3000()
     rec = ca.CTrec(4096, 3000, ca.ABS, "HANN", store=store)
     _, out = rec.reconstruct(d["P"], d["c"])
     got = out[0][d["rows"]]
