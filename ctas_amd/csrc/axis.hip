@@ -275,13 +275,14 @@ extern "C" int ctas_axis_scores(ctas_fbp *h, const float *axisR, int n, float *s
           hipMalloc(&h->d_scores, sizeof(float) * kScoreChunk) != hipSuccess)
         return fail(CTAS_E_NOMEM, "allocation failed");
     }
-    CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
-    CTAS_HIP_TRY(hipMemcpy(h->d_centres, centres.data(), sizeof(float) * m, hipMemcpyHostToDevice));
+    // all on the handle's stream: the centres' upload, the trial slices, the
+    // scores and their download, then one wait (centres outlives it)
+    CTAS_HIP_TRY(hipMemcpyAsync(h->d_centres, centres.data(), sizeof(float) * m, hipMemcpyHostToDevice, h->stream));
     if (int rc = ctas_fbp_repeat_dev(h, 0, h->d_out, centres.data(), m, h->stream)) return rc;
     CTAS_HIP_TRY(launch_ag_scores(h->d_out, m, w, h->d_centres, sigma, h->d_tmp, h->d_partial, h->d_scores,
                                   h->stream));
+    CTAS_HIP_TRY(hipMemcpyAsync(scores + k0, h->d_scores, sizeof(float) * m, hipMemcpyDeviceToHost, h->stream));
     CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
-    CTAS_HIP_TRY(hipMemcpy(scores + k0, h->d_scores, sizeof(float) * m, hipMemcpyDeviceToHost));
   }
   return CTAS_OK;
 }
