@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void minmax_kernel(const Item *__restrict__ it
 template <int ST>
 __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ items, int n_items,
                                                      int contrast) {
-  // The only LDS is the complex line (Z x 8 bytes), so Z = 8192 fits 64 KiB.
+  // The only LDS is the complex line (Z x 8 bytes): 128 KiB at Z = 16384.
   extern __shared__ float2 buf[];
 
   const int b = blockIdx.x;
@@ -164,30 +164,35 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
   // Padded line z[n], n in [0, Z): row samples at s0..s0+w-1 and, from the
   // inclusive copy range of lct.cpp:580-584, the next row's first sample at
   // s0+w (0 after the last row). Stored at bit-reversed positions.
-  // The loads of up to 32 points per thread are issued before any is used,
-  // at indices clamped into the rows (samples outside are zeroed after).
+  // The points t + 256 k of each thread are loaded in chunks of 32 (two for
+  // Z = 16384), all loads of a chunk issued before any is used, at indices
+  // clamped into the rows (samples outside are zeroed after).
   const float *r1c = has1 ? r1 : r0;
   const int per = (Z + 255) >> 8;  // points per thread (the last partial below 256 points)
-  float ra[32], rb[32];
+  for (int k0 = 0; k0 < per; k0 += 32) {
+    float ra[32], rb[32];
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    if (k < per && tid + 256 * k < Z) {
-      const int x = tid + 256 * k - s0;
-      const bool in = x >= 0 && x < w;
-      const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
-      ra[k] = gload(r0, in0 ? x : 0);
-      rb[k] = gload(r1c, in1 ? x : 0);
+    for (int k = 0; k < 32; ++k) {
+      const int kk = k0 + k;
+      if (kk < per && tid + 256 * kk < Z) {
+        const int x = tid + 256 * kk - s0;
+        const bool in = x >= 0 && x < w;
+        const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+        ra[k] = gload(r0, in0 ? x : 0);
+        rb[k] = gload(r1c, in1 ? x : 0);
+      }
     }
-  }
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    if (k < per && tid + 256 * k < Z) {
-      const int n = tid + 256 * k, x = n - s0;
-      const bool in = x >= 0 && x < w;
-      const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
-      const float a = absorb ? deabs(ra[k]) : ra[k], c = absorb ? deabs(rb[k]) : rb[k];
-      buf[lds_pad(__brev(static_cast<unsigned>(n)) >> (32 - logZ), psh)] =
-          make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
+    for (int k = 0; k < 32; ++k) {
+      const int kk = k0 + k;
+      if (kk < per && tid + 256 * kk < Z) {
+        const int n = tid + 256 * kk, x = n - s0;
+        const bool in = x >= 0 && x < w;
+        const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+        const float a = absorb ? deabs(ra[k]) : ra[k], c = absorb ? deabs(rb[k]) : rb[k];
+        buf[lds_pad(__brev(static_cast<unsigned>(n)) >> (32 - logZ), psh)] =
+            make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
+      }
     }
   }
   __syncthreads();

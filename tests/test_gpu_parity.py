@@ -128,9 +128,36 @@ def test_prepare_matches_oracle_8192_point_lines(w, contrast, filt):
     assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("w,contrast", [(5000, ca.ABS), (8192, ca.REF)])
+def test_widths_beyond_4096_zidth_16384(w, contrast):
+    """Widths 4097-8192 pad to 16384 points: the generic filter holds the
+    128 KiB line in LDS and loads each thread's 64 points in two chunks;
+    the backprojection and the 16-bit pads do not depend on the width.
+    FP32 against the oracle's CTAS CPU path, 12 angles. FP16 storage, whose
+    zero pads grow with w, against the FP64 oracle: with 12 angles each
+    sample is a twelfth of a pixel, and the samples whose 16.16 coordinate
+    lands on the other side of a bin edge than the exact one (those within
+    ~3e-5 px of it) give 2.7e-3 to 3.7e-3 relative RMS at every width from
+    3000 to 8192 (1.2e-4 to 5.4e-4 with 360 to 3000 angles): the bar is set
+    for that, and catches a wrong pad or window outright."""
+    N = 12
+    P = _phantom(N, w, 3.0, 8, 1e5, seed=w, scale=w / 1024)
+    if contrast == ca.REF:
+        P = (-np.log(P)).astype(np.float32)
+    rec = ca.CTrec(w, N, contrast, "HANN")
+    filt, out = rec.reconstruct(P, 3.0)
+    ref_f, ref = oracle.fbp_batch(P, [3.0], contrast, 5, 0.0, 180.0)
+    assert rel_rms(filt, ref_f[0]) <= 2e-5
+    assert rel_rms(out[0], ref[0]) <= 1e-4
+    if contrast == ca.ABS:
+        _, out16 = ca.CTrec(w, N, contrast, "HANN", store=ca.STORE_FP16).reconstruct(P, 3.0)
+        ref64 = oracle.bp_f64(ref_f[0], 3.0, np.pi, 1.0)
+        assert rel_rms(out16[0], ref64) <= 6e-3
+
+
 @pytest.mark.parametrize("contrast", [ca.ABS, ca.REF])
 def test_width_4096_zidth_8192(contrast):
-    """The widest configuration (w = 4096, zidth 8192: a 64 KiB LDS line)."""
+    """w = 4096, zidth 8192: a 64 KiB LDS line in the 32 x 16 x 16 filter."""
     N, w = 24, 4096
     P = _phantom(N, w, 5.0, 12, 1e5, seed=90, scale=4.0)
     if contrast == ca.REF:
