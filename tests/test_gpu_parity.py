@@ -558,10 +558,11 @@ def test_raxis_directional_walk(c0):
 
 
 @pytest.mark.parametrize("w,N,c", [(2, 1, 0.0), (3, 2, 0.5), (5, 3, -1.0), (16, 7, 2.5), (63, 5, 30.0),
-                                   (64, 1, -31.5), (65, 4, 0.25)])
+                                   (64, 1, -31.5), (65, 4, 0.25), (256, 7200, 1.5)])
 def test_tiny_and_edge_shapes(w, N, c):
     """Widths 2-65, one to seven angles, centres up to the |c| < w/2 limit:
-    a single tile, partly masked tiles, lines of 4 to 128 points. FLT: the
+    a single tile, partly masked tiles, lines of 4 to 128 points; and 7200
+    angles (3601 mirror slots) at 256 px. FLT: the
     backprojection bit for bit; ABS/HANN end to end within 1e-4 of the
     oracle (the slice is all zero where the mask covers it); FP16 storage
     finite and zero where FP32 is."""
