@@ -187,3 +187,35 @@ def test_uneven_batch_reduced_precision_mirrored(torch_cuda, store):
         err = rel_rms(got, refs[k])
         print(f"store={store:#x} w={w} N={N} arc={arc}: rel_rms {err:.2e}")
         assert err <= 2e-3
+
+
+@pytest.mark.parametrize("contrast,store", [(ca.ABS, ca.STORE_FP32), (ca.REF, ca.STORE_FP32),
+                                            (ca.ABS, ca.STORE_FP16)])
+def test_uneven_batch_edge_widths(torch_cuda, contrast, store):
+    """An uneven batch spanning every filter kernel and line length: widths
+    3 (4-point lines), 100, 1100 (4096), 3001 (8192) and 5000 (16384
+    points), angle counts 2 to 90, REF contrast included. FP32 against the
+    oracle's CTAS CPU path within 1e-4 (all-zero slices where the mask
+    covers them); FP16 finite and zero where FP32 is."""
+    torch = torch_cuda
+    shapes = [(3, 2, 0.5), (100, 90, -3.0), (1100, 30, 2.25), (3001, 12, 7.0), (5000, 12, -4.5)]
+    items, Ps = [], []
+    for k, (w, N, c) in enumerate(shapes):
+        P = oracle.phantom_sino(N, w, c, oracle.random_ellipses(6, w, max(w / 1024, 0.05), 60 + k), 1e5, 60 + k)
+        if contrast == ca.REF:
+            P = (-np.log(np.maximum(P, 1e-3))).astype(np.float32)
+        Ps.append(P)
+        items.append((torch.from_numpy(P).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), c))
+    ca.batch_uneven(items, contrast, 5, 0.0, 180.0, store)
+    torch.cuda.synchronize()
+    for k, (w, N, c) in enumerate(shapes):
+        got = items[k][1].cpu().numpy()
+        if store == ca.STORE_FP16:
+            assert np.isfinite(got).all()
+            continue
+        _, ref = oracle.fbp_batch(Ps[k], [c], contrast, 5, 0.0, 180.0)
+        print(f"contrast={contrast} w={w} N={N} c={c}")
+        if np.abs(ref[0]).max() == 0:
+            assert not got.any()
+        else:
+            assert rel_rms(got, ref[0]) <= 1e-4
