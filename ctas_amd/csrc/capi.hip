@@ -1027,6 +1027,8 @@ int ctas_fbp_pipeline(ctas_fbp *h, long n_slices, int batch, const float *center
                       ctas_read_fn read, ctas_write_fn write, void *user, int io_threads) {
   if (!h || !centers || !read || !write || n_slices < 0 || batch < 1 || ring_box < 0 || io_threads < 1)
     return fail(CTAS_E_ARG, "bad arguments");
+  if (ring_box)
+    if (int rc = check_ring_box(ring_box, h->w)) return rc;
   if (n_slices == 0) return CTAS_OK;
   for (long i = 0; i < n_slices; ++i)
     if (int rc = check_center(h->w, centers[i])) return rc;
@@ -1221,6 +1223,7 @@ int ctas_fbp_pipeline(ctas_fbp *h, long n_slices, int batch, const float *center
 int ctas_ring_filter(float *sino, int N, int w, int box) {
   if (!sino || N <= 0 || w <= 0 || box < 0) return fail(CTAS_E_ARG, "bad arguments");
   if (box == 0) return CTAS_OK;
+  if (int rc = check_ring_box(box, w)) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CTAS_E_DEVICE, "no HIP device is available");
   float *d = nullptr, *avg = nullptr;
@@ -1241,6 +1244,7 @@ int ctas_ring_filter(float *sino, int N, int w, int box) {
 int ctas_ring_filter_dev(float *d_sino, int S, int N, int w, int box, void *stream) {
   if (!d_sino || S <= 0 || N <= 0 || w <= 0 || box < 0) return fail(CTAS_E_ARG, "bad arguments");
   if (box == 0) return CTAS_OK;
+  if (int rc = check_ring_box(box, w)) return rc;
   // The column means need S*w floats of scratch; it is freed after the
   // stream drains.
   float *avg = nullptr;

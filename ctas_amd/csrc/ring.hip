@@ -79,6 +79,7 @@ extern "C" int ctas_ring_filter_dev_noalloc(float *d_sino, int S, int N, int w, 
                                 std::to_string(w) + " columns exceed its " + std::to_string(kRingScratchFloats) +
                                 " column means of scratch");
   if (box == 0) return CTAS_OK;
+  if (int rc = check_ring_box(box, w)) return rc;
   int dev = 0;
   CTAS_HIP_TRY(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDevices) return fail(CTAS_E_DEVICE, "device index " + std::to_string(dev) + " out of range");

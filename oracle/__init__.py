@@ -49,7 +49,7 @@ def lib(fast: bool = False) -> C.CDLL:
         "orc_bp_f64": (None, [fp, i, i, d, d, d, i, dp]),
         "orc_bp_f64_rows": (None, [fp, i, i, d, d, d, i, C.POINTER(i), i, dp]),
         "orc_fbp_batch": (i, [fp, i, i, i, i, i, f, f, fp, f, fp, i]),
-        "orc_ring_filter": (None, [fp, i, i, i]),
+        "orc_ring_filter": (i, [fp, i, i, i]),
         "orc_ag_score": (d, [fp, i, i, fp, f, f, fp]),
         "orc_raxis": (i, [fp, i, i, f, f, i, fp]),
         "orc_raxis_pair": (i, [fp, fp, i, i, f, fp]),
@@ -173,7 +173,8 @@ def fbp_batch(sino: np.ndarray, centers, contrast=ABS, ftype=1, fparam=0.0, anar
 
 def ring_filter(P: np.ndarray, box: int) -> np.ndarray:
     x = np.array(P, np.float32, copy=True, order="C")
-    lib().orc_ring_filter(_f(x), x.shape[0], x.shape[1], box)
+    if lib().orc_ring_filter(_f(x), x.shape[0], x.shape[1], box) < 0:
+        raise ValueError(f"ring box {box} is not below the width {x.shape[1]}")
     return x
 
 

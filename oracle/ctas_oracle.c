@@ -459,8 +459,11 @@ int orc_fbp_batch(float *sino, int S, int N, int w, int contrast, int ftype,
 
 /* RingFilter::apply (lct.cpp:295-327) without a mask (the ct tool passes
  * none, ct.cpp:200). */
-void orc_ring_filter(float *P, int N, int w, int box) {
-  if (box <= 0 || N <= 0) return;
+int orc_ring_filter(float *P, int N, int w, int box) {
+  if (box <= 0 || N <= 0) return 0;
+  /* temp(ii) and temp(w-1-ii) for every ii < box (lct.cpp:317-320) lie in
+     the line only when box < w; wider boxes index outside it there */
+  if (box >= w) return -1;
   float *avg = (float *)calloc((size_t)w, sizeof(float));
   float *tmp = (float *)calloc((size_t)w, sizeof(float));
   for (int r = 0; r < N; r++)
@@ -481,6 +484,7 @@ void orc_ring_filter(float *P, int N, int w, int box) {
     for (int x = 0; x < w; x++) P[(size_t)r * w + x] = P[(size_t)r * w + x] + tmp[x];
   free(avg);
   free(tmp);
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */

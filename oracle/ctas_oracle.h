@@ -59,7 +59,7 @@ void  orc_bp_f64_rows(const float *P, int N, int w, double arc_rad, double c,
 int   orc_fbp_batch(float *sino, int S, int N, int w, int contrast, int ftype,
                     float fparam, float anarc, const float *centers,
                     float recCof, float *out, int nthreads);
-void  orc_ring_filter(float *P, int N, int w, int box);
+int   orc_ring_filter(float *P, int N, int w, int box);  /* -1: box >= w */
 double orc_ag_score(const float *Pf, int N, int w, const float *sincos,
                     float sigma, float axisR, float *work);
 int   orc_raxis(const float *sino, int N, int w, float anarc, float maxDev,

@@ -352,10 +352,14 @@ def test_uneven_batch_persistent(torch_cuda):
 
 
 def test_ring_filter_bit_exact():
+    """Boxes from 1 to w - 1 (48 and 96: the reference's edge loops overlap)
+    bit for bit; a box of the width is refused."""
     rng = np.random.default_rng(2)
     P = rng.random((50, 97)).astype(np.float32)
-    for box in (1, 3, 8):
+    for box in (1, 3, 8, 48, 96):
         assert np.array_equal(ca.ring_filter(P, box), oracle.ring_filter(P, box))
+    with pytest.raises(ca.CtasError):
+        ca.ring_filter(P, 97)
 
 
 def _dots(w, n=8, seed=0):

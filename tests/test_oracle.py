@@ -355,6 +355,27 @@ def test_ring_filter_edge_columns_closed_form(box):
     assert np.abs(exp[:, 0] - (P[:, 0] + avg[:box + 1].sum() / (2 * box + 1) - avg[0])).max() > 1e-3
 
 
+def test_ring_filter_box_limits():
+    """A box of the width or more is refused (RingFilter::apply, lct.cpp:317-320,
+    would divide entries outside the line); below it the reference's loops
+    run as written, overlapping for box >= w/2. A 5-wide line with box 4:
+    every column's clipped window is the whole line, the interior division
+    (columns box .. w-1-box) is empty, and the edge loop ii = 0..3 divides
+    columns 0 and 4 by 5, columns 1 and 3 by 6 and by 8, column 2 twice by 7."""
+    rng = np.random.default_rng(5)
+    P = rng.random((6, 5)).astype(np.float32)
+    for box in (5, 9):
+        with pytest.raises(ValueError):
+            oracle.ring_filter(P, box)
+    out = oracle.ring_filter(P, 4)
+    assert np.isfinite(out).all()
+    avg = P.astype(np.float64).mean(0)
+    tot = avg.sum()
+    div = np.array([5.0, 48.0, 49.0, 48.0, 5.0])
+    exp = P.astype(np.float64) + (tot / div - avg)[None, :]
+    assert np.abs(out - exp).max() <= 2e-6 * np.abs(exp).max()
+
+
 def test_phantom_noise_statistics():
     w, N = 64, 200
     ell = np.array([[0, 0, 20, 20, 0, 0.02]])
