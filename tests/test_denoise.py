@@ -58,7 +58,8 @@ def test_threshold_rules():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("shape,rad,thr", [((23, 31), 1, 0.0), ((64, 300), 2, 0.05), ((180, 257), 3, -0.1),
-                                           ((7, 5), 5, 0.0), ((1800, 2048), 3, 0.02)])
+                                           ((7, 5), 5, 0.0), ((1800, 2048), 3, 0.02), ((1, 1), 50, 0.5),
+                                           ((1, 7), 2, 0.0), ((2, 2), 5, -0.5), ((3, 1), 0, 0.0)])
 def test_gpu_denoise_bit_exact(shape, rad, thr):
     if ca.device_count() < 1:
         pytest.skip("no HIP device")
