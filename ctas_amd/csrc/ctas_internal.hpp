@@ -15,7 +15,7 @@ namespace ctas {
 // ---- errors ---------------------------------------------------------------
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
-// RingFilter::apply (lct.cpp:313-319) divides temp(ii) and temp(w-1-ii) for
+// RingFilter::apply (lct.cpp:317-320) divides temp(ii) and temp(w-1-ii) for
 // every ii < box: a box of w or more indexes outside the line there, so such
 // a box is rejected (CTAS_E_ARG); narrower boxes keep the reference's sums.
 inline int check_ring_box(int box, int w) {
