@@ -294,6 +294,90 @@ def test_recof_physics():
     assert np.allclose(out[0][inside], 0.25 * r, rtol=1e-6)
 
 
+def _dead_pixels(P, w, N, seed):
+    """Marks samples of slice 1 non-positive (dead pixels), among them the
+    first sample of row 2 (also read into the padded line of row 1,
+    lct.cpp:580-584) and the last sample, plus a positive one below
+    max/1e6; slice 2 gets a positive sample below its own max/1e6, which
+    stays unclamped: its minimum is positive."""
+    rng = np.random.default_rng(seed)
+    P[1, rng.integers(0, N, 5), rng.integers(0, w, 5)] = 0.0
+    P[1, 2, 0] = -3.0
+    P[1, N - 1, w - 1] = 0.0
+    P[1, 3, w // 2] = 1e-12
+    P[2, 4, w // 3] = 1e-3 * P[2].max() / 1e6
+
+
+@pytest.mark.parametrize("w", [300, 2048, 4096, 5000])
+def test_deabs_clamp_path(w):
+    """deAbs (physical.world.cpp:88-100): a sinogram whose minimum is not
+    positive has every sample at or below max/1e6 raised to that value before
+    -log; the rule is per sinogram. Three slices in one call, only the middle
+    one with such samples. The widths take the four filter kernels (1024-,
+    4096-, 8192- and 16384-point lines); N = 9 leaves a single-row line."""
+    N, S = 9, 3
+    P = np.stack([_phantom(N, w, n_ell=6, seed=w + s) for s in range(S)])
+    _dead_pixels(P, w, N, w)
+    rec = ca.CTrec(w, N, ca.ABS, "HANN")
+    got = rec.prepare(P)
+    W = oracle.filter_window(*ca.parse_filter("HANN"), w, ca.ABS)
+    for s in range(S):
+        ref = oracle.prepare(P[s], W, ca.ABS)
+        assert np.abs(got[s] - ref).max() <= 2e-5 * np.abs(ref).max(), s
+    assert (rec.status(S) == 0).all()
+
+
+def test_deabs_clamp_in_graph_replays(torch_cuda):
+    """A captured graph replayed over sinograms that alternate between
+    positive data and data with dead pixels: each replay equals the eager
+    reconstruction of the data it was given."""
+    torch = torch_cuda
+    N, w, S = 90, 2048, 3
+    good = np.stack([_phantom(N, w, 1.5, 6, 1e5, seed=80 + s) for s in range(S)])
+    bad = good.copy()
+    _dead_pixels(bad, w, N, 81)
+    rec = ca.CTrec(w, N, ca.ABS, "HANN")
+    eager = {k: rec.reconstruct(P, [1.5] * S)[1] for k, P in (("good", good), ("bad", bad))}
+    W = oracle.filter_window(*ca.parse_filter("HANN"), w, ca.ABS)
+    f1 = oracle.prepare(bad[1], W, ca.ABS)
+    ref_rows = oracle.bp_ctas(f1, oracle.angles(N), 1.5, rec.rec_cof())
+    assert rel_rms(eager["bad"][1], ref_rows) <= 1e-4
+    d_in = torch.from_numpy(good).cuda()
+    d_out = torch.zeros((S, w, w), dtype=torch.float32, device="cuda")
+    rec.graph_capture(d_in, d_out, [1.5] * S)
+    for k in ("good", "bad", "bad", "good", "bad"):
+        d_in.copy_(torch.from_numpy(good if k == "good" else bad))
+        d_out.zero_()
+        torch.cuda.synchronize()
+        rec.graph_launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(d_out.cpu().numpy(), eager[k]), k
+        assert (rec.status(S) == 0).all()
+
+
+def test_deabs_clamp_in_uneven_batch(torch_cuda):
+    """An uneven batch of four line lengths, dead pixels in the items at
+    1024-, 4096- and 8192-point lines and none at 2048 points or in the
+    second 4096-point item: every slice equals the one its own handle
+    reconstructs."""
+    torch = torch_cuda
+    shapes = [(45, 300), (30, 2048), (24, 4096), (45, 700), (30, 1500)]
+    items, refs = [], []
+    for k, (N, w) in enumerate(shapes):
+        P = _phantom(N, w, 0.5, 6, 1e5, seed=90 + k)
+        if k < 3:
+            P[N // 2, w // 4] = 0.0
+            P[0, 0] = -1.0
+        d_in = torch.from_numpy(P).cuda()
+        d_out = torch.zeros((w, w), dtype=torch.float32, device="cuda")
+        items.append((d_in, d_out, 0.5))
+        refs.append(ca.CTrec(w, N, ca.ABS, "RAMP").reconstruct(P, 0.5)[1][0])
+    ca.batch_uneven(items, ca.ABS, 1, 0.0, 180.0, 0)
+    torch.cuda.synchronize()
+    for k, ((d_in, d_out, c), ref) in enumerate(zip(items, refs)):
+        assert np.array_equal(d_out.cpu().numpy(), ref), k
+
+
 def test_bad_absorption_data():
     rec = ca.CTrec(64, 16, ca.ABS, "RAMP")
     with pytest.raises(ca.CtasError) as e:
