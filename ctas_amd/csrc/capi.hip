@@ -70,7 +70,7 @@ int ensure_slices(ctas_fbp *h, int S) {
   h->d_minmax = nullptr;
   h->d_status = nullptr;
   h->mm_cap = 0;
-  if (hipMalloc(&h->d_minmax, sizeof(float2) * kMinMaxBlocks * std::max(S, 1)) != hipSuccess ||
+  if (hipMalloc(&h->d_minmax, sizeof(float2) * kMinMaxStride * std::max(S, 1)) != hipSuccess ||
       hipMalloc(&h->d_status, sizeof(int) * std::max(S, 1)) != hipSuccess)
     return fail(CTAS_E_NOMEM, "device allocation of per-slice status failed");
   h->mm_cap = S;
@@ -246,7 +246,7 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
     it.pad = pad;
     it.filt_f32 = d_f32 ? d_f32 + slice_elems * i : nullptr;
     it.out = d_out ? d_out + static_cast<size_t>(h->w) * h->w * i : nullptr;
-    it.minmax = h->d_minmax + static_cast<size_t>(kMinMaxBlocks) * i;
+    it.minmax = h->d_minmax + static_cast<size_t>(kMinMaxStride) * i;
     it.status = h->d_status + i;
     it.center = do_bp ? centers[i] : 0.0f;
     it.tile_base = it.tiles * i;
@@ -256,7 +256,6 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   bool changed = false;
   if (int rc = upload_items(h, items, s, &changed, extra(h).resident)) return rc;
   if (pad) CTAS_HIP_TRY(launch_pad_zero(h->d_items, S, s));
-  if (h->contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(h->d_items, S, s));
   CTAS_HIP_TRY(launch_filter(h->d_items, S, pairs * S, h->logZ, h->logZ, h->contrast, st, s));
   if (do_bp)
     CTAS_HIP_TRY(launch_backproject(h->d_items, S, items[0].tiles * S, kind, choose_ac(h),
@@ -975,14 +974,14 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   BatchWork &bw = g_batch[dev];
   CTAS_HIP_TRY(hipStreamSynchronize(s));
   if (int rc = grow(bw.d_filt, bw.filt_cap, filt_bytes)) return rc;
-  if (int rc = grow(bw.d_mm, bw.mm_cap, sizeof(float2) * kMinMaxBlocks * n)) return rc;
+  if (int rc = grow(bw.d_mm, bw.mm_cap, sizeof(float2) * kMinMaxStride * n)) return rc;
   if (int rc = grow(bw.d_status, bw.st_cap, sizeof(int) * n)) return rc;
   if (int rc = grow(bw.d_items, bw.it_cap, sizeof(Item) * n)) return rc;
   if (int rc = grow(bw.d_items_f, bw.itf_cap, sizeof(Item) * n)) return rc;
   if (int rc = grow(bw.d_counter, bw.ctr_cap, sizeof(unsigned) * 64)) return rc;
   for (int oi = 0; oi < n; ++oi) {
     its[oi].filt = static_cast<unsigned char *>(bw.d_filt) + filt_off[oi];
-    its[oi].minmax = bw.d_mm + static_cast<size_t>(kMinMaxBlocks) * oi;
+    its[oi].minmax = bw.d_mm + static_cast<size_t>(kMinMaxStride) * oi;
     its[oi].status = bw.d_status + oi;
   }
   // The filter runs once per line length, each with its own kernel (the
@@ -1003,7 +1002,6 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   CTAS_HIP_TRY(hipMemcpy(bw.d_items, its.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
   CTAS_HIP_TRY(hipMemcpy(bw.d_items_f, fits.data(), sizeof(Item) * n, hipMemcpyHostToDevice));
   if (st != ST_F32) CTAS_HIP_TRY(launch_pad_zero(bw.d_items, n, s));
-  if (contrast == CTAS_CONTRAST_ABS) CTAS_HIP_TRY(launch_minmax(bw.d_items, n, s));
   for (const auto &g : groups)
     CTAS_HIP_TRY(launch_filter(bw.d_items_f + std::get<0>(g), std::get<1>(g), std::get<2>(g), std::get<3>(g),
                                std::get<3>(g), contrast, st, s));

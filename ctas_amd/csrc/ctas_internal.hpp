@@ -101,7 +101,7 @@ struct Item {
                            // rounded once (LINEAR and FP16/BF16 modes)
   const float *W;         // [Z] window
   const float2 *tw;       // [Z/2] twiddles exp(-2 pi i m / Z)
-  float2 *minmax;         // [kMinMaxBlocks] partial (min, max) of the raw sinogram
+  float2 *minmax;         // [kMinMaxStride]: kMinMaxBlocks partial (min, max) of the raw sinogram, then deAbs flags
   int *status;            // deAbs status (0 / CTAS_E_DATA)
   float center;
   float recCof;
@@ -125,6 +125,7 @@ struct Item {
 inline int filt_pad(int w) { return (3 * w) / 2 + 256; }
 
 constexpr int kMinMaxBlocks = 64;
+constexpr int kMinMaxStride = kMinMaxBlocks + 1;  // float2 entries per item
 
 enum Store { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
 enum Mode { MODE_CTAS = 0, MODE_LINEAR = 1 };
@@ -142,11 +143,10 @@ inline int tile_rows(int kind) { return kind == TK_64x64 ? 64 : 128; }
 inline int tile_cols(int kind) { return kind == TK_128x128 ? 128 : 64; }
 
 // ---- launchers (filter.hip, backproject.hip) ------------------------------
-// Per-item (min, max) partials of the raw sinograms (ABS contrast only).
-hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s);
 // Zeros the pads around the filtered sinograms of the items (Item::pad).
 hipError_t launch_pad_zero(const Item *d_items, int n_items, hipStream_t s);
-// deAbs + padded FFT filtering + REF prefix sum for all row pairs.
+// deAbs + padded FFT filtering + REF prefix sum for all row pairs; with ABS
+// contrast it also sets every item's status (0 / CTAS_E_DATA).
 // min_logZ == max_logZ == 12 (every line 4096 points) takes the radix-16 kernel.
 hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int min_logZ, int max_logZ,
                          int contrast, int store, hipStream_t s);

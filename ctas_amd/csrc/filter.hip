@@ -13,6 +13,7 @@
 // their bit-reversed positions. HBM traffic per row: w*4 bytes in, w*s out.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 
@@ -77,9 +78,101 @@ __device__ __forceinline__ int find_item_by_pair(const Item *items, int n, int b
   return lo;
 }
 
-// Partial (min, max) of every raw sinogram: grid (kMinMaxBlocks, n_items).
+// deAbs (physical.world.cpp:88-100): when a sinogram's minimum is not
+// positive, every sample at or below max/1e6 is raised to that value; then
+// -log. Absorption data is positive as a rule, so the filter runs first
+// without the clamp and notes the sinograms that hold a sample <= 0 (the
+// minimum's test, sample by sample: every sample is loaded by some row
+// pair, and NaN passes both tests alike); only for those the (min, max)
+// partials are formed and their row pairs filtered again with the clamp.
+// Two ints follow each item's kMinMaxBlocks partials: [0] the item holds a
+// sample <= 0, [1] some item of the launch does (kept by the launch's
+// first item).
+__device__ __forceinline__ int *deabs_flags(const Item &it) {
+  return reinterpret_cast<int *>(it.minmax + kMinMaxBlocks);
+}
+
+struct DeAbs {
+  bool absorb;  // ABS contrast: -log of the raw samples
+  bool note;    // first pass: no clamp, note samples <= 0
+  bool clamp;   // second pass of a sinogram whose minimum is not positive
+  float mina;   // max/1e6
+};
+
+__device__ __forceinline__ float deabs_apply(const DeAbs &d, float v) {
+  if (d.clamp && v <= d.mina) v = d.mina;
+  return -__logf(v);
+}
+
+// First pass: lo is the least raw sample the thread loaded (all of them
+// samples of the item's rows).
+__device__ __forceinline__ void deabs_note(const Item *__restrict__ items, const Item &it, float lo) {
+  if (lo <= 0.0f) {
+    deabs_flags(it)[0] = 1;
+    deabs_flags(items[0])[1] = 1;
+  }
+}
+
+// Second pass: the clamp rule from the item's partials. Every wave reduces
+// the 64 partial extrema itself (one per lane), so no LDS is needed.
+__device__ __forceinline__ DeAbs deabs_rule(const Item &it, int pair, int tid) {
+  const float2 m = it.minmax[tid & 63];
+  float mn = m.x, mx = m.y;
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = fminf(mn, __shfl_xor(mn, o));
+    mx = fmaxf(mx, __shfl_xor(mx, o));
+  }
+  const DeAbs d{true, false, mn <= 0.0f, static_cast<float>(static_cast<double>(mx) / 1000000.0)};
+  if (pair == 0 && tid == 0 && d.clamp && d.mina <= 0.0f) *it.status = CTAS_E_DATA;
+  return d;
+}
+
+// Clears the flags and the status of every item of a launch.
+__global__ __launch_bounds__(256) void deabs_reset_kernel(const Item *__restrict__ items, int n_items) {
+  for (int i = threadIdx.x; i < n_items; i += blockDim.x) {
+    int *f = deabs_flags(items[i]);
+    f[0] = 0;
+    f[1] = 0;
+    *items[i].status = 0;
+  }
+}
+
+// Row pairs of a filter launch. The first pass takes the pair of its
+// workgroup; the second walks, on a small grid, the pairs of the items
+// noted by the first (skipping the others item by item) and ends at once
+// when there are none.
+template <bool FIX, class F>
+__device__ __forceinline__ void for_pairs(const Item *__restrict__ items, int n_items, int total_pairs,
+                                          int contrast, F &&f) {
+  const int tid = threadIdx.x;
+  if (!FIX) {
+    const int b = blockIdx.x;
+    const Item &it = items[find_item_by_pair(items, n_items, b)];
+    const bool absorb = contrast == CTAS_CONTRAST_ABS;
+    f(it, b - it.pair_base, DeAbs{absorb, absorb, false, 0.0f});
+    return;
+  }
+  if (deabs_flags(items[0])[1] == 0) return;
+  const int G = gridDim.x;
+  for (int b = blockIdx.x; b < total_pairs; b += G) {
+    const Item &it = items[find_item_by_pair(items, n_items, b)];
+    if (deabs_flags(it)[0] == 0) {
+      const int end = it.pair_base + (it.N + 1) / 2;  // first pair of the next item
+      b += ((end - 1 - b) / G) * G;
+      continue;
+    }
+    const int pair = b - it.pair_base;
+    const DeAbs d = deabs_rule(it, pair, tid);
+    __syncthreads();  // the previous pair's last LDS reads
+    f(it, pair, d);
+  }
+}
+
+// Partial (min, max) of the raw sinograms noted by the first filter pass:
+// grid (kMinMaxBlocks, n_items).
 __global__ __launch_bounds__(256) void minmax_kernel(const Item *__restrict__ items) {
   const Item &it = items[blockIdx.y];
+  if (deabs_flags(it)[0] == 0) return;
   const float *p = it.sino;
   const size_t n = static_cast<size_t>(it.N) * it.w;
   float mn = FLT_MAX, mx = -FLT_MAX;
@@ -120,46 +213,21 @@ __global__ __launch_bounds__(256) void minmax_kernel(const Item *__restrict__ it
   }
 }
 
+// One row pair on a line of any length in LDS (buf: the complex line).
 template <int ST>
-__global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ items, int n_items,
-                                                     int contrast) {
-  // The only LDS is the complex line (Z x 8 bytes): 128 KiB at Z = 16384.
-  extern __shared__ float2 buf[];
-
-  const int b = blockIdx.x;
-  const Item &it = items[find_item_by_pair(items, n_items, b)];
-  const int pair = b - it.pair_base;
+__device__ __forceinline__ void filter_pair(const Item *__restrict__ items, const Item &it, int pair, float2 *buf,
+                                            int contrast, const DeAbs &d) {
   const int w = it.w, N = it.N, Z = it.Z, logZ = it.logZ;
   const int psh = line_psh(Z);
   const int s0 = (Z - w) >> 1;
   const int t0 = 2 * pair, t1 = t0 + 1;
   const bool has1 = t1 < N;
   const float *__restrict__ r0 = it.sino + static_cast<size_t>(t0) * w;
-  const float *__restrict__ r1 = r0 + w;
   const float2 *__restrict__ tw = it.tw;
   const int tid = threadIdx.x;
-
-  // deAbs (physical.world.cpp:88-100): clamp at max/1e6 only when the
-  // sinogram's minimum is not positive, then -log. Every wave reduces the 64
-  // partial extrema itself (one per lane), so no LDS is needed.
-  const bool absorb = (contrast == CTAS_CONTRAST_ABS);
-  bool clamp = false;
-  float mina = 0.0f;
-  if (absorb) {
-    const float2 m = it.minmax[tid & 63];
-    float mn = m.x, mx = m.y;
-    for (int o = 32; o > 0; o >>= 1) {
-      mn = fminf(mn, __shfl_xor(mn, o));
-      mx = fmaxf(mx, __shfl_xor(mx, o));
-    }
-    clamp = (mn <= 0.0f);
-    mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
-    if (pair == 0 && tid == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
-  }
-  auto deabs = [&](float v) {
-    if (clamp && v <= mina) v = mina;
-    return -__logf(v);
-  };
+  const bool absorb = d.absorb;
+  auto deabs = [&](float v) { return deabs_apply(d, v); };
+  float lo = FLT_MAX;
 
   // Padded line z[n], n in [0, Z): row samples at s0..s0+w-1 and, from the
   // inclusive copy range of lct.cpp:580-584, the next row's first sample at
@@ -167,7 +235,7 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
   // The points t + 256 k of each thread are loaded in chunks of 32 (two for
   // Z = 16384), all loads of a chunk issued before any is used, at indices
   // clamped into the rows (samples outside are zeroed after).
-  const float *r1c = has1 ? r1 : r0;
+  const float *r1c = has1 ? r0 + w : r0;
   const int per = (Z + 255) >> 8;  // points per thread (the last partial below 256 points)
   for (int k0 = 0; k0 < per; k0 += 32) {
     float ra[32], rb[32];
@@ -189,12 +257,14 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
         const int n = tid + 256 * kk, x = n - s0;
         const bool in = x >= 0 && x < w;
         const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1 + 1 < N));
+        lo = fminf(lo, fminf(ra[k], rb[k]));
         const float a = absorb ? deabs(ra[k]) : ra[k], c = absorb ? deabs(rb[k]) : rb[k];
         buf[lds_pad(__brev(static_cast<unsigned>(n)) >> (32 - logZ), psh)] =
             make_float2(in0 ? a : 0.0f, in1 ? c : 0.0f);
       }
     }
   }
+  if (d.note) deabs_note(items, it, lo);
   __syncthreads();
 
   line_filter(buf, tw, it.W, Z, logZ, tid, psh);
@@ -229,6 +299,16 @@ __global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ it
       if (it.filt_f32) it.filt_f32[rowb + x] = run;
     }
   }
+}
+
+template <int ST, bool FIX>
+__global__ __launch_bounds__(256) void filter_kernel(const Item *__restrict__ items, int n_items, int total_pairs,
+                                                     int contrast) {
+  // The only LDS is the complex line (Z x 8 bytes): 128 KiB at Z = 16384.
+  extern __shared__ float2 buf[];
+  for_pairs<FIX>(items, n_items, total_pairs, contrast, [&](const Item &it, int pair, const DeAbs &d) {
+    filter_pair<ST>(items, it, pair, buf, contrast, d);
+  });
 }
 
 // ---- Z = 4096: radix-16 passes in registers --------------------------------
@@ -283,34 +363,22 @@ __device__ __forceinline__ void load16(const Item &it, int pair, int tid, Raw16 
 
 // One row pair: deAbs, the 4096-point transform, window, inverse, stores.
 template <int ST>
-__device__ __forceinline__ void filter16_pair(const Item &it, int pair, const Raw16 &r, float2 *xch, int contrast,
-                                              int tid) {
+__device__ __forceinline__ void filter16_pair(const Item *__restrict__ items, const Item &it, int pair,
+                                              const Raw16 &r, float2 *xch, int contrast, const DeAbs &d, int tid) {
   constexpr int Z = 4096;
   auto at = [](int row, int col) { return row * 16 + (col ^ (row & 15)); };
   const int w = it.w, N = it.N;  // w <= 2048: zidth(w) = 4096
   const int t0 = 2 * pair, t1 = t0 + 1;
   const bool has1 = t1 < N;
   const float2 *__restrict__ tw = it.tw;
-
-  // deAbs (physical.world.cpp:88-100), as in filter_kernel.
-  const bool absorb = (contrast == CTAS_CONTRAST_ABS);
-  bool clamp = false;
-  float mina = 0.0f;
-  if (absorb) {
-    const float2 m = it.minmax[tid & 63];
-    float mn = m.x, mx = m.y;
-    for (int o = 32; o > 0; o >>= 1) {
-      mn = fminf(mn, __shfl_xor(mn, o));
-      mx = fmaxf(mx, __shfl_xor(mx, o));
-    }
-    clamp = (mn <= 0.0f);
-    mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
-    if (pair == 0 && tid == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
+  const bool absorb = d.absorb;
+  auto deabs = [&](float v) { return deabs_apply(d, v); };
+  if (d.note) {
+    float lo = fminf(r.ea, r.ec);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lo = fminf(lo, fminf(r.a[j], r.c[j]));
+    deabs_note(items, it, lo);
   }
-  auto deabs = [&](float v) {
-    if (clamp && v <= mina) v = mina;
-    return -__logf(v);
-  };
 
   float2 v[16];
 #pragma unroll
@@ -419,10 +487,10 @@ __device__ __forceinline__ void filter16_pair(const Item &it, int pair, const Ra
   }
 }
 
-// The 4096-point filter: one row pair per workgroup, its loads issued
-// before any is used.
-template <int ST>
-__global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ items, int n_items,
+// The 4096-point filter: one row pair per workgroup in the first pass, its
+// loads issued before any is used.
+template <int ST, bool FIX>
+__global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ items, int n_items, int total_pairs,
                                                        int contrast) {
   // Exchange buffer of 256 rows of 16 points, column c of row r stored at
   // r*16 + (c ^ (r % 16)): every access pattern of the transposes spreads
@@ -430,11 +498,11 @@ __global__ __launch_bounds__(256) void filter16_kernel(const Item *__restrict__ 
   // line itself.
   __shared__ float2 xch[256 * 16];
   const int tid = threadIdx.x;
-  const Item &it = items[find_item_by_pair(items, n_items, blockIdx.x)];
-  const int pair = blockIdx.x - it.pair_base;
-  Raw16 r;
-  load16(it, pair, tid, r);
-  filter16_pair<ST>(it, pair, r, xch, contrast, tid);
+  for_pairs<FIX>(items, n_items, total_pairs, contrast, [&](const Item &it, int pair, const DeAbs &d) {
+    Raw16 r;
+    load16(it, pair, tid, r);
+    filter16_pair<ST>(items, it, pair, r, xch, contrast, d, tid);
+  });
 }
 
 // ---- Z = 8192: 32 x 16 x 16 in registers --------------------------------------
@@ -451,13 +519,9 @@ constexpr int kPA8 = 33, kPB8 = 17;
 constexpr size_t kFilter8192Lds = sizeof(float2) * 512 * kPB8;
 
 template <int ST>
-__global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict__ items, int n_items,
-                                                         int contrast) {
+__device__ __forceinline__ void filter8192_pair(const Item *__restrict__ items, const Item &it, int pair, float2 *xb,
+                                                int contrast, const DeAbs &d) {
   constexpr int Z = 8192;
-  extern __shared__ float2 xb[];
-  const int b = blockIdx.x;
-  const Item &it = items[find_item_by_pair(items, n_items, b)];
-  const int pair = b - it.pair_base;
   const int w = it.w, N = it.N;  // w <= 4096: zidth(w) = 8192
   const int t0 = 2 * pair, t1r = t0 + 1;
   const bool has1 = t1r < N;
@@ -465,26 +529,8 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
   const float *__restrict__ r1 = r0 + w;
   const float2 *__restrict__ tw = it.tw;
   const int tid = threadIdx.x;
-
-  // deAbs (physical.world.cpp:88-100), as in filter_kernel.
-  const bool absorb = (contrast == CTAS_CONTRAST_ABS);
-  bool clamp = false;
-  float mina = 0.0f;
-  if (absorb) {
-    const float2 m = it.minmax[tid & 63];
-    float mn = m.x, mx = m.y;
-    for (int o = 32; o > 0; o >>= 1) {
-      mn = fminf(mn, __shfl_xor(mn, o));
-      mx = fmaxf(mx, __shfl_xor(mx, o));
-    }
-    clamp = (mn <= 0.0f);
-    mina = static_cast<float>(static_cast<double>(mx) / 1000000.0);
-    if (pair == 0 && tid == 0) *it.status = (clamp && mina <= 0.0f) ? CTAS_E_DATA : 0;
-  }
-  auto deabs = [&](float v) {
-    if (clamp && v <= mina) v = mina;
-    return -__logf(v);
-  };
+  const bool absorb = d.absorb;
+  auto deabs = [&](float v) { return deabs_apply(d, v); };
 
   // Padded line, circularly shifted as in filter16_kernel: the row at
   // n = 0..w-1 and the next row's first sample at n = w (lct.cpp:580-584;
@@ -500,6 +546,12 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
     const bool in = x < w;
     const bool in0 = in || (x == w && t0 + 1 < N), in1 = has1 && (in || (x == w && t1r + 1 < N));
     v[j] = make_float2(gload(r0, in0 ? x : 0), gload(r1c, in1 ? x : 0));
+  }
+  if (d.note) {
+    float lo = FLT_MAX;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) lo = fminf(lo, fminf(v[j].x, v[j].y));
+    deabs_note(items, it, lo);
   }
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
@@ -622,6 +674,15 @@ __global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict_
   }
 }
 
+template <int ST, bool FIX>
+__global__ __launch_bounds__(256) void filter8192_kernel(const Item *__restrict__ items, int n_items, int total_pairs,
+                                                         int contrast) {
+  extern __shared__ float2 xb[];
+  for_pairs<FIX>(items, n_items, total_pairs, contrast, [&](const Item &it, int pair, const DeAbs &d) {
+    filter8192_pair<ST>(items, it, pair, xb, contrast, d);
+  });
+}
+
 // FLT contrast: no filtering, only the storage conversion (and float copy).
 template <int ST>
 __global__ __launch_bounds__(256) void passthrough_kernel(const Item *__restrict__ items,
@@ -659,11 +720,52 @@ hipError_t launch_pad_zero(const Item *d_items, int n_items, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_minmax(const Item *d_items, int n_items, hipStream_t s) {
-  dim3 grid(kMinMaxBlocks, n_items);
-  hipLaunchKernelGGL(minmax_kernel, grid, dim3(256), 0, s, d_items);
+namespace {
+
+// Workgroups of the second filter pass (the pairs of noted items only).
+constexpr int kFixBlocks = 1024;
+
+// One filter pass: the first takes one workgroup per row pair, the second
+// kFixBlocks workgroups walking the pairs of the noted items. Both choose the
+// same kernel for the same lines.
+template <bool FIX>
+hipError_t filter_pass(const Item *d_items, int n_items, int total_pairs, int min_logZ, int max_logZ,
+                       int contrast, int store, hipStream_t s) {
+  const dim3 grid(FIX ? std::min(total_pairs, kFixBlocks) : total_pairs), block(256);
+  if (min_logZ == 12 && max_logZ == 12) {
+    switch (store) {
+      case ST_F32: hipLaunchKernelGGL((filter16_kernel<ST_F32, FIX>), grid, block, 0, s, d_items, n_items, total_pairs, contrast); break;
+      case ST_F16: hipLaunchKernelGGL((filter16_kernel<ST_F16, FIX>), grid, block, 0, s, d_items, n_items, total_pairs, contrast); break;
+      default: hipLaunchKernelGGL((filter16_kernel<ST_BF16, FIX>), grid, block, 0, s, d_items, n_items, total_pairs, contrast); break;
+    }
+    return hipGetLastError();
+  }
+  // The 8192-point register kernel's exchange buffer must fit the device's
+  // LDS per workgroup; otherwise the radix-2 LDS kernel filters these lines.
+  int dev = 0, lds_max = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return hipErrorInvalidDevice;
+  if (min_logZ == 13 && max_logZ == 13 && kFilter8192Lds <= static_cast<size_t>(lds_max)) {
+    switch (store) {
+      case ST_F32: hipLaunchKernelGGL((filter8192_kernel<ST_F32, FIX>), grid, block, kFilter8192Lds, s, d_items, n_items, total_pairs, contrast); break;
+      case ST_F16: hipLaunchKernelGGL((filter8192_kernel<ST_F16, FIX>), grid, block, kFilter8192Lds, s, d_items, n_items, total_pairs, contrast); break;
+      default: hipLaunchKernelGGL((filter8192_kernel<ST_BF16, FIX>), grid, block, kFilter8192Lds, s, d_items, n_items, total_pairs, contrast); break;
+    }
+    return hipGetLastError();
+  }
+  // a padded line of any item fits in that of the largest
+  const size_t lds = sizeof(float2) * padded_len(1 << max_logZ, line_psh(1 << max_logZ));
+  if (lds > static_cast<size_t>(lds_max)) return hipErrorInvalidValue;
+  switch (store) {
+    case ST_F32: hipLaunchKernelGGL((filter_kernel<ST_F32, FIX>), grid, block, lds, s, d_items, n_items, total_pairs, contrast); break;
+    case ST_F16: hipLaunchKernelGGL((filter_kernel<ST_F16, FIX>), grid, block, lds, s, d_items, n_items, total_pairs, contrast); break;
+    default: hipLaunchKernelGGL((filter_kernel<ST_BF16, FIX>), grid, block, lds, s, d_items, n_items, total_pairs, contrast); break;
+  }
   return hipGetLastError();
 }
+
+}  // namespace
 
 hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int min_logZ, int max_logZ,
                          int contrast, int store, hipStream_t s) {
@@ -676,37 +778,18 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
     }
     return hipGetLastError();
   }
-  if (min_logZ == 12 && max_logZ == 12) {
-    switch (store) {
-      case ST_F32: hipLaunchKernelGGL(filter16_kernel<ST_F32>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
-      case ST_F16: hipLaunchKernelGGL(filter16_kernel<ST_F16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
-      default: hipLaunchKernelGGL(filter16_kernel<ST_BF16>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items, contrast); break;
-    }
-    return hipGetLastError();
+  const bool absorb = contrast == CTAS_CONTRAST_ABS;
+  if (absorb) {
+    hipLaunchKernelGGL(deabs_reset_kernel, dim3(1), dim3(256), 0, s, d_items, n_items);
+    if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
   }
-  // The 8192-point register kernel's exchange buffer must fit the device's
-  // LDS per workgroup; otherwise the radix-2 LDS kernel filters these lines.
-  int dev = 0, lds_max = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-    return hipErrorInvalidDevice;
-  if (min_logZ == 13 && max_logZ == 13 && kFilter8192Lds <= static_cast<size_t>(lds_max)) {
-    switch (store) {
-      case ST_F32: hipLaunchKernelGGL(filter8192_kernel<ST_F32>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
-      case ST_F16: hipLaunchKernelGGL(filter8192_kernel<ST_F16>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
-      default: hipLaunchKernelGGL(filter8192_kernel<ST_BF16>, dim3(total_pairs), dim3(256), kFilter8192Lds, s, d_items, n_items, contrast); break;
-    }
-    return hipGetLastError();
-  }
-  // a padded line of any item fits in that of the largest
-  const size_t lds = sizeof(float2) * padded_len(1 << max_logZ, line_psh(1 << max_logZ));
-  if (lds > static_cast<size_t>(lds_max)) return hipErrorInvalidValue;
-  switch (store) {
-    case ST_F32: hipLaunchKernelGGL(filter_kernel<ST_F32>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
-    case ST_F16: hipLaunchKernelGGL(filter_kernel<ST_F16>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
-    default: hipLaunchKernelGGL(filter_kernel<ST_BF16>, dim3(total_pairs), dim3(256), lds, s, d_items, n_items, contrast); break;
-  }
-  return hipGetLastError();
+  if (const hipError_t e = filter_pass<false>(d_items, n_items, total_pairs, min_logZ, max_logZ, contrast, store, s);
+      e != hipSuccess || !absorb)
+    return e;
+  // sinograms with a sample <= 0: their extrema, then their rows again with the clamp
+  hipLaunchKernelGGL(minmax_kernel, dim3(kMinMaxBlocks, n_items), dim3(256), 0, s, d_items);
+  if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+  return filter_pass<true>(d_items, n_items, total_pairs, min_logZ, max_logZ, contrast, store, s);
 }
 
 }  // namespace ctas
