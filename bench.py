@@ -3,7 +3,7 @@
 256 slices of 2048 px x 1800 angles over 180 deg, seeded 20-ellipse phantoms
 (semi-axes x4), Poisson noise at 1e5 photons/ray, ABS contrast, HANN filter
 padded to 4096, centre offset +3.37 px, output 2048^2 x 256 float32, run as
-one hipGraph-captured launch sequence (min/max -> filter -> backprojection).
+one hipGraph-captured launch sequence (deAbs + filter -> backprojection).
 
 One step = one reconstruction of the whole stack with the inputs already in
 HBM. Prints one JSON line. The roofline object prices the backprojection's
