@@ -722,46 +722,53 @@ hipError_t launch_pad_zero(const Item *d_items, int n_items, hipStream_t s) {
 
 namespace {
 
-// Workgroups of the second filter pass (the pairs of noted items only).
-constexpr int kFixBlocks = 1024;
+using FilterKernel = void (*)(const Item *, int, int, int);
 
-// One filter pass: the first takes one workgroup per row pair, the second
-// kFixBlocks workgroups walking the pairs of the noted items. Both choose the
-// same kernel for the same lines.
+template <template <int, bool> class K, bool FIX>
+struct Pick {
+  static FilterKernel of(int store) {
+    return store == ST_F32 ? K<ST_F32, FIX>::fn : store == ST_F16 ? K<ST_F16, FIX>::fn : K<ST_BF16, FIX>::fn;
+  }
+};
+template <int ST, bool FIX> struct K16 { static constexpr FilterKernel fn = filter16_kernel<ST, FIX>; };
+template <int ST, bool FIX> struct K8192 { static constexpr FilterKernel fn = filter8192_kernel<ST, FIX>; };
+template <int ST, bool FIX> struct KLds { static constexpr FilterKernel fn = filter_kernel<ST, FIX>; };
+
+// One filter pass: the first takes one workgroup per row pair, the second as
+// many workgroups as the device holds at once (one round), walking the pairs
+// of the noted items. Both choose the same kernel for the same lines.
 template <bool FIX>
 hipError_t filter_pass(const Item *d_items, int n_items, int total_pairs, int min_logZ, int max_logZ,
                        int contrast, int store, hipStream_t s) {
-  const dim3 grid(FIX ? std::min(total_pairs, kFixBlocks) : total_pairs), block(256);
-  if (min_logZ == 12 && max_logZ == 12) {
-    switch (store) {
-      case ST_F32: hipLaunchKernelGGL((filter16_kernel<ST_F32, FIX>), grid, block, 0, s, d_items, n_items, total_pairs, contrast); break;
-      case ST_F16: hipLaunchKernelGGL((filter16_kernel<ST_F16, FIX>), grid, block, 0, s, d_items, n_items, total_pairs, contrast); break;
-      default: hipLaunchKernelGGL((filter16_kernel<ST_BF16, FIX>), grid, block, 0, s, d_items, n_items, total_pairs, contrast); break;
-    }
-    return hipGetLastError();
-  }
-  // The 8192-point register kernel's exchange buffer must fit the device's
-  // LDS per workgroup; otherwise the radix-2 LDS kernel filters these lines.
   int dev = 0, lds_max = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     return hipErrorInvalidDevice;
-  if (min_logZ == 13 && max_logZ == 13 && kFilter8192Lds <= static_cast<size_t>(lds_max)) {
-    switch (store) {
-      case ST_F32: hipLaunchKernelGGL((filter8192_kernel<ST_F32, FIX>), grid, block, kFilter8192Lds, s, d_items, n_items, total_pairs, contrast); break;
-      case ST_F16: hipLaunchKernelGGL((filter8192_kernel<ST_F16, FIX>), grid, block, kFilter8192Lds, s, d_items, n_items, total_pairs, contrast); break;
-      default: hipLaunchKernelGGL((filter8192_kernel<ST_BF16, FIX>), grid, block, kFilter8192Lds, s, d_items, n_items, total_pairs, contrast); break;
-    }
-    return hipGetLastError();
+  FilterKernel k;
+  size_t lds = 0;
+  if (min_logZ == 12 && max_logZ == 12) {
+    k = Pick<K16, FIX>::of(store);
+  } else if (min_logZ == 13 && max_logZ == 13 && kFilter8192Lds <= static_cast<size_t>(lds_max)) {
+    // the 8192-point register kernel's exchange buffer fits the device's
+    // LDS per workgroup; otherwise the radix-2 LDS kernel filters these lines
+    k = Pick<K8192, FIX>::of(store);
+    lds = kFilter8192Lds;
+  } else {
+    // a padded line of any item fits in that of the largest
+    lds = sizeof(float2) * padded_len(1 << max_logZ, line_psh(1 << max_logZ));
+    if (lds > static_cast<size_t>(lds_max)) return hipErrorInvalidValue;
+    k = Pick<KLds, FIX>::of(store);
   }
-  // a padded line of any item fits in that of the largest
-  const size_t lds = sizeof(float2) * padded_len(1 << max_logZ, line_psh(1 << max_logZ));
-  if (lds > static_cast<size_t>(lds_max)) return hipErrorInvalidValue;
-  switch (store) {
-    case ST_F32: hipLaunchKernelGGL((filter_kernel<ST_F32, FIX>), grid, block, lds, s, d_items, n_items, total_pairs, contrast); break;
-    case ST_F16: hipLaunchKernelGGL((filter_kernel<ST_F16, FIX>), grid, block, lds, s, d_items, n_items, total_pairs, contrast); break;
-    default: hipLaunchKernelGGL((filter_kernel<ST_BF16, FIX>), grid, block, lds, s, d_items, n_items, total_pairs, contrast); break;
+  int grid = total_pairs;
+  if (FIX) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k), 256, lds) !=
+            hipSuccess)
+      return hipErrorInvalidDevice;
+    grid = std::max(1, std::min(total_pairs, std::max(per_cu, 1) * cus));
   }
+  hipLaunchKernelGGL(k, dim3(grid), dim3(256), lds, s, d_items, n_items, total_pairs, contrast);
   return hipGetLastError();
 }
 
