@@ -813,41 +813,55 @@ int ctas_fbp_autotune(ctas_fbp *h, int S) {
     if (d_in) (void)hipFree(d_in);
     return fail(CTAS_E_NOMEM, "autotune allocation failed");
   }
-  // Transmission-like data in (0.5, 1]: a constant is enough to time the
-  // gather, whose cost does not depend on the values.
-  std::vector<float> ones(slice_elems * S, 0.75f);
-  CTAS_HIP_TRY(hipMemcpy(d_in, ones.data(), sizeof(float) * ones.size(), hipMemcpyHostToDevice));
-  std::vector<float> centers(S, 0.0f);
-  hipEvent_t e0, e1;
-  CTAS_HIP_TRY(hipEventCreate(&e0));
-  CTAS_HIP_TRY(hipEventCreate(&e1));
+  // Every path below ends at the clean-up: buffers and events freed, the
+  // handle's own tile settings restored.
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   float best = 1e30f;
   int bk = TK_128x128, bac = 32;
-  const int kinds[] = {TK_64x64, TK_128x64, TK_128x128};
-  const int acs[] = {16, 32};
-  int rc = CTAS_OK;
-  for (int k : kinds) {
-    for (int ac : acs) {
-      h->tile_kind = k;
-      h->angles_per_pass = ac;
-      rc = enqueue_slices(h, d_in, d_o, nullptr, centers.data(), S, true, h->stream);
-      if (rc) break;
-      CTAS_HIP_TRY(hipEventRecord(e0, h->stream));
-      for (int r = 0; r < 3 && !rc; ++r) rc = enqueue_slices(h, d_in, d_o, nullptr, centers.data(), S, true, h->stream);
-      CTAS_HIP_TRY(hipEventRecord(e1, h->stream));
-      CTAS_HIP_TRY(hipEventSynchronize(e1));
-      float ms = 0.f;
-      CTAS_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        bk = k;
-        bac = ac;
+  auto hip = [](hipError_t e, const char *what) {
+    return e == hipSuccess ? CTAS_OK : fail(CTAS_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  auto run = [&]() -> int {
+    // Transmission-like data in (0.5, 1]: a constant is enough to time the
+    // gather, whose cost does not depend on the values.
+    std::vector<float> ones(slice_elems * S, 0.75f);
+    if (int rc = hip(hipMemcpy(d_in, ones.data(), sizeof(float) * ones.size(), hipMemcpyHostToDevice), "hipMemcpy"))
+      return rc;
+    if (int rc = hip(hipEventCreate(&e0), "hipEventCreate")) return rc;
+    if (int rc = hip(hipEventCreate(&e1), "hipEventCreate")) return rc;
+    std::vector<float> centers(S, 0.0f);
+    const int kinds[] = {TK_64x64, TK_128x64, TK_128x128};
+    const int acs[] = {16, 32};
+    for (int k : kinds) {
+      for (int ac : acs) {
+        h->tile_kind = k;
+        h->angles_per_pass = ac;
+        if (int rc = enqueue_slices(h, d_in, d_o, nullptr, centers.data(), S, true, h->stream)) return rc;
+        // the fastest of five runs: one slow run (a clock or cache state)
+        // must not decide between candidates a few percent apart
+        float ms = 1e30f;
+        for (int r = 0; r < 5; ++r) {
+          if (int rc = hip(hipEventRecord(e0, h->stream), "hipEventRecord")) return rc;
+          if (int rc = enqueue_slices(h, d_in, d_o, nullptr, centers.data(), S, true, h->stream)) return rc;
+          if (int rc = hip(hipEventRecord(e1, h->stream), "hipEventRecord")) return rc;
+          if (int rc = hip(hipEventSynchronize(e1), "hipEventSynchronize")) return rc;
+          float one = 0.f;
+          if (int rc = hip(hipEventElapsedTime(&one, e0, e1), "hipEventElapsedTime")) return rc;
+          ms = std::min(ms, one);
+        }
+        if (ms < best) {
+          best = ms;
+          bk = k;
+          bac = ac;
+        }
       }
     }
-    if (rc) break;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+    return CTAS_OK;
+  };
+  const int rc = run();
+  (void)hipStreamSynchronize(h->stream);  // nothing in flight reads the buffers
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   (void)hipFree(d_in);
   (void)hipFree(d_o);
   h->tile_kind = keep_kind;
