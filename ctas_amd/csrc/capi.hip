@@ -195,6 +195,7 @@ Item base_item(const ctas_fbp *h, int kind, bool mirror) {
   it.tiles_x = (cols + tile_cols(kind) - 1) / tile_cols(kind);
   it.tiles = it.tiles_x * ((h->w + tile_rows(kind) - 1) / tile_rows(kind));
   it.kind = kind;
+  it.half_max = mirror ? kHalfSumMax : kHalfMax;
   if (mirror) {
     it.slots = h->d_slots;
     it.slot_tab = h->d_slot_tab;
@@ -298,12 +299,23 @@ int enqueue_repeat(ctas_fbp *h, int slice, float *d_out, const float *centers, i
   return CTAS_OK;
 }
 
+// The status words carry deAbs (ABS contrast) and the FP16 range check (FP16
+// storage of the other contrasts); other handles have nothing to report.
+bool has_status(int contrast, int store) { return contrast == CTAS_CONTRAST_ABS || store == ST_F16; }
+
+int status_error(int code) {
+  if (code == CTAS_E_DATA) return fail(CTAS_E_DATA, "Bad absorption data (maximum is sub-zero).");
+  return fail(CTAS_E_RANGE, "a filtered sample exceeds the FP16 range of the backprojection (" +
+                                std::to_string(static_cast<int>(kHalfSumMax)) +
+                                " with mirrored angle pairs, else 65504): use BF16 or FP32 storage");
+}
+
 int read_status(ctas_fbp *h, int S) {
-  if (h->contrast != CTAS_CONTRAST_ABS) return CTAS_OK;
+  if (!has_status(h->contrast, store_of(h->flags))) return CTAS_OK;
   std::vector<int> st(S);
   CTAS_HIP_TRY(hipMemcpy(st.data(), h->d_status, sizeof(int) * S, hipMemcpyDeviceToHost));
   for (int i = 0; i < S; ++i)
-    if (st[i] != 0) return fail(CTAS_E_DATA, "Bad absorption data (maximum is sub-zero).");
+    if (st[i] != 0) return status_error(st[i]);
   return CTAS_OK;
 }
 
@@ -750,7 +762,7 @@ int ctas_fbp_backproject_dev(ctas_fbp *h, float *d_out, const float *centers, in
 int ctas_fbp_status(ctas_fbp *h, int S, int *status) {
   if (!h || !status || S <= 0 || S > h->mm_cap) return fail(CTAS_E_ARG, "bad arguments");
   CTAS_HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->contrast != CTAS_CONTRAST_ABS) {
+  if (!has_status(h->contrast, store_of(h->flags))) {
     for (int i = 0; i < S; ++i) status[i] = 0;
     return CTAS_OK;
   }
@@ -969,6 +981,7 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
     const int cols = e.mirror ? bi.w / 2 + 1 : bi.w;
     it.tiles_x = (cols + tile_cols(e.kind) - 1) / tile_cols(e.kind);
     it.tiles = it.tiles_x * ((bi.w + tile_rows(e.kind) - 1) / tile_rows(e.kind));
+    it.half_max = e.mirror ? kHalfSumMax : kHalfMax;
     if (e.mirror) {
       it.slots = e.g.d_slots;
       it.slot_tab = e.g.d_slot_tab;
@@ -1026,11 +1039,11 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
                                                bw.d_counter + l, cus, s));
   }
   CTAS_HIP_TRY(hipStreamSynchronize(s));
-  if (contrast == CTAS_CONTRAST_ABS) {
+  if (has_status(contrast, st)) {
     std::vector<int> stv(n);
     CTAS_HIP_TRY(hipMemcpy(stv.data(), bw.d_status, sizeof(int) * n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i)
-      if (stv[i]) return fail(CTAS_E_DATA, "Bad absorption data (maximum is sub-zero).");
+      if (stv[i]) return status_error(stv[i]);
   }
   return CTAS_OK;
 }

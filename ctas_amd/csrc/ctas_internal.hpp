@@ -115,6 +115,7 @@ struct Item {
   const MirrorSlot *slots; // mirrored launch: [n_slots]
   const int *slot_tab;     // mirrored launch: [n_slots][kSlotTab]
   int pad;                 // zero samples before and after filt (16-bit storage), 0 otherwise
+  float half_max;          // FP16 storage: largest filtered magnitude kept finite (kHalfMax / kHalfSumMax)
 };
 
 // Zero samples kept before and after each filtered sinogram of 16-bit
@@ -123,6 +124,11 @@ struct Item {
 // then reads inside the allocation, and its reads outside the sinogram
 // return 0 without a bounds check.
 inline int filt_pad(int w) { return (3 * w) / 2 + 256; }
+
+// FP16 storage: 65504 is the largest half; the mirrored kernel adds four
+// samples (two angle pairs) in half precision, so its samples must stay within
+// 65504 / 4. A finite filtered sample above the limit sets CTAS_E_RANGE.
+constexpr float kHalfMax = 65504.0f, kHalfSumMax = 16376.0f;
 
 constexpr int kMinMaxBlocks = 64;
 constexpr int kMinMaxStride = kMinMaxBlocks + 1;  // float2 entries per item

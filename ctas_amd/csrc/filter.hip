@@ -104,6 +104,22 @@ __device__ __forceinline__ float deabs_apply(const DeAbs &d, float v) {
   return -__logf(v);
 }
 
+// FP16 storage of PHS, REF and FLT data: the largest finite magnitude a
+// thread stores, against the item's half_max once the row pair is written
+// (CTAS_E_RANGE above it). -log and the filter keep ABS data below 1e3.
+template <int ST>
+struct RangeCheck {
+  float big = 0.0f;
+  __device__ __forceinline__ void see(float v) {
+    if constexpr (ST == ST_F16)
+      if (__builtin_isfinite(v)) big = fmaxf(big, fabsf(v));
+  }
+  __device__ __forceinline__ void done(const Item &it, int contrast) const {
+    if constexpr (ST == ST_F16)
+      if (contrast != CTAS_CONTRAST_ABS && big > it.half_max) *it.status = CTAS_E_RANGE;
+  }
+};
+
 // First pass: lo is the least raw sample the thread loaded (all of them
 // samples of the item's rows).
 __device__ __forceinline__ void deabs_note(const Item *__restrict__ items, const Item &it, float lo) {
@@ -127,7 +143,8 @@ __device__ __forceinline__ DeAbs deabs_rule(const Item &it, int pair, int tid) {
   return d;
 }
 
-// Clears the flags and the status of every item of a launch.
+// Clears the flags and the status of every item of a launch (every
+// contrast: the status also carries CTAS_E_RANGE of FP16 storage).
 __global__ __launch_bounds__(256) void deabs_reset_kernel(const Item *__restrict__ items, int n_items) {
   for (int i = threadIdx.x; i < n_items; i += blockDim.x) {
     int *f = deabs_flags(items[i]);
@@ -272,17 +289,21 @@ __device__ __forceinline__ void filter_pair(const Item *__restrict__ items, cons
   const float invZ = 1.0f / static_cast<float>(Z);  // exact: Z is a power of two
   const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
   const unsigned rsh = 32 - logZ;
+  RangeCheck<ST> range;
   if (contrast != CTAS_CONTRAST_REF) {
     for (int x = tid; x < w; x += 256) {
       const float2 v = buf[lds_pad(__brev(static_cast<unsigned>(s0 + x)) >> rsh, psh)];
       const float a = v.x * invZ, c = v.y * invZ;
       gstore<ST>(it.filt, row0 + x, a);
+      range.see(a);
       if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
         gstore<ST>(it.filt, row1 + x, c);
+        range.see(c);
         if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row1 + x, c);
       }
     }
+    range.done(it, contrast);
     return;
   }
   // REF (lct.cpp:590-595): prefix sum along each filtered row, one lane per
@@ -296,8 +317,10 @@ __device__ __forceinline__ void filter_pair(const Item *__restrict__ items, cons
       const float f = (im ? v.y : v.x) * invZ;
       run = (x == 0) ? f : run + f;
       store_val<ST>(it.filt, rowb + x, run);
+      range.see(run);
       if (it.filt_f32) it.filt_f32[rowb + x] = run;
     }
+    range.done(it, contrast);
   }
 }
 
@@ -452,6 +475,7 @@ __device__ __forceinline__ void filter16_pair(const Item *__restrict__ items, co
   dft16_out8<true>(v);  // the points t + 256 j, j < 8: n < 2048
 
   const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
+  RangeCheck<ST> range;
   if (contrast != CTAS_CONTRAST_REF) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -459,12 +483,15 @@ __device__ __forceinline__ void filter16_pair(const Item *__restrict__ items, co
       if (x >= w) continue;
       const float a = v[j].x, c = v[j].y;
       gstore<ST>(it.filt, row0 + x, a);
+      range.see(a);
       if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
         gstore<ST>(it.filt, row1 + x, c);
+        range.see(c);
         if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row1 + x, c);
       }
     }
+    range.done(it, contrast);
     return;
   }
   // REF (lct.cpp:590-595): the line in natural order through LDS, then the
@@ -482,8 +509,10 @@ __device__ __forceinline__ void filter16_pair(const Item *__restrict__ items, co
       const float f = im ? q.y : q.x;
       run = (x == 0) ? f : run + f;
       store_val<ST>(it.filt, rowb + x, run);
+      range.see(run);
       if (it.filt_f32) it.filt_f32[rowb + x] = run;
     }
+    range.done(it, contrast);
   }
 }
 
@@ -639,6 +668,7 @@ __device__ __forceinline__ void filter8192_pair(const Item *__restrict__ items, 
   dft32_out16<true>(v);  // the points t + 256 j, j < 16: n < 4096
 
   const size_t row0 = static_cast<size_t>(t0) * w, row1 = row0 + w;
+  RangeCheck<ST> range;
   if (contrast != CTAS_CONTRAST_REF) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -646,12 +676,15 @@ __device__ __forceinline__ void filter8192_pair(const Item *__restrict__ items, 
       if (x >= w) continue;
       const float a = v[j].x, c = v[j].y;
       gstore<ST>(it.filt, row0 + x, a);
+      range.see(a);
       if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row0 + x, a);
       if (has1) {
         gstore<ST>(it.filt, row1 + x, c);
+        range.see(c);
         if (it.filt_f32) gstore<ST_F32>(it.filt_f32, row1 + x, c);
       }
     }
+    range.done(it, contrast);
     return;
   }
   // REF (lct.cpp:590-595): the line in natural order through LDS, then the
@@ -669,8 +702,10 @@ __device__ __forceinline__ void filter8192_pair(const Item *__restrict__ items, 
       const float f = im ? q.y : q.x;
       run = (x == 0) ? f : run + f;
       gstore<ST>(it.filt, rowb + x, run);
+      range.see(run);
       if (it.filt_f32) it.filt_f32[rowb + x] = run;
     }
+    range.done(it, contrast);
   }
 }
 
@@ -694,12 +729,14 @@ __global__ __launch_bounds__(256) void passthrough_kernel(const Item *__restrict
   const int t0 = 2 * pair;
   const int rows = (t0 + 1 < N) ? 2 : 1;
   const size_t base = static_cast<size_t>(t0) * w;
+  RangeCheck<ST> range;
   for (int x = threadIdx.x; x < rows * w; x += 256) {
     const float v = it.sino[base + x];
     store_val<ST>(it.filt, base + x, v);
+    range.see(v);
     if (it.filt_f32) it.filt_f32[base + x] = v;
   }
-  if (pair == 0 && threadIdx.x == 0) *it.status = 0;
+  range.done(it, CTAS_CONTRAST_FLT);
 }
 
 // Zeros of the pads around one item's filtered sinogram (16-bit samples).
@@ -777,6 +814,8 @@ hipError_t filter_pass(const Item *d_items, int n_items, int total_pairs, int mi
 hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int min_logZ, int max_logZ,
                          int contrast, int store, hipStream_t s) {
   if (total_pairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(deabs_reset_kernel, dim3(1), dim3(256), 0, s, d_items, n_items);
+  if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
   if (contrast == CTAS_CONTRAST_FLT) {
     switch (store) {
       case ST_F32: hipLaunchKernelGGL(passthrough_kernel<ST_F32>, dim3(total_pairs), dim3(256), 0, s, d_items, n_items); break;
@@ -786,10 +825,6 @@ hipError_t launch_filter(const Item *d_items, int n_items, int total_pairs, int 
     return hipGetLastError();
   }
   const bool absorb = contrast == CTAS_CONTRAST_ABS;
-  if (absorb) {
-    hipLaunchKernelGGL(deabs_reset_kernel, dim3(1), dim3(256), 0, s, d_items, n_items);
-    if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
-  }
   if (const hipError_t e = filter_pass<false>(d_items, n_items, total_pairs, min_logZ, max_logZ, contrast, store, s);
       e != hipSuccess || !absorb)
     return e;

@@ -34,6 +34,10 @@ typedef struct ctas_fbp ctas_fbp;
 #define CTAS_E_STATE -7    /* call order (e.g. repeat before sino)           */
 #define CTAS_E_AXIS -8     /* axis search failed (lct.cpp:1136-1210 throws)  */
 #define CTAS_E_IO -9       /* a reader or writer callback reported failure   */
+#define CTAS_E_RANGE -10   /* FP16 storage: a filtered sample beyond 65504, or
+                            * beyond 16376 where the mirrored kernel adds four
+                            * samples in FP16 (no reference counterpart: the
+                            * reference has no FP16 storage)                 */
 
 /* contrast codes = Contrast::Cntype (physical.world.h:22-27) */
 #define CTAS_CONTRAST_ABS 0
@@ -141,7 +145,8 @@ int ctas_fbp_backproject_dev(ctas_fbp *h, float *d_out, const float *centers, in
 int ctas_fbp_graph_capture(ctas_fbp *h, const float *d_sino, float *d_out,
                            const float *centers, int S, void *stream);
 int ctas_fbp_graph_launch(ctas_fbp *h, void *stream);
-/* Last per-slice deAbs status of a _dev call (0 ok, CTAS_E_DATA bad data);
+/* Last per-slice status of a _dev call: 0 ok, CTAS_E_DATA (deAbs, ABS
+ * contrast) or CTAS_E_RANGE (FP16 storage of PHS, REF or FLT data);
  * synchronises the handle's stream. */
 int ctas_fbp_status(ctas_fbp *h, int S, int *status);
 

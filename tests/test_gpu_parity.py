@@ -385,6 +385,48 @@ def test_bad_absorption_data():
     assert e.value.code == ca.CTAS_E_DATA
 
 
+@pytest.mark.parametrize("contrast", [ca.FLT, ca.PHS, ca.REF])
+def test_fp16_range_is_reported(contrast, torch_cuda):
+    """FP16 storage keeps a filtered sample finite up to 65504, and the
+    mirrored kernel adds four samples in FP16, so there a sample must stay
+    within 16376: a larger finite one is reported (CTAS_E_RANGE) instead of
+    infinities in the slice. BF16 storage, and FP16 with LINEAR sampling
+    (samples staged as floats, limit 65504), take the same data."""
+    torch = torch_cuda
+    N, w = 180, 256
+    rng = np.random.default_rng(11)
+    P = rng.uniform(0.5, 1.0, (N, w)).astype(np.float32)
+    spike = np.zeros_like(P)
+    spike[40, 100] = 1.0
+    gain = np.abs(ca.CTrec(w, N, contrast, "RAMP").prepare(spike)).max()
+
+    def with_peak(peak):
+        Q = P.copy()
+        Q[40, 100] += peak / gain
+        return Q
+
+    hi, lo = with_peak(3e4), with_peak(1e4)
+    ref = ca.CTrec(w, N, contrast, "RAMP").reconstruct(hi, 0.0)[1][0]
+    with pytest.raises(ca.CtasError) as e:
+        ca.CTrec(w, N, contrast, "RAMP", store=ca.STORE_FP16).reconstruct(hi, 0.0)
+    assert e.value.code == ca.CTAS_E_RANGE
+    for store, interp, Q in ((ca.STORE_BF16, ca.INTERP_CTAS, hi), (ca.STORE_FP16, ca.INTERP_LINEAR, hi),
+                             (ca.STORE_FP16, ca.INTERP_CTAS, lo)):
+        out = ca.CTrec(w, N, contrast, "RAMP", store=store, interp=interp).reconstruct(Q, 0.0)[1][0]
+        assert np.isfinite(out).all(), (store, interp)
+        if Q is hi and interp == ca.INTERP_CTAS:
+            assert rel_rms(out, ref) <= 5e-3
+    # device path: the status names the slice
+    S = 3
+    d_in = torch.from_numpy(np.stack([lo, hi, lo])).cuda()
+    d_out = torch.zeros((S, w, w), dtype=torch.float32, device="cuda")
+    rec = ca.CTrec(w, N, contrast, "RAMP", store=ca.STORE_FP16)
+    rec.reconstruct_dev(d_in, d_out, [0.0] * S)
+    assert list(rec.status(S)) == [0, ca.CTAS_E_RANGE, 0]
+    rec.reconstruct_dev(d_in[:1], d_out[:1], [0.0])
+    assert list(rec.status(1)) == [0]
+
+
 def test_centre_out_of_range():
     rec = ca.CTrec(64, 16, ca.ABS, "RAMP")
     with pytest.raises(ca.CtasError) as e:
