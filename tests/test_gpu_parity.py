@@ -427,6 +427,25 @@ def test_fp16_range_is_reported(contrast, torch_cuda):
     assert list(rec.status(1)) == [0]
 
 
+def test_fp16_range_in_uneven_batch(torch_cuda):
+    """The uneven batch reports the FP16 range as the handles do: one FLT
+    item with a sample of 3e4 among clean items of other widths (all on the
+    mirrored kernel); BF16 storage takes the same batch."""
+    torch = torch_cuda
+    items = []
+    for k, (N, w) in enumerate([(180, 256), (90, 300), (120, 512)]):
+        P = np.random.default_rng(k).uniform(0.5, 1.0, (N, w)).astype(np.float32)
+        if k == 1:
+            P[10, 20] = 3e4
+        items.append((torch.from_numpy(P).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), 0.0))
+    with pytest.raises(ca.CtasError) as e:
+        ca.batch_uneven(items, ca.FLT, 1, 0.0, 180.0, ca.STORE_FP16)
+    assert e.value.code == ca.CTAS_E_RANGE
+    ca.batch_uneven(items, ca.FLT, 1, 0.0, 180.0, ca.STORE_BF16)
+    torch.cuda.synchronize()
+    assert all(torch.isfinite(o).all().item() for _, o, _ in items)
+
+
 def test_centre_out_of_range():
     rec = ca.CTrec(64, 16, ca.ABS, "RAMP")
     with pytest.raises(ca.CtasError) as e:
