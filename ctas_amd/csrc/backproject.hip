@@ -74,6 +74,14 @@ __device__ __forceinline__ float load_val(const void *base, size_t i) {
 template <int MODE> struct WinT { using type = float; };
 template <> struct WinT<MODE_LINEAR> { using type = float2; };
 
+// floor(x) as an int in one instruction (v_cvt_flr_i32_f32; floorf and a
+// conversion take two). x is a finite window coordinate.
+__device__ __forceinline__ int cvt_floor(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // True when every pixel of the block rows [r0, r0+nr) x cols [c0, c0+nc)
 // lies outside the reconstruction disc (radius at most hp + |c| around the
 // axis point (hp, hp + c)) or the column band, so that all are masked.
@@ -419,8 +427,11 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
           float ff[TC];
 #pragma unroll
           for (int q = 0; q < TC; ++q) {
+            // element floor(t) with weight t - floor(t): arcs past 180 degrees
+            // put in-mask pixels at t < 0 when c < 0, where truncation is not
+            // the floor that fract is taken from
             const float t = aj[r] + ci[q];
-            const int k = static_cast<int>(t);
+            const int k = cvt_floor(t);
             ff[q] = __builtin_amdgcn_fractf(t);
             vv[q] = *reinterpret_cast<const WT *>(wb + (k << 3));
           }
