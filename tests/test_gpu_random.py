@@ -1,0 +1,120 @@
+"""Seeded random configurations through the C ABI against the oracle.
+
+Each case draws a width (2 to 2049 px: every filter kernel's line lengths and
+the single-block widths), an angle count, an arc, a contrast, a filter and a
+centre offset; a third of the absorption cases carry dead pixels (zero or
+negative samples), which take deAbs's clamp. Three checks per case:
+
+* the filtered rows against the oracle's prepare_sino (lct.cpp:569-597),
+  within 2e-5 of their largest magnitude (float FFT against FP64);
+* the backprojection of the oracle's filtered rows (an FLT handle),
+  bit-identical to reconstructOnCPU (lct.cpp:688-729);
+* the end-to-end slice within 1e-4 relative RMS of the oracle's.
+"""
+import numpy as np
+import pytest
+
+import ctas_amd as ca
+import oracle
+from tests.conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+FILTERS = ["RAMP", "HANN", "HAMMING", "PARZEN", "KAISER:2.5", "GAUSS:0.4", "LANCKZOS", "BLACKMAN",
+           "WELCH", "NONE"]
+WIDTHS = [2, 3, 17, 64, 255, 256, 300, 511, 700, 1023, 1025, 1100, 2047, 2048, 2049]
+
+
+def _case(k):
+    rng = np.random.default_rng(7000 + k)
+    w = int(rng.choice(WIDTHS))
+    N = int(rng.integers(1, 160 if w > 1024 else 400))
+    arc = float(rng.choice([180.0, 360.0, 150.0, 200.0]))
+    contrast = [ca.ABS, ca.PHS, ca.REF][int(rng.integers(3))]
+    filt = FILTERS[int(rng.integers(len(FILTERS)))]
+    c = float(np.round(rng.uniform(-0.45, 0.45) * w * 8) / 8)
+    ell = oracle.random_ellipses(int(rng.integers(1, 12)), max(w, 8), 1.0, 7000 + k)
+    P = oracle.phantom_sino(N, w, c, ell, 1e5, 7000 + k)
+    if contrast == ca.ABS and k % 3 == 0:
+        n_dead = int(rng.integers(1, 6))
+        P[rng.integers(0, N, n_dead), rng.integers(0, w, n_dead)] = rng.choice([0.0, -1.0])
+    if contrast != ca.ABS:
+        P = -np.log(np.maximum(P, 1e-3)).astype(np.float32)
+    return w, N, arc, contrast, filt, c, np.ascontiguousarray(P, np.float32)
+
+
+@pytest.mark.parametrize("k", range(48))
+def test_random_configuration(k):
+    w, N, arc, contrast, filt, c, P = _case(k)
+    tag = f"w={w} N={N} arc={arc} contrast={contrast} {filt} c={c}"
+    rec = ca.CTrec(w, N, contrast, filt, anarc=arc)
+    f, p = ca.parse_filter(filt)
+    ref_f = oracle.prepare(P, oracle.filter_window(f, p, w, contrast), contrast)
+    got_f = rec.prepare(P)
+    scale = np.abs(ref_f).max()
+    assert np.abs(got_f - ref_f).max() <= 2e-5 * scale, tag
+
+    sc = oracle.angles(N, arc)
+    flt = ca.CTrec(w, N, ca.FLT, "RAMP", anarc=arc)
+    assert np.array_equal(flt.angles(), sc), tag
+    ref = oracle.bp_ctas(ref_f, sc, c)  # recCof 1: no physics set
+    assert rec.rec_cof() == 1.0 and flt.rec_cof() == 1.0
+    assert np.array_equal(flt.reconstruct(ref_f, c)[1][0], ref), tag
+
+    out = rec.reconstruct(P, c)[1][0]
+    if not ref.any():
+        assert not out.any(), tag
+    else:
+        assert rel_rms(out, ref) <= 1e-4, tag
+
+
+MODES = [(ca.STORE_FP16, ca.INTERP_CTAS), (ca.STORE_BF16, ca.INTERP_CTAS),
+         (ca.STORE_FP32, ca.INTERP_LINEAR), (ca.STORE_FP16, ca.INTERP_LINEAR)]
+
+
+@pytest.mark.parametrize("k", range(24))
+def test_random_precision_mode(k):
+    """The precision modes (FP64 angle table; FP16/BF16 storage with CTAS
+    sampling, LINEAR sampling from FP32 and FP16 storage) on random
+    configurations, against the FP64 oracle in the same sampling mode.
+
+    Bars. LINEAR: 2e-3 relative RMS from 16-bit storage; from FP32 storage
+    1e-4 scaled by max(1, (w + 2|c|) / 1024), as the float ray coordinates
+    reach w + 2|c| and carry that magnitude's rounding into the
+    interpolation. CTAS sampling: truncation moves a sample to the
+    neighbouring bin wherever a coordinate error straddles an integer, so
+    with few angles, wide rows and noisy filtered rows a slice can stand far
+    from FP64 whatever its storage: the reference's own float path
+    (reconstructOnCPU, lct.cpp:688-729, restated by the oracle) does in these
+    cases (7e-3 to 1e-1 measured). The 16-bit paths, whose 16.16 fixed-point
+    coordinates round the per-pixel steps to 2^-16 px (an error that grows
+    across a tile), must stay within 2e-3
+    or within twice that float path's distance from FP64, whichever is
+    larger; at the workloads' angle counts (1800, 3000) both are far below
+    2e-3. At least 90 angles."""
+    rng = np.random.default_rng(9000 + k)
+    w = int(rng.choice([64, 255, 256, 300, 511, 700, 1023, 1025, 1100, 2047, 2048, 2049]))
+    N = int(rng.integers(90, 200 if w > 1024 else 400))
+    arc = float(rng.choice([180.0, 360.0, 150.0, 200.0]))
+    contrast = [ca.ABS, ca.PHS, ca.REF][int(rng.integers(3))]
+    filt = FILTERS[int(rng.integers(len(FILTERS)))]
+    c = float(np.round(rng.uniform(-0.45, 0.45) * w * 8) / 8)
+    store, interp = MODES[k % len(MODES)]
+    ell = oracle.random_ellipses(int(rng.integers(1, 12)), w, 1.0, 9000 + k)
+    P = oracle.phantom_sino(N, w, c, ell, 1e5, 9000 + k)
+    if contrast != ca.ABS:
+        P = -np.log(np.maximum(P, 1e-3)).astype(np.float32)
+    tag = f"w={w} N={N} arc={arc} contrast={contrast} {filt} c={c} store={store:#x} interp={interp}"
+    f, p = ca.parse_filter(filt)
+    ref_f = oracle.prepare(P, oracle.filter_window(f, p, w, contrast), contrast)
+    linear = interp == ca.INTERP_LINEAR
+    ref = oracle.bp_f64(ref_f, c, arc * np.pi / 180.0, 1.0, linear=linear)
+    out = ca.CTrec(w, N, contrast, filt, anarc=arc, store=store, interp=interp).reconstruct(P, c)[1][0]
+    err = rel_rms(out, ref)
+    if linear:
+        bar = 2e-3 if store != ca.STORE_FP32 else 1e-4 * max(1.0, (w + 2 * abs(c)) / 1024)
+    else:
+        float_path = rel_rms(oracle.bp_ctas(ref_f, oracle.angles(N, arc), c), ref)
+        bar = max(2e-3, 2 * float_path)
+    print(tag, f"rel_rms={err:.3e} bar={bar:.3e}")
+    assert err <= bar, tag
