@@ -32,7 +32,7 @@ def _case(k):
     arc = float(rng.choice([180.0, 360.0, 150.0, 200.0]))
     contrast = [ca.ABS, ca.PHS, ca.REF][int(rng.integers(3))]
     filt = FILTERS[int(rng.integers(len(FILTERS)))]
-    c = float(np.round(rng.uniform(-0.45, 0.45) * w * 8) / 8)
+    c = float(np.round(rng.uniform(-0.9, 0.9) * (w // 2) * 8) / 8)  # |c| < w/2, integer half
     ell = oracle.random_ellipses(int(rng.integers(1, 12)), max(w, 8), 1.0, 7000 + k)
     P = oracle.phantom_sino(N, w, c, ell, 1e5, 7000 + k)
     if contrast == ca.ABS and k % 3 == 0:
@@ -98,7 +98,7 @@ def test_random_precision_mode(k):
     arc = float(rng.choice([180.0, 360.0, 150.0, 200.0]))
     contrast = [ca.ABS, ca.PHS, ca.REF][int(rng.integers(3))]
     filt = FILTERS[int(rng.integers(len(FILTERS)))]
-    c = float(np.round(rng.uniform(-0.45, 0.45) * w * 8) / 8)
+    c = float(np.round(rng.uniform(-0.9, 0.9) * (w // 2) * 8) / 8)  # |c| < w/2, integer half
     store, interp = MODES[k % len(MODES)]
     ell = oracle.random_ellipses(int(rng.integers(1, 12)), w, 1.0, 9000 + k)
     P = oracle.phantom_sino(N, w, c, ell, 1e5, 9000 + k)
@@ -118,3 +118,44 @@ def test_random_precision_mode(k):
         bar = max(2e-3, 2 * float_path)
     print(tag, f"rel_rms={err:.3e} bar={bar:.3e}")
     assert err <= bar, tag
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_uneven_batch(seed, torch_cuda):
+    """Uneven batches of ten random items (2 to 2049 px, 1 to 300 angles,
+    random centres, a third with dead pixels; one arc and filter per batch)
+    in one ctas_fbp_batch_uneven call: every slice within 1e-4 relative RMS
+    of the oracle's end-to-end slice, and the same batch in FLT contrast on
+    the oracle's filtered sinograms bit for bit (the persistent kernel, its
+    tile kinds and the per-line-length filter launches)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(11000 + seed)
+    arc = float(rng.choice([180.0, 360.0, 150.0, 200.0]))
+    filt = FILTERS[int(rng.integers(len(FILTERS)))]
+    f, p = ca.parse_filter(filt)
+    items, flt_items, refs, tags = [], [], [], []
+    for k in range(10):
+        w = int(rng.choice(WIDTHS))
+        N = int(rng.integers(1, 200 if w > 1024 else 300))
+        c = float(np.round(rng.uniform(-0.9, 0.9) * (w // 2) * 8) / 8)  # |c| < w/2, integer half
+        ell = oracle.random_ellipses(int(rng.integers(1, 12)), max(w, 8), 1.0, 11000 + 16 * seed + k)
+        P = oracle.phantom_sino(N, w, c, ell, 1e5, 11000 + 16 * seed + k)
+        if k % 3 == 0:
+            P[rng.integers(0, N, 3), rng.integers(0, w, 3)] = 0.0
+        if P.max() <= 0.0:  # every ray absorbed (tiny widths): keep one count
+            P[0, 0] = 1.0
+        Pf = oracle.prepare(P, oracle.filter_window(f, p, w, ca.ABS), ca.ABS)
+        refs.append(oracle.bp_ctas(Pf, oracle.angles(N, arc), c))
+        tags.append(f"item {k}: w={w} N={N} c={c} arc={arc} {filt}")
+        items.append((torch.from_numpy(P).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), c))
+        flt_items.append((torch.from_numpy(Pf).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), c))
+    ca.batch_uneven(items, ca.ABS, f, p, arc, 0)
+    ca.batch_uneven(flt_items, ca.FLT, f, p, arc, 0)
+    torch.cuda.synchronize()
+    for k, ref in enumerate(refs):
+        out = items[k][1].cpu().numpy()
+        if ref.any():
+            assert rel_rms(out, ref) <= 1e-4, tags[k]
+        else:
+            assert not out.any(), tags[k]
+        assert np.array_equal(flt_items[k][1].cpu().numpy(), ref), tags[k]
