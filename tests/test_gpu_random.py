@@ -159,3 +159,62 @@ def test_random_uneven_batch(seed, torch_cuda):
         else:
             assert not out.any(), tags[k]
         assert np.array_equal(flt_items[k][1].cpu().numpy(), ref), tags[k]
+
+
+@pytest.mark.parametrize("k", range(16))
+def test_random_tomosynthesis(k):
+    """ts_add (lct.cpp:739-787) on random projections: rows, pixels (1 to
+    5000: every line kernel), contrast, filter, angle, centre and plane.
+    The resampled columns are the reference's float index arithmetic, so
+    the result differs from the oracle by the filter's float rounding only:
+    1e-5 of the largest filtered magnitude per filtered sample (1e-4 for
+    REF's running sums), every result pixel within 1e-4 of it beyond the
+    rounding of its own float sum."""
+    rng = np.random.default_rng(13000 + k)
+    rows = int(rng.integers(1, 12))
+    n = int(rng.choice([1, 2, 3, 17, 64, 255, 256, 300, 1000, 1025, 2047, 2048, 2049, 3001, 4096, 5000]))
+    contrast = [ca.ABS, ca.PHS, ca.REF][int(rng.integers(3))]
+    filt = FILTERS[int(rng.integers(len(FILTERS)))]
+    angle = float(np.float32(rng.uniform(-np.pi, np.pi)))
+    c = float(np.round(rng.uniform(-0.2, 0.2) * n * 8) / 8)
+    plane = int(rng.integers(-n // 4 - 1, n // 4 + 2))
+    x = np.linspace(-1, 1, n)
+    P = (0.9 * np.exp(-((x[None, :] - rng.uniform(-0.3, 0.3, (rows, 1))) / 0.3) ** 2)
+         + 0.05 * rng.random((rows, n))).astype(np.float32)
+    if contrast == ca.ABS:
+        P = np.exp(-P).astype(np.float32)
+    R0 = rng.standard_normal((rows, n)).astype(np.float32)
+    f, p = ca.parse_filter(filt)
+    fref, rref = oracle.ts_add(P, R0, oracle.filter_fill(f, p, n), c, contrast, angle, plane)
+    Pd, Rd = P.copy(), R0.copy()
+    ca.ts_add(Pd, Rd, filt, 0.0, c, contrast, angle, plane)
+    tag = f"rows={rows} n={n} contrast={contrast} {filt} angle={angle} c={c} plane={plane}"
+    # KAISER at an odd width: Filter::fill takes sqrt(1 - f^2) at f = (n + 1)/n
+    # (lct.cpp:152-153, 192), a NaN bin that the inverse transform spreads over
+    # every filtered row; the device path gives NaN alike
+    assert np.array_equal(np.isnan(Pd), np.isnan(fref)) and np.array_equal(np.isnan(Rd), np.isnan(rref)), tag
+    fin = ~np.isnan(fref)
+    if not fin.any():
+        return
+    scale = float(np.abs(fref[fin]).max()) or 1.0
+    tol = 1e-4 if contrast == ca.REF else 1e-5
+    assert float(np.abs(Pd - fref)[fin].max()) <= tol * scale, tag
+    fr = ~np.isnan(rref)
+    slack = np.abs(Rd - rref)[fr] - 4 * np.finfo(np.float32).eps * np.abs(rref)[fr]
+    assert float(slack.max()) <= 1e-4 * scale, tag
+
+
+@pytest.mark.parametrize("k", range(12))
+def test_random_ring_filter_and_denoiser(k):
+    """RingFilter::apply (lct.cpp:295-327) and the Denoiser on random shapes,
+    boxes below the width, radii and thresholds: bit for bit."""
+    rng = np.random.default_rng(14000 + k)
+    N, w = int(rng.integers(1, 300)), int(rng.integers(1, 3000))
+    P = rng.uniform(0.2, 1.0, (N, w)).astype(np.float32)
+    P += (0.2 * rng.random(w)).astype(np.float32)[None, :]  # column stripes
+    box = int(rng.integers(0, w))
+    assert np.array_equal(ca.ring_filter(P, box), oracle.ring_filter(P, box)), (N, w, box)
+    img = P.copy()
+    img[rng.random(img.shape) < 0.01] = 0.0
+    rad, thr = int(rng.integers(0, 8)), float(rng.choice([0.0, 0.05, -0.1, 0.5]))
+    assert np.array_equal(ca.denoise(img, rad, thr), oracle.denoise(img, rad, thr)), (N, w, rad, thr)
