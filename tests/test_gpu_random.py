@@ -218,3 +218,29 @@ def test_random_ring_filter_and_denoiser(k):
     img[rng.random(img.shape) < 0.01] = 0.0
     rad, thr = int(rng.integers(0, 8)), float(rng.choice([0.0, 0.05, -0.1, 0.5]))
     assert np.array_equal(ca.denoise(img, rad, thr), oracle.denoise(img, rad, thr)), (N, w, rad, thr)
+
+
+@pytest.mark.parametrize("k", range(12))
+def test_random_axis_scores(k):
+    """AverageGradient scores (lct.cpp:804-917) of random trial axes on random
+    sinograms (64 to 600 px over 180 or 360 degrees), with and without the
+    Gaussian smoothing (sigma = w/(N - 1) on either side of 1): within 2e-4
+    of the oracle's, as the fixed test holds them. The angle step below one
+    degree, as `ctas ax` passes it (a value of 1 or more is read as the whole
+    arc, lct.cpp:470-484)."""
+    rng = np.random.default_rng(15000 + k)
+    w = int(rng.integers(64, 601))
+    arc = float(rng.choice([180.0, 360.0]))
+    N = int(rng.integers(int(arc) + 2, int(arc) + 300))
+    c0 = float(np.round(rng.uniform(-0.05, 0.05) * w * 4) / 4)
+    ell = oracle.random_ellipses(int(rng.integers(3, 12)), w, w / 512.0, 15000 + k)
+    P = oracle.phantom_sino(N, w, c0, ell, 1e5, 15000 + k)
+    step = arc / (N - 1)
+    rec = ca.CTrec(w, N, ca.ABS, "RAMP", anarc=step)
+    rec.sino(P)
+    axes = (0.5 * (w - 1) + c0 + rng.uniform(-0.06, 0.06, 4) * w).astype(np.float32)
+    got = rec.axis_scores(axes)
+    Pf = oracle.prepare(P, oracle.filter_window(1, 0.0, w, ca.ABS), ca.ABS)
+    sigma = np.float32(oracle.angO(step, N) * np.float32(w) / np.pi)
+    ref = np.array([oracle.ag_score(Pf, oracle.angles(N, step), float(sigma), float(a)) for a in axes])
+    assert np.allclose(got, ref, rtol=2e-4), (w, N, arc, c0, axes, got, ref)
