@@ -456,10 +456,14 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
 // then taken at k-1). One 8-byte LDS read serves both pixels; half as many
 // LDS reads as the float path, and each sample costs an integer add, a
 // compare and a select instead of float arithmetic and a conversion. The
-// ray coordinate of pixel (y, x) of the tile is bt + S y + C x in 16.16
-// fixed point, bt, S and C formed per angle from FP64 constants (within
-// ~1e-4 px of the FP64 coordinate, finer than the float arithmetic of the
-// parity path).
+// ray coordinate of pixel (y, x) of the tile is bt + S y + C x in 14.18
+// fixed point (the word index of the sample in LDS, the window's offset
+// included, and 18 fraction bits), bt, S and C rounded per angle from FP64
+// constants: within (1 + y + x) 2^-19 px of the FP64 coordinate, 5e-4 px at
+// the far corner of a 128 x 128 tile. (With 16.16 the rounded steps reached
+// 2e-3 px there, and the bins those errors straddled moved samples enough to
+// stand 2x to 20x further from the FP64 oracle than the reference's float
+// path with few angles.)
 // The fixed-point paths index windows by floor(t); the reference reads
 // sample (long)t, truncated (lct.cpp:710-713), which for a non-integer t < 0
 // is floor(t) + 1. Window position k therefore holds the sample at
@@ -481,6 +485,10 @@ __device__ __forceinline__ float half_to_float(uint32_t h) {
     return __uint_as_float(h << 16);
 }
 
+// 14.18: fraction bits of the fixed-point coordinates of StagePair
+constexpr int kFxFrac = 18;
+constexpr unsigned kFxOne = 1u << kFxFrac;
+
 template <int KIND, int AC, int ST>
 struct StagePair {
   using G = KindGeo<KIND>;
@@ -490,6 +498,8 @@ struct StagePair {
   static constexpr size_t HALFBUF = WIN + 2 * sizeof(int4) * AC;
   static constexpr size_t ANGS = 2 * HALFBUF;
   static constexpr size_t BYTES = ANGS + 2 * sizeof(double2) * AC;
+  // every word index of the two windows fits the integer part
+  static_assert(ANGS / sizeof(WT) <= (size_t{1} << (32 - kFxFrac)), "14.18 word index");
   uint16_t v[EPT][2];
   int4 prm0, prm1;
   double2 sd, sd_next;  // table entries of the pass being fetched and of the one after
@@ -514,12 +524,13 @@ struct StagePair {
     const double tc = T0 + sd.x * (0.5 * (G::RT - 1)) + sd.y * (0.5 * (G::CT - 1));
     const int lo = static_cast<int>(floor(tc)) - G::HALF - 2;
     const bool neg = sd.x < 0.0;
-    const int b0 = static_cast<int>(llrint((T0 - lo + static_cast<double>(a) * L) * 65536.0));
-    const int bt = b0 - (neg ? 65536 : 0);
-    const int S = static_cast<int>(llrint(sd.x * 65536.0)), C = static_cast<int>(llrint(sd.y * 65536.0));
+    constexpr double one = static_cast<double>(kFxOne);
+    const int b0 = static_cast<int>(llrint((T0 - lo + static_cast<double>(a) * L) * one));
+    const int bt = b0 - (neg ? static_cast<int>(kFxOne) : 0);
+    const int S = static_cast<int>(llrint(sd.x * one)), C = static_cast<int>(llrint(sd.y * one));
     // every coordinate of the tile an integer: sample k at position k
-    const bool exact = ((b0 | S | C) & 0xFFFF) == 0;
-    const int thr = neg ? -S : 65536 - S;
+    const bool exact = ((b0 | S | C) & static_cast<int>(kFxOne - 1)) == 0;
+    const int thr = neg ? -S : static_cast<int>(kFxOne) - S;
     prm0 = make_int4(bt, S, C, thr | (neg ? (1 << 30) : 0));
     prm1 = make_int4(8 * S, 16 * C, 0, 0);
     // reads outside the sinogram land in its zero pads
@@ -549,7 +560,8 @@ struct StagePair {
       // the word size
       int4 *prm = reinterpret_cast<int4 *>(buf + WIN);
       int4 p0 = prm0;
-      p0.x += static_cast<int>((buf - base) / sizeof(WT)) << 16;
+      p0.x = static_cast<int>(static_cast<unsigned>(p0.x) +
+                              (static_cast<unsigned>((buf - base) / sizeof(WT)) << kFxFrac));
       prm[2 * a] = p0;
       prm[2 * a + 1] = prm1;
     }
@@ -620,25 +632,27 @@ __device__ __forceinline__ void bp_tile_px2(const Item &it, int tile, unsigned c
     auto angle = [&](const int4 p, const int4 pd, auto hi_first) {
       constexpr bool HF = decltype(hi_first)::value;
       const unsigned thr = static_cast<unsigned>(p.w & 0x3FFFFFFF);
-      int A[RP], C[TC];
-      A[0] = __mul24(p.y, ty) + p.x;
+      // unsigned: the word index reaches 2^14 (bt near 2^32); the sums wrap
+      // to the coordinate, which lies inside the window
+      unsigned A[RP], C[TC];
+      A[0] = static_cast<unsigned>(__mul24(p.y, ty)) + static_cast<unsigned>(p.x);
 #pragma unroll
-      for (int rp = 1; rp < RP; ++rp) A[rp] = A[rp - 1] + pd.x;
-      C[0] = __mul24(p.z, tx);
+      for (int rp = 1; rp < RP; ++rp) A[rp] = A[rp - 1] + static_cast<unsigned>(pd.x);
+      C[0] = static_cast<unsigned>(__mul24(p.z, tx));
 #pragma unroll
-      for (int q = 1; q < TC; ++q) C[q] = C[q - 1] + pd.y;
+      for (int q = 1; q < TC; ++q) C[q] = C[q - 1] + static_cast<unsigned>(pd.y);
 #pragma unroll
       for (int rp = 0; rp < RP; ++rp) {
         unsigned u[TC];
         W P[TC];
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
-          u[q] = static_cast<unsigned>(A[rp] + C[q]);
-          P[q] = *reinterpret_cast<const W *>(lds0 + ((u[q] >> 16) * sizeof(W)));
+          u[q] = A[rp] + C[q];
+          P[q] = *reinterpret_cast<const W *>(lds0 + ((u[q] >> kFxFrac) * sizeof(W)));
         }
 #pragma unroll
         for (int q = 0; q < TC; ++q) {
-          const bool up = (u[q] & 0xFFFFu) >= thr;
+          const bool up = (u[q] & (kFxOne - 1)) >= thr;
           acc[2 * rp][q] += HF ? P[q].y : P[q].x;
           acc[2 * rp + 1][q] += up ? P[q].y : P[q].x;
         }
