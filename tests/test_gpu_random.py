@@ -244,3 +244,41 @@ def test_random_axis_scores(k):
     sigma = np.float32(oracle.angO(step, N) * np.float32(w) / np.pi)
     ref = np.array([oracle.ag_score(Pf, oracle.angles(N, step), float(sigma), float(a)) for a in axes])
     assert np.allclose(got, ref, rtol=2e-4), (w, N, arc, c0, axes, got, ref)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_uneven_batch_precision_modes(seed, torch_cuda):
+    """Uneven batches of eight random items (64 to 2049 px, 90 to 300
+    angles) in FP16 or BF16 storage with CTAS sampling, or LINEAR sampling
+    from FP32 storage, against the FP64 oracle at the bars of
+    test_random_precision_mode (the persistent kernel's mirrored, fixed-point
+    and float paths per width class)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(16000 + seed)
+    store, interp = [(ca.STORE_FP16, ca.INTERP_CTAS), (ca.STORE_BF16, ca.INTERP_CTAS),
+                     (ca.STORE_FP32, ca.INTERP_LINEAR), (ca.STORE_FP16, ca.INTERP_CTAS)][seed]
+    arc = float(rng.choice([180.0, 360.0, 150.0]))
+    filt = FILTERS[int(rng.integers(len(FILTERS)))]
+    f, p = ca.parse_filter(filt)
+    linear = interp == ca.INTERP_LINEAR
+    items, cases = [], []
+    for k in range(8):
+        w = int(rng.choice([64, 255, 256, 300, 511, 700, 1023, 1025, 1100, 2047, 2048, 2049]))
+        N = int(rng.integers(90, 200 if w > 1024 else 300))
+        c = float(np.round(rng.uniform(-0.9, 0.9) * (w // 2) * 8) / 8)
+        ell = oracle.random_ellipses(int(rng.integers(1, 12)), w, w / 512.0, 16000 + 16 * seed + k)
+        P = oracle.phantom_sino(N, w, c, ell, 1e5, 16000 + 16 * seed + k)
+        Pf = oracle.prepare(P, oracle.filter_window(f, p, w, ca.ABS), ca.ABS)
+        ref = oracle.bp_f64(Pf, c, arc * np.pi / 180.0, 1.0, linear=linear)
+        if linear:
+            bar = 1e-4 * max(1.0, (w + 2 * abs(c)) / 1024)
+        else:
+            bar = max(2e-3, 2 * rel_rms(oracle.bp_ctas(Pf, oracle.angles(N, arc), c), ref))
+        cases.append((ref, bar, f"item {k}: w={w} N={N} c={c} arc={arc} {filt} store={store:#x} interp={interp}"))
+        items.append((torch.from_numpy(P).cuda(), torch.zeros((w, w), dtype=torch.float32, device="cuda"), c))
+    ca.batch_uneven(items, ca.ABS, f, p, arc, store | interp)
+    torch.cuda.synchronize()
+    for (_, d_out, _), (ref, bar, tag) in zip(items, cases):
+        err = rel_rms(d_out.cpu().numpy(), ref)
+        print(tag, f"rel_rms={err:.3e} bar={bar:.3e}")
+        assert err <= bar, tag
