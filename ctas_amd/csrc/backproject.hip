@@ -461,9 +461,9 @@ __device__ __forceinline__ void bp_tile(const Item &it, int tile, unsigned char 
 // included, and 18 fraction bits), bt, S and C rounded per angle from FP64
 // constants: within (1 + y + x) 2^-19 px of the FP64 coordinate, 5e-4 px at
 // the far corner of a 128 x 128 tile. (With 16.16 the rounded steps reached
-// 2e-3 px there, and the bins those errors straddled moved samples enough to
-// stand 2x to 20x further from the FP64 oracle than the reference's float
-// path with few angles.)
+// 2e-3 px there; with few angles, the samples those errors moved across a
+// bin edge put slices 2x to 20x further from the FP64 oracle than the
+// reference's float path, and 14.18 halves that distance.)
 // The fixed-point paths index windows by floor(t); the reference reads
 // sample (long)t, truncated (lct.cpp:710-713), which for a non-integer t < 0
 // is floor(t) + 1. Window position k therefore holds the sample at

@@ -86,9 +86,10 @@ def test_random_precision_mode(k):
     with few angles, wide rows and noisy filtered rows a slice can stand far
     from FP64 whatever its storage: the reference's own float path
     (reconstructOnCPU, lct.cpp:688-729, restated by the oracle) does in these
-    cases (7e-3 to 1e-1 measured). The 16-bit paths, whose 16.16 fixed-point
-    coordinates round the per-pixel steps to 2^-16 px (an error that grows
-    across a tile), must stay within 2e-3
+    cases (7e-3 to 1e-1 measured). The 16-bit paths, whose fixed-point
+    coordinates round the per-pixel steps (to 2^-18 px on the non-mirrored
+    path, an error that grows across a tile; per table entry to 2^-16 px on
+    the mirrored one), must stay within 2e-3
     or within twice that float path's distance from FP64, whichever is
     larger; at the workloads' angle counts (1800, 3000) both are far below
     2e-3. At least 90 angles."""
