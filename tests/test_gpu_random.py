@@ -283,3 +283,37 @@ def test_random_uneven_batch_precision_modes(seed, torch_cuda):
         err = rel_rms(d_out.cpu().numpy(), ref)
         print(tag, f"rel_rms={err:.3e} bar={bar:.3e}")
         assert err <= bar, tag
+
+
+def _dots(w, n, rng):
+    """n small dense discs inside the inner 60% of a w-pixel field."""
+    xy = rng.uniform(-0.3 * w, 0.3 * w, (n, 2))
+    return np.array([[x, y, 2.0, 2.0, 0.0, 2.0 / w] for x, y in xy])
+
+
+@pytest.mark.parametrize("k", range(16))
+def test_random_raxis(k):
+    """raxis (lct.cpp:1133-1212), every algorithm, on random noise-free
+    sinograms (96 to 192 px, 182 to 300 angles over 180 degrees, centre
+    offsets up to 8% of the width, maximum deviations 0.1 and 0.25): the
+    oracle's search and the device's agree to its last bisection step
+    (0.25 px), as the fixed test holds them; where the oracle's search fails,
+    the device's reports CTAS_E_AXIS."""
+    rng = np.random.default_rng(17000 + k)
+    w = int(rng.integers(96, 193))
+    N = int(rng.integers(182, 301))
+    c0 = float(np.round(rng.uniform(-0.08, 0.08) * w * 4) / 4)
+    algo = k % 4
+    max_dev = float(rng.choice([0.1, 0.25]))
+    P = np.exp(-oracle.phantom_sino(N, w, c0, _dots(w, int(rng.integers(4, 12)), rng), 0.0, 0)).astype(np.float32)
+    step = 180.0 / (N - 1)
+    tag = f"w={w} N={N} c0={c0} algo={algo} max_dev={max_dev}"
+    try:
+        ref = oracle.raxis(P, step, max_dev, algo)
+    except ValueError:
+        with pytest.raises(ca.CtasError) as e:
+            ca.raxis(P, step, max_dev, algo)
+        assert e.value.code == ca.CTAS_E_AXIS, tag
+        return
+    got = ca.raxis(P, step, max_dev, algo)
+    assert abs(got - ref) <= 0.25, (tag, got, ref)
