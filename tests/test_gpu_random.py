@@ -317,3 +317,33 @@ def test_random_raxis(k):
         return
     got = ca.raxis(P, step, max_dev, algo)
     assert abs(got - ref) <= 0.25, (tag, got, ref)
+
+
+@pytest.mark.parametrize("k", range(12))
+def test_random_raxis_pair(k):
+    """raxis(proj0, proj180) (lct.cpp:1217-1294) on random projection pairs
+    (32 to 400 px, 4 to 80 rows): the 180-degree projection is the 0-degree
+    one mirrored about (w-1)/2 + c0 with a little noise of its own; the
+    device agrees with the oracle within 0.5 px, as the fixed test holds it,
+    and both fail alike."""
+    rng = np.random.default_rng(18000 + k)
+    w = int(rng.integers(32, 401))
+    n = int(rng.integers(4, 81))
+    c0 = float(np.round(rng.uniform(-0.1, 0.1) * w * 2) / 2)
+    max_dev = float(rng.choice([0.15, 0.25, 0.4]))
+    P = np.exp(-oracle.phantom_sino(n, w, 0.0, _dots(w, int(rng.integers(3, 10)), rng), 0.0, 0)).astype(np.float32)
+    s = int(round(2 * c0))
+    src = (w - 1 + s) - np.arange(w)
+    ok = (src >= 0) & (src < w)
+    p180 = np.ones_like(P)
+    p180[:, ok] = P[:, src[ok]]
+    p180 = (p180 * (1.0 + 1e-3 * rng.standard_normal(p180.shape))).astype(np.float32)
+    tag = f"w={w} rows={n} c0={c0} max_dev={max_dev}"
+    try:
+        ref = oracle.raxis_pair(P, p180, max_dev)
+    except ValueError:
+        with pytest.raises(ca.CtasError):
+            ca.raxis_pair(P, p180, max_dev)
+        return
+    got = ca.raxis_pair(P, p180, max_dev)
+    assert abs(got - ref) <= 0.5, (tag, got, ref)
