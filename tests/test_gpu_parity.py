@@ -452,6 +452,35 @@ def test_fp16_range_in_uneven_batch(torch_cuda):
     assert all(torch.isfinite(o).all().item() for _, o, _ in items)
 
 
+def test_fp16_range_status_in_graph_replays(torch_cuda):
+    """A captured graph (FP16 storage, PHS contrast) replayed over data that
+    alternates between in-range filtered samples and one beyond 16376: the
+    per-slice status follows the data of each replay (the reset kernel runs
+    inside the graph), and the in-range replays equal the eager result."""
+    torch = torch_cuda
+    N, w, S = 180, 256, 2
+    rng = np.random.default_rng(21)
+    ok = rng.uniform(0.5, 1.0, (S, N, w)).astype(np.float32)
+    spike = np.zeros((N, w), np.float32)
+    spike[50, 128] = 1.0
+    gain = np.abs(ca.CTrec(w, N, ca.PHS, "RAMP").prepare(spike)).max()
+    bad = ok.copy()
+    bad[1, 50, 128] += 3e4 / gain
+    rec = ca.CTrec(w, N, ca.PHS, "RAMP", store=ca.STORE_FP16)
+    eager = rec.reconstruct(ok, [0.0] * S)[1]
+    d_in = torch.from_numpy(ok).cuda()
+    d_out = torch.zeros((S, w, w), dtype=torch.float32, device="cuda")
+    rec.graph_capture(d_in, d_out, [0.0] * S)
+    for data, want in ((ok, [0, 0]), (bad, [0, ca.CTAS_E_RANGE]), (ok, [0, 0]), (bad, [0, ca.CTAS_E_RANGE])):
+        d_in.copy_(torch.from_numpy(data))
+        torch.cuda.synchronize()
+        rec.graph_launch()
+        torch.cuda.synchronize()
+        assert list(rec.status(S)) == want
+        if data is ok:
+            assert np.array_equal(d_out.cpu().numpy(), eager)
+
+
 def test_centre_out_of_range():
     rec = ca.CTrec(64, 16, ca.ABS, "RAMP")
     with pytest.raises(ca.CtasError) as e:
