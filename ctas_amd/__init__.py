@@ -22,8 +22,7 @@ LIB_PATH = _PKG / "libctas_hip.so"
 
 CTAS_OK = 0
 CTAS_E_ARG, CTAS_E_SHAPE, CTAS_E_CENTER, CTAS_E_DATA = -1, -2, -3, -4
-CTAS_E_RANGE = -10
-CTAS_E_DEVICE, CTAS_E_NOMEM, CTAS_E_STATE, CTAS_E_AXIS = -5, -6, -7, -8
+CTAS_E_DEVICE, CTAS_E_NOMEM, CTAS_E_STATE, CTAS_E_AXIS, CTAS_E_IO, CTAS_E_RANGE = -5, -6, -7, -8, -9, -10
 
 ABS, REF, PHS, FLT = 0, 1, 2, 3
 FILTERS = ["NONE", "RAMP", "BARLETT", "WELCH", "PARZEN", "HANN", "HAMMING", "BLACKMAN",

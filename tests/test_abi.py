@@ -96,3 +96,21 @@ def test_arc_helpers():
     assert ca.lib().ctas_arc_rad(180.0, 10) == oracle.arcO(180.0, 10)
     assert ca.lib().ctas_step_rad(180.0, 1801) == oracle.angO(180.0, 1801)
     assert ca.lib().ctas_arc_rad(0.25, 721) == oracle.arcO(0.25, 721)
+
+
+def test_python_constants_match_the_header():
+    """The binding's return codes, contrasts, filter names and flag values
+    are the header's #defines."""
+    text = (ROOT / "include" / "ctas_hip.h").read_text()
+    defs = {m.group(1): int(m.group(2), 0)
+            for m in re.finditer(r"^#define (CTAS_[A-Z0-9_]+)\s+(-?0x[0-9a-fA-F]+|-?\d+)", text, re.M)}
+    codes = [n for n in defs if n == "CTAS_OK" or n.startswith("CTAS_E_")]
+    assert len(codes) >= 11
+    for n in codes:
+        assert getattr(ca, n) == defs[n], n
+    for py in ("ABS", "REF", "PHS", "FLT"):
+        assert getattr(ca, py) == defs["CTAS_CONTRAST_" + py], py
+    for py in ("INTERP_CTAS", "INTERP_LINEAR", "STORE_FP32", "STORE_FP16", "STORE_BF16"):
+        assert getattr(ca, py) == defs["CTAS_" + py], py
+    for i, name in enumerate(ca.FILTERS):
+        assert defs["CTAS_FILTER_" + name] == i, name
