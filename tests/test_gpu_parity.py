@@ -481,6 +481,42 @@ def test_fp16_range_status_in_graph_replays(torch_cuda):
             assert np.array_equal(d_out.cpu().numpy(), eager)
 
 
+def test_concurrent_handles_from_threads():
+    """RecInThread's pattern (ct.cpp:156-236): one CTrec clone per thread,
+    all reconstructing at once (the C calls release the GIL). Four threads,
+    two of them on clones of one handle and two on handles of other widths
+    and storage, three rounds each: every slice equals the one the same
+    handle gives alone, and the error text stays per thread."""
+    import threading
+    N = 120
+    base = ca.CTrec(256, N, ca.ABS, "HANN")
+    cases = [(base.clone(), 256, 0), (base.clone(), 256, 1),
+             (ca.CTrec(300, N, ca.ABS, "RAMP"), 300, 2), (ca.CTrec(512, N, ca.ABS, "HANN", store=ca.STORE_FP16), 512, 3)]
+    data = [np.stack([_phantom(N, w, 1.25, 6, 1e5, seed=90 + 10 * k + s) for s in range(2)]) for _, w, k in cases]
+    alone = [rec.reconstruct(d, [1.25, 1.25])[1] for (rec, _, _), d in zip(cases, data)]
+    results, errors = [None] * 4, []
+
+    def work(k):
+        try:
+            rec = cases[k][0]
+            for _ in range(3):
+                results[k] = rec.reconstruct(data[k], [1.25, 1.25])[1]
+                assert np.array_equal(results[k], alone[k])
+            with pytest.raises(ca.CtasError) as e:
+                rec.reconstruct(data[k], [1e4, 0.0])
+            assert "outside the sinogram width" in str(e.value)
+        except BaseException as exc:  # reported on the main thread
+            errors.append((k, repr(exc)))
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
+
+
 def test_centre_out_of_range():
     rec = ca.CTrec(64, 16, ca.ABS, "RAMP")
     with pytest.raises(ca.CtasError) as e:
