@@ -503,3 +503,24 @@ def test_bp_f64_rows_is_bp_f64_restricted():
             sc = oracle.angles(120)
             np.testing.assert_array_equal(oracle.bp_ctas_rows(P, sc, c, rows), oracle.bp_ctas(P, sc, c)[rows])
 
+
+
+@pytest.mark.parametrize("n", [3, 17, 257, 1025, 2049])
+def test_filter_fill_odd_width_middle_bin(n):
+    """Filter::fill (lct.cpp:150-153) takes f = 2 (n - pix) / n for pix >= n/2
+    in integer division, so at an odd width the middle bin pix = (n - 1)/2
+    has f = (n + 1)/n > 1: KAISER's sqrt(1 - f^2) (lct.cpp:192) is NaN there
+    and only there; the other windows stay finite (WELCH goes negative)."""
+    # filter codes of the C ABI (the order of Filter::Ftype, ct.h:92-136)
+    RAMP, WELCH, PARZEN, HANN, BLACKMAN, LANCKZOS, KAISER, GAUSS = 1, 3, 4, 5, 7, 8, 9, 10
+    kaiser, welch = (KAISER, 2.5), (WELCH, 0.0)
+    K = oracle.filter_fill(*kaiser, n)
+    assert np.flatnonzero(np.isnan(K)).tolist() == [(n - 1) // 2]
+    for f, p in ((RAMP, 0.0), (HANN, 0.0), (WELCH, 0.0), (PARZEN, 0.0), (GAUSS, 0.4), (LANCKZOS, 0.0),
+                 (BLACKMAN, 0.0)):
+        assert np.isfinite(oracle.filter_fill(f, p, n)).all(), f
+    for even in (n - 1, n + 1):
+        assert np.isfinite(oracle.filter_fill(*kaiser, even)).all()
+    W = oracle.filter_fill(*welch, n)
+    W_even = oracle.filter_fill(*welch, n + 1)
+    assert W[(n - 1) // 2] != 0.0 and np.isfinite(W_even).all()
