@@ -511,7 +511,7 @@ def test_filter_fill_odd_width_middle_bin(n):
     in integer division, so at an odd width the middle bin pix = (n - 1)/2
     has f = (n + 1)/n > 1: KAISER's sqrt(1 - f^2) (lct.cpp:192) is NaN there
     and only there; the other windows stay finite (WELCH goes negative)."""
-    # filter codes of the C ABI (the order of Filter::Ftype, ct.h:92-136)
+    # filter codes of the C ABI (the order of Filter::Ftype, ct.h:92-132)
     RAMP, WELCH, PARZEN, HANN, BLACKMAN, LANCKZOS, KAISER, GAUSS = 1, 3, 4, 5, 7, 8, 9, 10
     kaiser, welch = (KAISER, 2.5), (WELCH, 0.0)
     K = oracle.filter_fill(*kaiser, n)
