@@ -1,7 +1,7 @@
 """Seeded random configurations through the C ABI against the oracle.
 
-Each case draws a width (2 to 2049 px: every filter kernel's line lengths and
-the single-block widths), an angle count, an arc, a contrast, a filter and a
+Each case draws a width (2 to 2049 px, and 3001 to 5000 px in six cases:
+every filter kernel's line lengths and the single-block widths), an angle count, an arc, a contrast, a filter and a
 centre offset; a third of the absorption cases carry dead pixels (zero or
 negative samples), which take deAbs's clamp. Three checks per case:
 
@@ -29,6 +29,9 @@ def _case(k):
     rng = np.random.default_rng(7000 + k)
     w = int(rng.choice(WIDTHS))
     N = int(rng.integers(1, 160 if w > 1024 else 400))
+    if k >= 48:  # the widest lines: 8192 and 16384 points
+        w = int(rng.choice([3001, 4096, 5000]))
+        N = int(rng.integers(1, 40))
     arc = float(rng.choice([180.0, 360.0, 150.0, 200.0]))
     contrast = [ca.ABS, ca.PHS, ca.REF][int(rng.integers(3))]
     filt = FILTERS[int(rng.integers(len(FILTERS)))]
@@ -43,7 +46,7 @@ def _case(k):
     return w, N, arc, contrast, filt, c, np.ascontiguousarray(P, np.float32)
 
 
-@pytest.mark.parametrize("k", range(48))
+@pytest.mark.parametrize("k", range(54))
 def test_random_configuration(k):
     w, N, arc, contrast, filt, c, P = _case(k)
     tag = f"w={w} N={N} arc={arc} contrast={contrast} {filt} c={c}"
