@@ -564,6 +564,10 @@ def main() -> int:
                              "measured in this run (peak_b32: the same for ds_read_b32); binding: the "
                              "instruction-issue roof nearest to full (valu.frac, lds_issue.frac)"},
         "stages_ms": {"prepare": round(prep_ms, 3), "backproject": round(bp_ms, 3)},
+        # the prepare stage's compulsory HBM bytes: each raw sample read once
+        # (float32) and each filtered sample written once (float32 storage)
+        "prepare_hbm": {"GBs": round(8.0 * S * N * w / (prep_ms / 1e3) / 1e9, 1),
+                        "frac": round(8.0 * S * N * w / (prep_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
         "peaks": peaks,
     }
     if pmc:
