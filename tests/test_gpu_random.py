@@ -386,3 +386,46 @@ def test_random_pipeline(k):
     ref_in = np.stack([ca.ring_filter(s, ring) for s in sinos]) if ring else sinos
     _, ref = ca.CTrec(w, N, ca.ABS, "HANN", store=store).reconstruct(ref_in, [c] * S)
     assert np.array_equal(got, ref), tag
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_random_custom_angle_tables(k):
+    """Caller-supplied {sin, cos} tables (ctas_fbp_set_angles): random
+    angles over the full turn, shuffled, with exact theta / pi - theta
+    partners (equal sines, negated cosines) in half the draws so that the
+    16-bit path pairs them. The FP32 parity path is bit-identical to
+    reconstructOnCPU (lct.cpp:688-729) on the same table, FLT and end to end
+    within 1e-4; FP16 storage stays within 1e-2 of FP32 (measured: 2e-4 to
+    1.6e-3), a guard against a wrong pairing or window, which puts slices off
+    by tens of percent (FP16's accuracy against FP64 is held by the fixed
+    tests)."""
+    rng = np.random.default_rng(20000 + k)
+    w = int(rng.choice([64, 128, 255, 300, 511, 700]))
+    N = int(rng.integers(20, 300))
+    c = float(np.round(rng.uniform(-0.9, 0.9) * (w // 2) * 4) / 4)
+    th = rng.uniform(0.0, 2 * np.pi, N)
+    sc = np.stack([np.sin(th), np.cos(th)], 1).astype(np.float32)
+    if k % 2:
+        h = N // 2
+        sc[h:2 * h, 0] = sc[:h, 0]
+        sc[h:2 * h, 1] = -sc[:h, 1]
+    sc = np.ascontiguousarray(sc[rng.permutation(N)])
+    tag = f"w={w} N={N} c={c} pairs={bool(k % 2)}"
+    Pf = rng.standard_normal((N, w)).astype(np.float32)
+    flt = ca.CTrec(w, N, ca.FLT, "RAMP")
+    flt.set_angles(sc)
+    assert np.array_equal(flt.angles(), sc)
+    ref = oracle.bp_ctas(Pf, sc, c)
+    assert np.array_equal(flt.reconstruct(Pf, c)[1][0], ref), tag
+    P = oracle.phantom_sino(N, w, c, oracle.random_ellipses(6, w, w / 512.0, 20000 + k), 1e5, 20000 + k)
+    rec = ca.CTrec(w, N, ca.ABS, "HANN")
+    rec.set_angles(sc)
+    f, p = ca.parse_filter("HANN")
+    Pa = oracle.prepare(P, oracle.filter_window(f, p, w, ca.ABS), ca.ABS)
+    out = rec.reconstruct(P, c)[1][0]
+    assert rel_rms(out, oracle.bp_ctas(Pa, sc, c)) <= 1e-4, tag
+    r16 = ca.CTrec(w, N, ca.ABS, "HANN", store=ca.STORE_FP16)
+    r16.set_angles(sc)
+    err16 = rel_rms(r16.reconstruct(P, c)[1][0], out)
+    print(tag, f"FP16 vs FP32 {err16:.2e}")
+    assert err16 <= 1e-2, tag
