@@ -815,7 +815,14 @@ struct StageMirror {
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;
       if (e >= L2) continue;
-      win[a * L2 + e] = __builtin_bit_cast(uint32_t, v[j]);
+      if constexpr (ST == ST_BF16_H) {
+        // BF16 -> FP32 (exact) -> FP16, rounded to nearest even
+        const half2v h = {static_cast<_Float16>(__uint_as_float(static_cast<uint32_t>(v[j].x) << 16)),
+                          static_cast<_Float16>(__uint_as_float(static_cast<uint32_t>(v[j].y) << 16))};
+        win[a * L2 + e] = __builtin_bit_cast(uint32_t, h);
+      } else {
+        win[a * L2 + e] = __builtin_bit_cast(uint32_t, v[j]);
+      }
     }
     // word addresses are taken from the start of the LDS block
     const int off = static_cast<int>((buf - base) / sizeof(uint32_t)) + a * L2;
@@ -938,7 +945,7 @@ __device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigne
         WB[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ub >> 16) << 2));
       }
     };
-    if constexpr (ST == ST_F16) {
+    if constexpr (ST == ST_F16 || ST == ST_BF16_H) {
       // FP16: one packed add forms both pixels' pair sums of a slot, (lo WA
       // + hi WB, hi WA + lo WB) (the swap is the add's operand select); the
       // sums of two slots are added packed, each step rounded once to FP16,
@@ -1144,6 +1151,9 @@ hipError_t launch_persist_k(const Item *d_items, int n_items, int total_tiles, i
     if (store == ST_F16)
       hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 8, ST_F16>), dim3(g), dim3(256), 0, s, d_items, n_items,
                          total_tiles, counter);
+    else if (store == ST_BF16_H)
+      hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 8, ST_BF16_H>), dim3(g), dim3(256), 0, s, d_items,
+                         n_items, total_tiles, counter);
     else
       hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 8, ST_BF16>), dim3(g), dim3(256), 0, s, d_items, n_items,
                          total_tiles, counter);
@@ -1177,6 +1187,8 @@ template <int KIND, int AC>
 hipError_t launch_mirror(const Item *d_items, int n_items, int total_tiles, int store, hipStream_t s) {
   if (store == ST_F16)
     hipLaunchKernelGGL((bp_mirror_kernel<KIND, AC, ST_F16>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items);
+  else if (store == ST_BF16_H)
+    hipLaunchKernelGGL((bp_mirror_kernel<KIND, AC, ST_BF16_H>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items);
   else
     hipLaunchKernelGGL((bp_mirror_kernel<KIND, AC, ST_BF16>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items);
   return hipGetLastError();

@@ -260,7 +260,7 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   CTAS_HIP_TRY(launch_filter(h->d_items, S, pairs * S, h->logZ, h->logZ, h->contrast, st, s));
   if (do_bp)
     CTAS_HIP_TRY(launch_backproject(h->d_items, S, items[0].tiles * S, kind, choose_ac(h),
-                                    mode_of(h->flags), st, mirror, s));
+                                    mode_of(h->flags), bp_store(st, h->contrast, mirror), mirror, s));
   h->last_S = S;
   return CTAS_OK;
 }
@@ -295,7 +295,7 @@ int enqueue_repeat(ctas_fbp *h, int slice, float *d_out, const float *centers, i
   bool changed = false;
   if (int rc = upload_items(h, items, s, &changed, extra(h).resident)) return rc;
   CTAS_HIP_TRY(launch_backproject(h->d_items, n, items[0].tiles * n, kind, choose_ac(h),
-                                  mode_of(h->flags), st, mirror, s));
+                                  mode_of(h->flags), bp_store(st, h->contrast, mirror), mirror, s));
   return CTAS_OK;
 }
 
@@ -1035,7 +1035,8 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
   for (size_t l = 0; l < launches.size(); ++l) {
     const auto &L = launches[l];
     CTAS_HIP_TRY(launch_backproject_persistent(bw.d_items + std::get<0>(L), std::get<1>(L), std::get<2>(L),
-                                               std::get<3>(L), std::get<4>(L), mode, st, std::get<5>(L),
+                                               std::get<3>(L), std::get<4>(L), mode,
+                                               bp_store(st, contrast, std::get<5>(L)), std::get<5>(L),
                                                bw.d_counter + l, cus, s));
   }
   CTAS_HIP_TRY(hipStreamSynchronize(s));

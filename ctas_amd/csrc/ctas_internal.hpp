@@ -133,7 +133,21 @@ constexpr float kHalfMax = 65504.0f, kHalfSumMax = 16376.0f;
 constexpr int kMinMaxBlocks = 64;
 constexpr int kMinMaxStride = kMinMaxBlocks + 1;  // float2 entries per item
 
-enum Store { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2 };
+enum Store { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2,
+             // the mirrored backprojection of BF16 storage staging its samples
+             // as FP16 and summing them as the FP16 path does (see bp_store)
+             ST_BF16_H = 3 };
+
+// Storage code of the backprojection launch. BF16 samples convert to FP16
+// exactly above 2^-14 in magnitude (8 significant bits into 11) and below
+// it round by at most 2^-25; the mirrored kernel adds four in FP16, which
+// stays finite below 16376. ABS data is far inside that: -log of a positive
+// float lies in [-88.8, 103.3] and no window's spatial kernel sums to more
+// than 6.5 in magnitude (PARZEN's quirk, lct.cpp:170-172), so every
+// filtered sample is below 700. Other contrasts keep the BF16 path.
+inline int bp_store(int st, int contrast, bool mirror) {
+  return (mirror && st == ST_BF16 && contrast == CTAS_CONTRAST_ABS) ? ST_BF16_H : st;
+}
 enum Mode { MODE_CTAS = 0, MODE_LINEAR = 1 };
 
 inline int store_of(int flags) {
