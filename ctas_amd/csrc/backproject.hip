@@ -736,13 +736,18 @@ struct StageMirror {
   static constexpr int RT = G::RT, CT = G::CT, TR = G::TR, TC = G::TC;
   static constexpr int TAB = 2 * RT + CT;  // ints per slot: RA, RB, CX
   static constexpr int TPT = (RT + CT + TPA - 1) / TPA;  // SY and CX entries per thread
-  static constexpr size_t WIN = sizeof(uint32_t) * AC * L2;
+  // LINEAR (FP32 storage): word k of a window holds (v_a[k-1], (v_a[k] -
+  // v_a[k-1]) 2^-16, v_b[k-1], (v_b[k] - v_b[k-1]) 2^-16), zero outside the
+  // rows; the CTAS words hold the two 16-bit samples (v_a[k], v_b[k])
+  static constexpr bool LIN = ST == ST_F32_LIN;
+  using WordT = typename std::conditional<LIN, float4, uint32_t>::type;
+  static constexpr size_t WIN = sizeof(WordT) * AC * L2;
   static constexpr size_t HALFBUF = WIN + sizeof(int) * AC * TAB;
   static constexpr size_t ANGS = 2 * HALFBUF;
   static constexpr size_t BYTES = ANGS + 2 * sizeof(SlotConsts) * AC;
   static_assert(TR % 4 == 0 && TC % 4 == 0, "16-byte table reads");
   static_assert(AC <= 64, "one lane of the deriving wave per slot");
-  u16x2 v[EPT];  // the two samples of word j
+  typename std::conditional<LIN, float4, u16x2>::type v[EPT];  // the samples of word j
   int t[TPT];    // SY (rows) and CX (columns) entries of this thread
   SlotConsts k;  // constants of this thread's slot, pass being fetched
   MirrorSlot raw;  // deriving wave: the slot of its lane, pass after next
@@ -767,8 +772,10 @@ struct StageMirror {
     const int w = it.w, hp = w / 2;
     const double cd = static_cast<double>(it.center), s = raw.s, co = raw.c;
     const double rowterm = cd + hp * ((1.0 - s) - co) + s * (ty0 - cd);
-    const double TA = rowterm + co * tx0;             // pixel (ty0, tx0)
-    const double TB = rowterm + co * (2 * hp - tx0);  // its mirror
+    // LINEAR interpolates between bin centres: word floor(t + 1/2)
+    const double half = LIN ? 0.5 : 0.0;
+    const double TA = rowterm + co * tx0 + half;             // pixel (ty0, tx0)
+    const double TB = rowterm + co * (2 * hp - tx0) + half;  // its mirror
     const double hy = s * (0.5 * (RT - 1)), hx = co * (0.5 * (CT - 1));
     SlotConsts c;
     c.loA = static_cast<int>(floor(TA + hy + hx)) - LO;
@@ -779,7 +786,7 @@ struct StageMirror {
     c.rb = raw.row_b >= 0 ? it.pad + raw.row_b * w : -1;
     // the row and column terms are whole pixels: the base's fraction is
     // that of every coordinate of the window
-    c.exact = raw.integral ? ((c.ba & 0xFFFF) == 0 ? 1 : 0) | ((c.bb & 0xFFFF) == 0 ? 2 : 0) : 0;
+    c.exact = (!LIN && raw.integral) ? ((c.ba & 0xFFFF) == 0 ? 1 : 0) | ((c.bb & 0xFFFF) == 0 ? 2 : 0) : 0;
     c.unused = 0;
     if (l < AC) reinterpret_cast<SlotConsts *>(lds + ANGS)[par * AC + l] = c;
   }
@@ -796,26 +803,52 @@ struct StageMirror {
     for (int j = 0; j < TPT; ++j) t[j] = ldg32(st, static_cast<unsigned>(min(sub + j * TPA, RT + CT - 1)));
     // offsets from the start of the zero pad before the sinogram; a slot
     // without a partner reads its second samples from the pad's first element
-    const uint16_t *__restrict__ base16 = static_cast<const uint16_t *>(it.filt) - it.pad;
-    const int mb = ~(k.rb >> 31);
+    if constexpr (LIN) {
+      // FP32 rows without pads: samples k-1 and k of both rows, indices
+      // clamped into the row and the samples outside it zeroed (LINEAR reads
+      // nothing beyond the detector); a slot without a partner reads row A
+      // again for its zeroed second samples
+      const float *__restrict__ f32 = static_cast<const float *>(it.filt);
+      const int w = it.w;
+      const bool hb = k.rb >= 0;
+      const int rb = hb ? k.rb : k.ra;
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int e = sub + j * TPA;  // word e of the slot: [0, L) window A, [L, 2L) window B
-      const int kk = e < L ? window_pos(k.loA + e, k.exact & 1) : window_pos(k.loB + (e - L), k.exact & 2);
-      v[j].x = ldg32(base16, static_cast<unsigned>(k.ra + kk));
-      v[j].y = ldg32(base16, static_cast<unsigned>((k.rb + kk) & mb));
+      for (int j = 0; j < EPT; ++j) {
+        const int e = sub + j * TPA;
+        const int kk = e < L ? k.loA + e : k.loB + (e - L);
+        const bool in1 = kk >= 0 && kk < w, in0 = kk >= 1 && kk <= w;
+        const int i1 = in1 ? kk : 0, i0 = in0 ? kk - 1 : 0;
+        const float a0 = ldg(f32, k.ra + i0), a1 = ldg(f32, k.ra + i1);
+        const float b0 = ldg(f32, rb + i0), b1 = ldg(f32, rb + i1);
+        v[j] = make_float4(in0 ? a0 : 0.0f, in1 ? a1 : 0.0f, (hb && in0) ? b0 : 0.0f, (hb && in1) ? b1 : 0.0f);
+      }
+    } else {
+      const uint16_t *__restrict__ base16 = static_cast<const uint16_t *>(it.filt) - it.pad;
+      const int mb = ~(k.rb >> 31);
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        const int e = sub + j * TPA;  // word e of the slot: [0, L) window A, [L, 2L) window B
+        const int kk = e < L ? window_pos(k.loA + e, k.exact & 1) : window_pos(k.loB + (e - L), k.exact & 2);
+        v[j].x = ldg32(base16, static_cast<unsigned>(k.ra + kk));
+        v[j].y = ldg32(base16, static_cast<unsigned>((k.rb + kk) & mb));
+      }
     }
   }
 
   __device__ __forceinline__ void store(unsigned char *buf, int na) const {
     const int tid = threadIdx.x, a = tid / TPA, sub = tid - a * TPA;
     if (a >= na) return;
-    uint32_t *win = reinterpret_cast<uint32_t *>(buf);
+    WordT *win = reinterpret_cast<WordT *>(buf);
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
       const int e = sub + j * TPA;
       if (e >= L2) continue;
-      if constexpr (ST == ST_BF16_H) {
+      if constexpr (LIN) {
+        // the differences scaled by 2^-16 (exact), so that the 16-bit
+        // fraction of a coordinate weighs them as an integer
+        const float4 q = v[j];
+        win[a * L2 + e] = make_float4(q.x, (q.y - q.x) * 0x1p-16f, q.z, (q.w - q.z) * 0x1p-16f);
+      } else if constexpr (ST == ST_BF16_H) {
         // BF16 -> FP32 (exact) -> FP16, rounded to nearest even
         const half2v h = {static_cast<_Float16>(__uint_as_float(static_cast<uint32_t>(v[j].x) << 16)),
                           static_cast<_Float16>(__uint_as_float(static_cast<uint32_t>(v[j].y) << 16))};
@@ -825,7 +858,7 @@ struct StageMirror {
       }
     }
     // word addresses are taken from the start of the LDS block
-    const int off = static_cast<int>((buf - base) / sizeof(uint32_t)) + a * L2;
+    const int off = static_cast<int>((buf - base) / sizeof(WordT)) + a * L2;
     const int ba = k.ba + (off << 16), bb = k.bb + ((off + L) << 16);
     int *tab = reinterpret_cast<int *>(buf + WIN) + a * TAB;
 #pragma unroll
@@ -945,7 +978,38 @@ __device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigne
         WB[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ub >> 16) << 2));
       }
     };
-    if constexpr (ST == ST_F16 || ST == ST_BF16_H) {
+    if constexpr (ST == ST_F32_LIN) {
+      // LINEAR: the word of window A at t_a(p) and of window B at t_a(p')
+      // carry both angles' samples and differences; with their 16-bit
+      // fractions fa, fb: p += a(t_a(p)) + b(t_a(p')), p' += a(t_a(p')) +
+      // b(t_a(p)), two coordinates and two 16-byte reads for four updates
+#pragma unroll 2
+      for (int a = 0; a < na; ++a) {
+        int A[TR], B[TR], Cq[TC];
+        terms(a, A, B, Cq);
+#pragma unroll
+        for (int r = 0; r < TR; ++r) {
+          float4 WA[TC], WB[TC];
+          float FA[TC], FB[TC];
+#pragma unroll
+          for (int q = 0; q < TC; ++q) {
+            const unsigned ua = static_cast<unsigned>(A[r] + Cq[q]);
+            const unsigned ub = static_cast<unsigned>(B[r] - Cq[q]);
+            WA[q] = *reinterpret_cast<const float4 *>(lds0 + ((ua >> 16) << 4));
+            WB[q] = *reinterpret_cast<const float4 *>(lds0 + ((ub >> 16) << 4));
+            FA[q] = static_cast<float>(ua & 0xFFFFu);
+            FB[q] = static_cast<float>(ub & 0xFFFFu);
+          }
+#pragma unroll
+          for (int q = 0; q < TC; ++q) {
+            acc[r][q] = fmaf(FA[q], WA[q].y, acc[r][q] + WA[q].x);
+            acc[r][q] = fmaf(FB[q], WB[q].w, acc[r][q] + WB[q].z);
+            accm[r][q] = fmaf(FB[q], WB[q].y, accm[r][q] + WB[q].x);
+            accm[r][q] = fmaf(FA[q], WA[q].w, accm[r][q] + WA[q].z);
+          }
+        }
+      }
+    } else if constexpr (ST == ST_F16 || ST == ST_BF16_H) {
       // FP16: one packed add forms both pixels' pair sums of a slot, (lo WA
       // + hi WB, hi WA + lo WB) (the swap is the add's operand select); the
       // sums of two slots are added packed, each step rounded once to FP16,
@@ -1208,6 +1272,14 @@ hipError_t launch_k(const Item *d_items, int n_items, int total_tiles, int ac, i
 hipError_t launch_backproject(const Item *d_items, int n_items, int total_tiles, int kind,
                               int angles_per_pass, int mode, int store, bool mirror, hipStream_t s) {
   if (total_tiles <= 0) return hipSuccess;
+  if (mirror && mode == MODE_LINEAR) {
+    // FP32 storage, 64x64 tiles: the 16-byte words of both windows fill the
+    // 64 KiB of static LDS
+    if (store != ST_F32 || kind != TK_64x64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((bp_mirror_kernel<TK_64x64, 4, ST_F32_LIN>), dim3(total_tiles), dim3(256), 0, s, d_items,
+                       n_items);
+    return hipGetLastError();
+  }
   if (mirror && (mode != MODE_CTAS || store == ST_F32)) return hipErrorInvalidValue;
   switch (kind) {
     case TK_64x64: return launch_k<TK_64x64>(d_items, n_items, total_tiles, angles_per_pass, mode, store, mirror, s);

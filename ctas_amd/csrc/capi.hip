@@ -178,6 +178,14 @@ int filt_pad_of(const ctas_fbp *h) { return store_of(h->flags) == ST_F32 ? 0 : f
 // least half of the angles pair up as theta and pi - theta.
 bool use_mirror(const ctas_fbp *h) { return mirror_applies(h->flags, h->N, h->w, h->n_paired); }
 
+// LINEAR sampling of FP32 storage takes the mirrored path too, on 64x64 tiles
+// (the 16-byte words of its two windows fill the static LDS of a workgroup).
+bool use_mirror(const ctas_fbp *h, int kind) {
+  if (use_mirror(h)) return true;
+  return mode_of(h->flags) == MODE_LINEAR && store_of(h->flags) == ST_F32 && kind == TK_64x64 &&
+         4 * h->n_paired >= h->N;
+}
+
 Item base_item(const ctas_fbp *h, int kind, bool mirror) {
   Item it{};
   it.sincos = h->d_sincos;
@@ -237,7 +245,7 @@ int enqueue_slices(ctas_fbp *h, const float *d_sino, float *d_out, float *d_f32,
   if (int rc = grow(h->d_filt, h->filt_cap, stride * S * store_bytes(st))) return rc;
   if (int rc = ensure_slices(h, S)) return rc;
   const int kind = choose_kind(h);
-  const bool mirror = use_mirror(h);
+  const bool mirror = use_mirror(h, kind);
   const int pairs = (h->N + 1) / 2;
   std::vector<Item> items(S);
   for (int i = 0; i < S; ++i) {
@@ -277,7 +285,7 @@ int enqueue_repeat(ctas_fbp *h, int slice, float *d_out, const float *centers, i
     if (int rc = check_center(h->w, centers[i])) return rc;
   const int st = store_of(h->flags);
   const int kind = choose_kind(h);
-  const bool mirror = use_mirror(h);
+  const bool mirror = use_mirror(h, kind);
   const size_t slice_elems = static_cast<size_t>(h->N) * h->w;
   const int pad = filt_pad_of(h);
   const size_t stride = slice_elems + 2 * static_cast<size_t>(pad);

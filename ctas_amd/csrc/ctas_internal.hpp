@@ -136,7 +136,10 @@ constexpr int kMinMaxStride = kMinMaxBlocks + 1;  // float2 entries per item
 enum Store { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2,
              // the mirrored backprojection of BF16 storage staging its samples
              // as FP16 and summing them as the FP16 path does (see bp_store)
-             ST_BF16_H = 3 };
+             ST_BF16_H = 3,
+             // LINEAR sampling of FP32 storage through mirrored angle pairs
+             // (the mirrored backprojection launch only)
+             ST_F32_LIN = 4 };
 
 // Storage code of the backprojection launch. BF16 samples convert to FP16
 // exactly above 2^-14 in magnitude (8 significant bits into 11) and below
