@@ -83,8 +83,8 @@ def test_random_precision_mode(k):
 
     Bars. LINEAR: 2e-3 relative RMS from 16-bit storage; from FP32 storage
     1e-4 scaled by max(1, (w + 2|c|) / 1024), as the float ray coordinates
-    reach w + 2|c| and carry that magnitude's rounding into the
-    interpolation. CTAS sampling: truncation moves a sample to the
+    of the unpaired path reach w + 2|c| and carry that magnitude's rounding
+    into the interpolation (the mirrored path's 16.16 table sums are finer). CTAS sampling: truncation moves a sample to the
     neighbouring bin wherever a coordinate error straddles an integer, so
     with few angles, wide rows and noisy filtered rows a slice can stand far
     from FP64 whatever its storage: the reference's own float path
