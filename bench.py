@@ -590,10 +590,11 @@ def main() -> int:
                                          "binding": binding_of(vp), "valu": vp.get("valu"),
                                          "lds_issue": vp.get("lds_issue"),
                                          "kernel": "bp_mirror_kernel", "bytes_per_update": 2}
-        print("[bench] fp32 LINEAR", file=sys.stderr, flush=True)
-        _, lms, lbp, _ = run_variant(ca.STORE_FP32, ca.INTERP_LINEAR, max(2, args.steps // 2), 1)
-        var["fp32_linear"] = {"value": round(updates / (lms / 1e3) / 1e9, 2), "ms_per_step": round(lms, 3),
-                              "bp_kernel_ms": round(lbp, 3)}
+        for name, store in (("fp32_linear", ca.STORE_FP32), ("fp16_linear", ca.STORE_FP16)):
+            print(f"[bench] {name}", file=sys.stderr, flush=True)
+            _, lms, lbp, _ = run_variant(store, ca.INTERP_LINEAR, max(2, args.steps // 2), 1)
+            var[name] = {"value": round(updates / (lms / 1e3) / 1e9, 2), "ms_per_step": round(lms, 3),
+                         "bp_kernel_ms": round(lbp, 3)}
         result["variants"] = var
     host = host_cpu_info()
     if not args.no_axis:

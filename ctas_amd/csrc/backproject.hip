@@ -736,10 +736,11 @@ struct StageMirror {
   static constexpr int RT = G::RT, CT = G::CT, TR = G::TR, TC = G::TC;
   static constexpr int TAB = 2 * RT + CT;  // ints per slot: RA, RB, CX
   static constexpr int TPT = (RT + CT + TPA - 1) / TPA;  // SY and CX entries per thread
-  // LINEAR (FP32 storage): word k of a window holds (v_a[k-1], (v_a[k] -
-  // v_a[k-1]) 2^-16, v_b[k-1], (v_b[k] - v_b[k-1]) 2^-16), zero outside the
-  // rows; the CTAS words hold the two 16-bit samples (v_a[k], v_b[k])
-  static constexpr bool LIN = ST == ST_F32_LIN;
+  // LINEAR (any storage, samples as float): word k of a window holds
+  // (v_a[k-1], (v_a[k] - v_a[k-1]) 2^-16, v_b[k-1], (v_b[k] - v_b[k-1])
+  // 2^-16), zero outside the rows; the CTAS words hold the two 16-bit
+  // samples (v_a[k], v_b[k])
+  static constexpr bool LIN = ST == ST_F32_LIN || ST == ST_F16_LIN || ST == ST_BF16_LIN;
   using WordT = typename std::conditional<LIN, float4, uint32_t>::type;
   static constexpr size_t WIN = sizeof(WordT) * AC * L2;
   static constexpr size_t HALFBUF = WIN + sizeof(int) * AC * TAB;
@@ -804,22 +805,30 @@ struct StageMirror {
     // offsets from the start of the zero pad before the sinogram; a slot
     // without a partner reads its second samples from the pad's first element
     if constexpr (LIN) {
-      // FP32 rows without pads: samples k-1 and k of both rows, indices
-      // clamped into the row and the samples outside it zeroed (LINEAR reads
-      // nothing beyond the detector); a slot without a partner reads row A
-      // again for its zeroed second samples
-      const float *__restrict__ f32 = static_cast<const float *>(it.filt);
+      // samples k-1 and k of both rows, indices clamped into the row and the
+      // samples outside it zeroed (LINEAR reads nothing beyond the detector);
+      // a slot without a partner reads row A again for its zeroed second
+      // samples. Row offsets count from the allocation's start (the 16-bit
+      // sinograms' zero pad before them included).
       const int w = it.w;
       const bool hb = k.rb >= 0;
       const int rb = hb ? k.rb : k.ra;
+      auto sample = [&](int i) -> float {
+        if constexpr (ST == ST_F32_LIN) {
+          return ldg(static_cast<const float *>(it.filt), i);
+        } else {
+          const uint16_t *__restrict__ b16 = static_cast<const uint16_t *>(it.filt) - it.pad;
+          return half_to_float<ST == ST_F16_LIN ? ST_F16 : ST_BF16>(ldg(b16, i));
+        }
+      };
 #pragma unroll
       for (int j = 0; j < EPT; ++j) {
         const int e = sub + j * TPA;
         const int kk = e < L ? k.loA + e : k.loB + (e - L);
         const bool in1 = kk >= 0 && kk < w, in0 = kk >= 1 && kk <= w;
         const int i1 = in1 ? kk : 0, i0 = in0 ? kk - 1 : 0;
-        const float a0 = ldg(f32, k.ra + i0), a1 = ldg(f32, k.ra + i1);
-        const float b0 = ldg(f32, rb + i0), b1 = ldg(f32, rb + i1);
+        const float a0 = sample(k.ra + i0), a1 = sample(k.ra + i1);
+        const float b0 = sample(rb + i0), b1 = sample(rb + i1);
         v[j] = make_float4(in0 ? a0 : 0.0f, in1 ? a1 : 0.0f, (hb && in0) ? b0 : 0.0f, (hb && in1) ? b1 : 0.0f);
       }
     } else {
@@ -978,7 +987,7 @@ __device__ __forceinline__ void bp_tile_mirror(const Item &it, int tile, unsigne
         WB[q] = *reinterpret_cast<const uint32_t *>(lds0 + ((ub >> 16) << 2));
       }
     };
-    if constexpr (ST == ST_F32_LIN) {
+    if constexpr (S::LIN) {
       // LINEAR: the word of window A at t_a(p) and of window B at t_a(p')
       // carry both angles' samples and differences; with their 16-bit
       // fractions fa, fb: p += a(t_a(p)) + b(t_a(p')), p' += a(t_a(p')) +
@@ -1273,11 +1282,14 @@ hipError_t launch_backproject(const Item *d_items, int n_items, int total_tiles,
                               int angles_per_pass, int mode, int store, bool mirror, hipStream_t s) {
   if (total_tiles <= 0) return hipSuccess;
   if (mirror && mode == MODE_LINEAR) {
-    // FP32 storage, 64x64 tiles: the 16-byte words of both windows fill the
-    // 64 KiB of static LDS
-    if (store != ST_F32 || kind != TK_64x64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((bp_mirror_kernel<TK_64x64, 4, ST_F32_LIN>), dim3(total_tiles), dim3(256), 0, s, d_items,
-                       n_items);
+    // 64x64 tiles: the 16-byte float words of both windows (16-bit samples
+    // converted when staged) fill the static LDS of a workgroup at 4 slots
+    if (kind != TK_64x64) return hipErrorInvalidValue;
+    switch (store) {
+      case ST_F32: hipLaunchKernelGGL((bp_mirror_kernel<TK_64x64, 4, ST_F32_LIN>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items); break;
+      case ST_F16: hipLaunchKernelGGL((bp_mirror_kernel<TK_64x64, 4, ST_F16_LIN>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items); break;
+      default: hipLaunchKernelGGL((bp_mirror_kernel<TK_64x64, 4, ST_BF16_LIN>), dim3(total_tiles), dim3(256), 0, s, d_items, n_items); break;
+    }
     return hipGetLastError();
   }
   if (mirror && (mode != MODE_CTAS || store == ST_F32)) return hipErrorInvalidValue;

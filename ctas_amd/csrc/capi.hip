@@ -178,12 +178,12 @@ int filt_pad_of(const ctas_fbp *h) { return store_of(h->flags) == ST_F32 ? 0 : f
 // least half of the angles pair up as theta and pi - theta.
 bool use_mirror(const ctas_fbp *h) { return mirror_applies(h->flags, h->N, h->w, h->n_paired); }
 
-// LINEAR sampling of FP32 storage takes the mirrored path too, on 64x64 tiles
-// (the 16-byte words of its two windows fill the static LDS of a workgroup).
+// LINEAR sampling takes the mirrored path too, on 64x64 tiles (the 16-byte
+// float words of its two windows fill the static LDS of a workgroup).
 bool use_mirror(const ctas_fbp *h, int kind) {
   if (use_mirror(h)) return true;
-  return mode_of(h->flags) == MODE_LINEAR && store_of(h->flags) == ST_F32 && kind == TK_64x64 &&
-         4 * h->n_paired >= h->N;
+  return mode_of(h->flags) == MODE_LINEAR && kind == TK_64x64 && 4 * h->n_paired >= h->N &&
+         static_cast<long long>(h->N) * h->w + 2LL * filt_pad_of(h) < (1LL << 31);
 }
 
 Item base_item(const ctas_fbp *h, int kind, bool mirror) {

@@ -137,9 +137,9 @@ enum Store { ST_F32 = 0, ST_F16 = 1, ST_BF16 = 2,
              // the mirrored backprojection of BF16 storage staging its samples
              // as FP16 and summing them as the FP16 path does (see bp_store)
              ST_BF16_H = 3,
-             // LINEAR sampling of FP32 storage through mirrored angle pairs
-             // (the mirrored backprojection launch only)
-             ST_F32_LIN = 4 };
+             // LINEAR sampling through mirrored angle pairs of FP32, FP16 and
+             // BF16 storage (the mirrored backprojection launch only)
+             ST_F32_LIN = 4, ST_F16_LIN = 5, ST_BF16_LIN = 6 };
 
 // Storage code of the backprojection launch. BF16 samples convert to FP16
 // exactly above 2^-14 in magnitude (8 significant bits into 11) and below

@@ -170,22 +170,25 @@ def test_width_4096_zidth_8192(contrast):
 
 
 @pytest.mark.parametrize("arc,c", [(180.0, 2.0), (360.0, -5.5), (360.0, -40.5), (200.0, -20.25), (360.0, 40.5)])
-@pytest.mark.parametrize("store", [ca.STORE_FP32, ca.STORE_FP16])
+@pytest.mark.parametrize("store", [ca.STORE_FP32, ca.STORE_FP16, ca.STORE_BF16])
 @pytest.mark.parametrize("kind", [ca.TILE_64x64, ca.TILE_128x128])
 def test_linear_mode_vs_f64(arc, c, store, kind):
     """LINEAR sampling against the FP64 oracle. Arcs past 180 degrees with
     c < 0 put in-mask pixels at ray coordinates below 0 (down to 2c), where
     the window element must be floor(t), the point fract(t) is taken from
-    (a truncated index read the element above, 2e-2 to 4e-2 off). FP32
+    (a truncated index read the element above, 2e-2 to 4e-2 off). Every
     storage on 64x64 tiles takes the mirrored angle pairs (every arc here
-    pairs its angles as theta, pi - theta), 128x128 tiles the float path."""
+    pairs its angles as theta, pi - theta; 16-bit samples are converted to
+    float when staged), 128x128 tiles the float path. Bars: FP32 1e-4, FP16
+    2e-3, BF16 (unit roundoff 2^-9 per stored sample) 4e-3."""
     N, w = 240, 256
     P = _phantom(N, w, c, 8, 0.0, seed=9)  # line integrals
     rec = ca.CTrec(w, N, ca.FLT, "RAMP", anarc=arc, interp=ca.INTERP_LINEAR, store=store)
     rec.set_tiles(kind, 0)
     _, out = rec.reconstruct(P, c)
     ref = oracle.bp_f64(P, c, arc * np.pi / 180.0, 1.0, linear=True)
-    assert rel_rms(out[0], ref) <= (1e-4 if store == ca.STORE_FP32 else 2e-3)
+    bar = {ca.STORE_FP32: 1e-4, ca.STORE_FP16: 2e-3, ca.STORE_BF16: 4e-3}[store]
+    assert rel_rms(out[0], ref) <= bar
 
 
 @pytest.mark.parametrize("store", [ca.STORE_FP16, ca.STORE_BF16])
