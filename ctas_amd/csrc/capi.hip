@@ -203,7 +203,8 @@ Item base_item(const ctas_fbp *h, int kind, bool mirror) {
   it.tiles_x = (cols + tile_cols(kind) - 1) / tile_cols(kind);
   it.tiles = it.tiles_x * ((h->w + tile_rows(kind) - 1) / tile_rows(kind));
   it.kind = kind;
-  it.half_max = mirror ? kHalfSumMax : kHalfMax;
+  // the mirrored CTAS kernel adds four FP16 samples; LINEAR stages floats
+  it.half_max = mirror && mode_of(h->flags) == MODE_CTAS ? kHalfSumMax : kHalfMax;
   if (mirror) {
     it.slots = h->d_slots;
     it.slot_tab = h->d_slot_tab;
@@ -315,7 +316,7 @@ int status_error(int code) {
   if (code == CTAS_E_DATA) return fail(CTAS_E_DATA, "Bad absorption data (maximum is sub-zero).");
   return fail(CTAS_E_RANGE, "a filtered sample exceeds the FP16 range of the backprojection (" +
                                 std::to_string(static_cast<int>(kHalfSumMax)) +
-                                " with mirrored angle pairs, else 65504): use BF16 or FP32 storage");
+                                " with mirrored angle pairs in CTAS sampling, else 65504): use BF16 or FP32 storage");
 }
 
 int read_status(ctas_fbp *h, int S) {

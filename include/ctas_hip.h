@@ -35,8 +35,8 @@ typedef struct ctas_fbp ctas_fbp;
 #define CTAS_E_AXIS -8     /* axis search failed (lct.cpp:1136-1210 throws)  */
 #define CTAS_E_IO -9       /* a reader or writer callback reported failure   */
 #define CTAS_E_RANGE -10   /* FP16 storage: a filtered sample beyond 65504, or
-                            * beyond 16376 where the mirrored kernel adds four
-                            * samples in FP16 (no reference counterpart: the
+                            * beyond 16376 where the mirrored CTAS kernel adds
+                            * four samples in FP16 (no reference counterpart: the
                             * reference has no FP16 storage)                 */
 
 /* contrast codes = Contrast::Cntype (physical.world.h:22-27) */

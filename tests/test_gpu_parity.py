@@ -423,9 +423,14 @@ def test_fp16_range_is_reported(contrast, torch_cuda):
     with pytest.raises(ca.CtasError) as e:
         ca.CTrec(w, N, contrast, "RAMP", store=ca.STORE_FP16).reconstruct(hi, 0.0)
     assert e.value.code == ca.CTAS_E_RANGE
-    for store, interp, Q in ((ca.STORE_BF16, ca.INTERP_CTAS, hi), (ca.STORE_FP16, ca.INTERP_LINEAR, hi),
-                             (ca.STORE_FP16, ca.INTERP_CTAS, lo)):
-        out = ca.CTrec(w, N, contrast, "RAMP", store=store, interp=interp).reconstruct(Q, 0.0)[1][0]
+    for store, interp, Q, kind in ((ca.STORE_BF16, ca.INTERP_CTAS, hi, None),
+                                   (ca.STORE_FP16, ca.INTERP_LINEAR, hi, ca.TILE_64x64),  # mirrored LINEAR
+                                   (ca.STORE_FP16, ca.INTERP_LINEAR, hi, ca.TILE_128x128),  # float path
+                                   (ca.STORE_FP16, ca.INTERP_CTAS, lo, None)):
+        vrec = ca.CTrec(w, N, contrast, "RAMP", store=store, interp=interp)
+        if kind is not None:
+            vrec.set_tiles(kind, 0)
+        out = vrec.reconstruct(Q, 0.0)[1][0]
         assert np.isfinite(out).all(), (store, interp)
         if Q is hi and interp == ca.INTERP_CTAS:
             assert rel_rms(out, ref) <= 5e-3
