@@ -1219,6 +1219,20 @@ hipError_t launch_persist_st(const Item *d_items, int n_items, int total_tiles, 
 template <int KIND>
 hipError_t launch_persist_k(const Item *d_items, int n_items, int total_tiles, int ac, int mode, int store, bool mirror,
                             unsigned *counter, int cus, hipStream_t s) {
+  if (mirror && mode == MODE_LINEAR) {
+    // 64x64 tiles only (the 16-byte float words of both windows at 4 slots)
+    if constexpr (KIND == TK_64x64) {
+      const int g = resident_workgroups(cus, KindGeo<KIND>::WAVES, StageMirror<KIND, 4, ST_F32_LIN>::BYTES);
+      switch (store) {
+        case ST_F32: hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 4, ST_F32_LIN>), dim3(g), dim3(256), 0, s, d_items, n_items, total_tiles, counter); break;
+        case ST_F16: hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 4, ST_F16_LIN>), dim3(g), dim3(256), 0, s, d_items, n_items, total_tiles, counter); break;
+        default: hipLaunchKernelGGL((bp_persistent_mirror_kernel<KIND, 4, ST_BF16_LIN>), dim3(g), dim3(256), 0, s, d_items, n_items, total_tiles, counter); break;
+      }
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (mirror) {
     const int g = resident_workgroups(cus, KindGeo<KIND>::WAVES, StageMirror<KIND, 8, ST_F16>::BYTES);
     if (store == ST_F16)
@@ -1304,7 +1318,8 @@ hipError_t launch_backproject_persistent(const Item *d_items, int n_items, int t
                                          int angles_per_pass, int mode, int store, bool mirror,
                                          unsigned *d_counter, int cus, hipStream_t s) {
   if (total_tiles <= 0) return hipSuccess;
-  if (mirror && (mode != MODE_CTAS || store == ST_F32)) return hipErrorInvalidValue;
+  // mirrored: CTAS sampling of 16-bit storage, or LINEAR on 64x64 tiles
+  if (mirror && (mode == MODE_CTAS ? store == ST_F32 : kind != TK_64x64)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, d_counter);
   switch (kind) {
     case TK_64x64:

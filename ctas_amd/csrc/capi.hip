@@ -180,10 +180,13 @@ bool use_mirror(const ctas_fbp *h) { return mirror_applies(h->flags, h->N, h->w,
 
 // LINEAR sampling takes the mirrored path too, on 64x64 tiles (the 16-byte
 // float words of its two windows fill the static LDS of a workgroup).
+bool mirror_linear_applies(int flags, int N, int w, int n_paired, int kind) {
+  const int pad = store_of(flags) == ST_F32 ? 0 : filt_pad(w);
+  return mode_of(flags) == MODE_LINEAR && kind == TK_64x64 && 4 * n_paired >= N &&
+         static_cast<long long>(N) * w + 2LL * pad < (1LL << 31);
+}
 bool use_mirror(const ctas_fbp *h, int kind) {
-  if (use_mirror(h)) return true;
-  return mode_of(h->flags) == MODE_LINEAR && kind == TK_64x64 && 4 * h->n_paired >= h->N &&
-         static_cast<long long>(h->N) * h->w + 2LL * filt_pad_of(h) < (1LL << 31);
+  return use_mirror(h) || mirror_linear_applies(h->flags, h->N, h->w, h->n_paired, kind);
 }
 
 Item base_item(const ctas_fbp *h, int kind, bool mirror) {
@@ -947,7 +950,8 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
     const auto &pk = pick[std::make_pair(width_class(bi.w), width_class(bi.n_angles))];
     e.kind = pk.first;
     e.ac = pk.second;
-    e.mirror = mirror_applies(flags, bi.n_angles, bi.w, e.g.n_paired);
+    e.mirror = mirror_applies(flags, bi.n_angles, bi.w, e.g.n_paired) ||
+               mirror_linear_applies(flags, bi.n_angles, bi.w, e.g.n_paired, e.kind);
     e.cost = static_cast<double>(bi.w) * bi.w * bi.n_angles;
   }
   // One persistent launch per (tile kind, pass length, mirrored) group, its
@@ -990,7 +994,7 @@ int ctas_fbp_batch_uneven(const ctas_batch_item *items, int n, int contrast, int
     const int cols = e.mirror ? bi.w / 2 + 1 : bi.w;
     it.tiles_x = (cols + tile_cols(e.kind) - 1) / tile_cols(e.kind);
     it.tiles = it.tiles_x * ((bi.w + tile_rows(e.kind) - 1) / tile_rows(e.kind));
-    it.half_max = e.mirror ? kHalfSumMax : kHalfMax;
+    it.half_max = e.mirror && mode == MODE_CTAS ? kHalfSumMax : kHalfMax;
     if (e.mirror) {
       it.slots = e.g.d_slots;
       it.slot_tab = e.g.d_slot_tab;
