@@ -4,6 +4,8 @@
   through the captured hipGraph with the autotuned tile, against the
   oracle's CTAS CPU path (prepare_sino + reconstructOnCPU, lct.cpp:569-597,
   688-729) on 32 slice rows per slice.
+* LINEAR sampling at the same shape through mirrored angle pairs, from
+  FP32, FP16 and BF16 storage, against the FP64 oracle's LINEAR rows.
 * Uneven batch: one sinogram of each width 512 .. 4096 with angle counts
   from {360, 900, 1440, 1800}, in one persistent launch, against the oracle.
 * The 81-centre sweep (-20 .. 20 px, step 0.5) at 2048 px x 1800 angles:
@@ -85,6 +87,35 @@ def test_2048x1800_graph_autotuned(torch_cuda):
     out = d_out.cpu().numpy()
     for s in range(S):
         assert np.array_equal(out[s][ROWS], d["ref"][s]), f"slice {s}: {(out[s][ROWS] != d['ref'][s]).sum()} pixels"
+
+
+@pytest.mark.parametrize("store", [ca.STORE_FP32, ca.STORE_FP16, ca.STORE_BF16])
+def test_2048x1800_linear_mirrored(torch_cuda, store):
+    """LINEAR sampling at 2048 x 1800 on 64x64 tiles, where every angle but 0
+    and 90 degrees pairs as theta, pi - theta and the mirrored kernel runs
+    (16-bit samples converted to float when staged): 32 rows of both slices
+    of the oracle's filtered sinograms against the FP64 oracle's LINEAR
+    rows. At 1800 angles the storage rounding of each sample averages out
+    (measured: FP32 1.2e-6, FP16 1.8e-5, BF16 1.4e-4 relative RMS), so the
+    bars sit near ten times that: FP32 1e-5, FP16 2e-4, BF16 1e-3; a
+    fraction weighed at the wrong scale or a window read one word off would
+    exceed them."""
+    torch = torch_cuda
+    d = _inputs_2048x1800()
+    N, w, c, S = d["N"], d["w"], d["c"], d["S"]
+    rec = ca.CTrec(w, N, ca.FLT, "HANN", interp=ca.INTERP_LINEAR, store=store)
+    rec.set_tiles(ca.TILE_64x64, 0)
+    d_f = torch.from_numpy(d["Pf"]).cuda()
+    d_out = torch.zeros((S, w, w), dtype=torch.float32, device="cuda")
+    rec.reconstruct_dev(d_f, d_out, [c] * S)
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy()
+    bar = {ca.STORE_FP32: 1e-5, ca.STORE_FP16: 2e-4, ca.STORE_BF16: 1e-3}[store]
+    for s in range(S):
+        ref = oracle.bp_f64_rows(d["Pf"][s], c, ROWS, linear=True)
+        err = rel_rms(out[s][ROWS], ref)
+        print(f"store {store:#x} slice {s}: rel_rms {err:.2e} (bar {bar:.1e})")
+        assert err <= bar
 
 
 def test_uneven_widths_batch_against_oracle(torch_cuda):
